@@ -1,0 +1,346 @@
+"""chanflow: MI355X-native pseudo-spectral channel-flow time step.
+
+Python host mirror of the C-ABI in ``include/chanflow.h`` (ctypes).  The
+numerical work happens in ``libchanflow.so`` (hand-written HIP kernels for
+gfx950); this module only moves arrays and calls entry points.  Importing it
+fails loudly when the library has not been built: there is no CPU fallback.
+
+Host arrays use the reference's Fortran layouts, exposed as C-ordered numpy
+arrays with the axes reversed:
+
+* ``data1`` complex(nz, nx/2+1, ny+1, 3)  <->  ndarray [3, ny+1, nx/2+1, nz]
+* ``data2`` complex(ny+1, 3, nz, nx/2+1)  <->  ndarray [nx/2+1, nz, 3, ny+1]
+* ``data4`` complex(ny+1, nz, nx/2+1)     <->  ndarray [nx/2+1, nz, ny+1]
+* physical planes real(nx2, nz2, np)      <->  ndarray [np, nz2, nx2]
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libchanflow.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class ChanParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nx2", C.c_int), ("nz2", C.c_int),
+                ("lx", C.c_double), ("lz", C.c_double), ("re", C.c_double), ("beta", C.c_double),
+                ("fix_mass", C.c_int), ("dt_flag", C.c_int), ("cfl_max", C.c_double),
+                ("ctrlu", C.c_double), ("ctrlw", C.c_double), ("bl_grid", C.c_int)]
+
+
+class ChanOperators(C.Structure):
+    _fields_ = [(n, _dp) for n in ("d2h", "d2n", "dm", "dh", "dn", "hn", "iu", "dyh", "dy", "kx", "kz",
+                                   "buffer", "target_inflow")]
+
+
+ENTRY_POINTS = (
+    "chan_create", "chan_destroy", "chan_last_error", "chan_version", "chan_upload_operators",
+    "chan_build_operators", "chan_set_state", "chan_get_state", "chan_get_planes", "chan_set_scalars",
+    "chan_get_scalars", "chan_set_wall_bc", "chan_nonlinear", "chan_nonlinear_linearized",
+    "chan_set_baseflow", "chan_rhs_solve", "chan_step", "chan_boundary", "chan_initialize_flow",
+    "chan_remove_divergence", "chan_compute_initial_pressure", "chan_xz_fft_complex_to_real",
+    "chan_xz_fft_real_to_complex", "chan_compute_divergence_of_fluxes", "chan_small_fft_complex_to_real",
+    "chan_small_fft_real_to_complex", "chan_solve_helmholtz", "chan_solve_p", "chan_strip_divergence",
+    "chan_linearized_matvec", "chan_divergence_max", "chan_mass_flux", "chan_set_timing",
+    "chan_get_kernel_times", "chan_sync",
+)
+
+_lib = None
+
+
+def load_library(path: str = LIBPATH):
+    """Load libchanflow.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"libchanflow.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "chan_create": (C.c_int, [C.POINTER(ChanParams), C.POINTER(vp)]),
+        "chan_destroy": (C.c_int, [vp]),
+        "chan_last_error": (C.c_char_p, [vp]),
+        "chan_version": (C.c_char_p, []),
+        "chan_upload_operators": (C.c_int, [vp, C.POINTER(ChanOperators)]),
+        "chan_build_operators": (C.c_int, [vp, C.c_double, _dp, _dp]),
+        "chan_set_state": (C.c_int, [vp, _dp, _dp, _dp]),
+        "chan_get_state": (C.c_int, [vp, _dp, _dp, _dp]),
+        "chan_get_planes": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
+        "chan_set_scalars": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "chan_get_scalars": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "chan_set_wall_bc": (C.c_int, [vp, _dp, _dp]),
+        "chan_nonlinear": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp]),
+        "chan_nonlinear_linearized": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp]),
+        "chan_set_baseflow": (C.c_int, [vp, _dp]),
+        "chan_rhs_solve": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
+        "chan_step": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "chan_boundary": (C.c_int, [vp]),
+        "chan_initialize_flow": (C.c_int, [vp, C.c_int]),
+        "chan_remove_divergence": (C.c_int, [vp]),
+        "chan_compute_initial_pressure": (C.c_int, [vp, C.c_int]),
+        "chan_xz_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
+        "chan_xz_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
+        "chan_compute_divergence_of_fluxes": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_int]),
+        "chan_small_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
+        "chan_small_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
+        "chan_solve_helmholtz": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp,
+                                           C.c_int, C.c_int]),
+        "chan_solve_p": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "chan_strip_divergence": (C.c_int, [vp, _dp, _dp, C.c_double]),
+        "chan_linearized_matvec": (C.c_int, [vp, _dp, _dp, C.c_int, _dp]),
+        "chan_divergence_max": (C.c_int, [vp, _dp]),
+        "chan_mass_flux": (C.c_int, [vp, _dp]),
+        "chan_set_timing": (C.c_int, [vp, C.c_int]),
+        "chan_get_kernel_times": (C.c_int, [vp, _dp, C.POINTER(C.c_long), C.c_int]),
+        "chan_sync": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class ChanError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+KERNEL_KINDS = ("z-inverse", "x-pass", "z-forward", "solve", "other")
+
+
+class Channel:
+    """One channel instance resident on the GPU (wraps a chan_ctx)."""
+
+    def __init__(self, nx, ny, nz, nx2, nz2, lx, lz, re, beta=0.0, fix_mass=1, dt_flag=0, cfl_max=0.95,
+                 ctrlu=0.0, ctrlw=0.0):
+        self.lib = load_library()
+        self.nx, self.ny, self.nz, self.nx2, self.nz2 = nx, ny, nz, nx2, nz2
+        self.nxh = nx // 2 + 1
+        self.nyp = ny + 1
+        p = ChanParams(nx, ny, nz, nx2, nz2, lx, lz, re, beta, fix_mass, dt_flag, cfl_max, ctrlu, ctrlw, 0)
+        h = C.c_void_p()
+        if self.lib.chan_create(C.byref(p), C.byref(h)) != 0:
+            raise ChanError(self.lib.chan_last_error(None).decode())
+        self.h = h
+        self._keep = []
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.chan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise ChanError(self.lib.chan_last_error(self.h).decode())
+
+    # -- operators
+    def upload_operators(self, d2h, d2n, dm, dh, dn, hn, iu, dyh, dy, kx, kz, buffer=None, target_inflow=None):
+        """Arrays in numpy form [i, j] of the Fortran (i, j) arrays: d2h (3, ny+1),
+        d2n (3, ny), dm (3, 4), iu (2, ny), target_inflow (ny+1, 3); buffer [nz2, nx2]."""
+        arrs = dict(d2h=_f64(np.asarray(d2h).T), d2n=_f64(np.asarray(d2n).T), dm=_f64(np.asarray(dm).T),
+                    dh=_f64(dh), dn=_f64(dn), hn=_f64(hn), iu=_f64(np.asarray(iu).T), dyh=_f64(dyh), dy=_f64(dy),
+                    kx=_f64(kx), kz=_f64(kz),
+                    buffer=None if buffer is None else _f64(buffer),
+                    target_inflow=None if target_inflow is None else _f64(np.asarray(target_inflow).T))
+        o = ChanOperators(**{k: _ptr(v) for k, v in arrs.items()})
+        self._chk(self.lib.chan_upload_operators(self.h, C.byref(o)))
+
+    def upload_grid(self, g):
+        """Upload the operators of an oracle-style grid object (same attribute names)."""
+        self.upload_operators(g.d2h, g.d2n, g.dm, g.dh, g.dn, g.hn, g.iu, g.dyh, g.dy, g.kx, g.kz,
+                              getattr(g, "buffer", None), getattr(g, "target_inflow", None))
+
+    def build_operators(self, stretch=2.75):
+        y = np.empty(self.ny)
+        yh = np.empty(self.ny + 1)
+        self._chk(self.lib.chan_build_operators(self.h, stretch, _ptr(y), _ptr(yh)))
+        return y, yh
+
+    # -- state
+    def set_state(self, data1=None, data2=None, data4=None):
+        a1 = None if data1 is None else _c128(data1)
+        a2 = None if data2 is None else _c128(data2)
+        a4 = None if data4 is None else _c128(data4)
+        self._chk(self.lib.chan_set_state(self.h, _ptr(a1), _ptr(a2), _ptr(a4)))
+
+    def get_state(self):
+        d1 = np.empty((3, self.nyp, self.nxh, self.nz), np.complex128)
+        d2 = np.empty((self.nxh, self.nz, 3, self.nyp), np.complex128)
+        d4 = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
+        self._chk(self.lib.chan_get_state(self.h, _ptr(d1), _ptr(d2), _ptr(d4)))
+        return d1, d2, d4
+
+    def get_data1(self):
+        d1 = np.empty((3, self.nyp, self.nxh, self.nz), np.complex128)
+        self._chk(self.lib.chan_get_state(self.h, _ptr(d1), None, None))
+        return d1
+
+    def set_scalars(self, dt=None, t_total=None, p_grad=None, d_um_dy=None):
+        def box(v):
+            return None if v is None else C.byref(C.c_double(v))
+        dud = None if d_um_dy is None else _f64(d_um_dy)
+        args = [box(dt), box(t_total), box(p_grad)]
+        self._chk(self.lib.chan_set_scalars(self.h, *[a if a is not None else None for a in args], _ptr(dud)))
+
+    def get_scalars(self):
+        dt, tt, pg = C.c_double(), C.c_double(), C.c_double()
+        dud = np.empty(self.ny)
+        self._chk(self.lib.chan_get_scalars(self.h, C.byref(dt), C.byref(tt), C.byref(pg), _ptr(dud)))
+        return dict(dt=dt.value, t_total=tt.value, p_grad=pg.value, d_um_dy=dud)
+
+    def set_wall_bc(self, cbc_top=None, cbc_bot=None):
+        t = None if cbc_top is None else _c128(cbc_top)
+        b = None if cbc_bot is None else _c128(cbc_bot)
+        self._chk(self.lib.chan_set_wall_bc(self.h, _ptr(t), _ptr(b)))
+
+    def set_baseflow(self, base):
+        """base: physical base flow [3, ny+1, nz2, nx2] (Fortran baseflow(nx2, nz2, ny+1, 3))."""
+        self._chk(self.lib.chan_set_baseflow(self.h, _ptr(_f64(base))))
+
+    # -- step phases
+    def nonlinear(self, step, rk_step, dt=None, linearized=False):
+        d = C.c_double(0.0 if dt is None else dt)
+        cfl = C.c_double(0.0)
+        f = self.lib.chan_nonlinear_linearized if linearized else self.lib.chan_nonlinear
+        self._chk(f(self.h, step, rk_step, C.byref(d) if dt is not None else None, C.byref(cfl)))
+        return (d.value if dt is not None else None), cfl.value
+
+    def rhs_solve(self, step, rk_step):
+        pg = C.c_double(0.0)
+        self._chk(self.lib.chan_rhs_solve(self.h, step, rk_step, C.byref(pg)))
+        return pg.value
+
+    def step(self, nsteps=1, step0=1, linearized=False):
+        self._chk(self.lib.chan_step(self.h, step0, nsteps, 1 if linearized else 0))
+
+    def boundary(self):
+        self._chk(self.lib.chan_boundary(self.h))
+
+    def initialize_flow(self, linearized=False):
+        self._chk(self.lib.chan_initialize_flow(self.h, 1 if linearized else 0))
+
+    def remove_divergence(self):
+        self._chk(self.lib.chan_remove_divergence(self.h))
+
+    def compute_initial_pressure(self, linearized=False):
+        self._chk(self.lib.chan_compute_initial_pressure(self.h, 1 if linearized else 0))
+
+    # -- transforms (host arrays)
+    def xz_c2r(self, cdata):
+        cdata = _c128(cdata)
+        npl = cdata.size // (self.nxh * self.nz)
+        out = np.empty((npl, self.nz2, self.nx2))
+        self._chk(self.lib.chan_xz_fft_complex_to_real(self.h, _ptr(cdata), _ptr(out), npl))
+        return out
+
+    def xz_r2c(self, data):
+        data = _f64(data)
+        npl = data.size // (self.nx2 * self.nz2)
+        out = np.empty((npl, self.nxh, self.nz), np.complex128)
+        self._chk(self.lib.chan_xz_fft_real_to_complex(self.h, _ptr(data), _ptr(out), npl))
+        return out
+
+    def div_fluxes(self, fx, fy, fz):
+        fx, fy, fz = _f64(fx), _f64(fy), _f64(fz)
+        npl = fx.size // (self.nx2 * self.nz2)
+        out = np.empty((npl, self.nxh, self.nz), np.complex128)
+        self._chk(self.lib.chan_compute_divergence_of_fluxes(self.h, _ptr(fx), _ptr(fy), _ptr(fz), _ptr(out), npl))
+        return out
+
+    def small_c2r(self, cdata):
+        cdata = _c128(cdata)
+        npl = cdata.size // (self.nxh * self.nz)
+        out = np.empty((npl, self.nz, self.nx))
+        self._chk(self.lib.chan_small_fft_complex_to_real(self.h, _ptr(cdata), _ptr(out), npl))
+        return out
+
+    def small_r2c(self, data):
+        data = _f64(data)
+        npl = data.size // (self.nx * self.nz)
+        out = np.empty((npl, self.nxh, self.nz), np.complex128)
+        self._chk(self.lib.chan_small_fft_real_to_complex(self.h, _ptr(data), _ptr(out), npl))
+        return out
+
+    # -- solvers (host arrays)
+    def solve_helmholtz(self, f, alpha, diff, d2, topbc, botbc):
+        """f [nsys, n] complex (Fortran f(n, nsys)); alpha [nsys]; d2 [3, n] numpy form."""
+        f = _c128(f).copy()
+        nsys, n = f.shape
+        self._chk(self.lib.chan_solve_helmholtz(self.h, n, nsys, _ptr(f), _ptr(_f64(alpha)), diff,
+                                                _ptr(_f64(np.asarray(d2).T)), _ptr(_c128(topbc)),
+                                                _ptr(_c128(botbc)), 0, 0))
+        return f
+
+    def solve_p(self, p, alpha):
+        p = _c128(p).copy()
+        nsys = p.shape[0]
+        self._chk(self.lib.chan_solve_p(self.h, nsys, _ptr(p), _ptr(_f64(alpha))))
+        return p
+
+    def strip_divergence(self, cu, coef):
+        cu = _c128(cu).copy()
+        cp = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
+        self._chk(self.lib.chan_strip_divergence(self.h, _ptr(cu), _ptr(cp), coef))
+        return cu, cp
+
+    def linearized_matvec(self, v, tend):
+        v = _f64(v)
+        w = np.empty_like(v)
+        t = C.c_double()
+        self._chk(self.lib.chan_linearized_matvec(self.h, _ptr(v), _ptr(w), tend, C.byref(t)))
+        return w, t.value
+
+    # -- diagnostics
+    def divergence_max(self):
+        d = C.c_double()
+        self._chk(self.lib.chan_divergence_max(self.h, C.byref(d)))
+        return d.value
+
+    def mass_flux(self):
+        d = C.c_double()
+        self._chk(self.lib.chan_mass_flux(self.h, C.byref(d)))
+        return d.value
+
+    def set_timing(self, on=True):
+        self._chk(self.lib.chan_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = (C.c_double * 5)()
+        cnt = (C.c_long * 5)()
+        self._chk(self.lib.chan_get_kernel_times(self.h, ms, cnt, 5))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    def sync(self):
+        self._chk(self.lib.chan_sync(self.h))
+
+
+def version() -> str:
+    return load_library().chan_version().decode()

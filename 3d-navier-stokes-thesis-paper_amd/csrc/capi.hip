@@ -1,0 +1,912 @@
+// C-ABI of libchanflow: context, state transfer and the drop-in entry points
+// for the transform / pass1 / pass2 / solve / pass1_linearized / arnoldi
+// interfaces of the reference (each entry cites the interface it replaces in
+// include/chanflow.h).  Every compute entry runs on the GPU; there is no host
+// fallback.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <algorithm>
+#include "chan_internal.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+inline int fail(chan_ctx* c, const std::string& m) {
+  if (c) c->last_error = m;
+  else g_create_error = m;
+  return 1;
+}
+
+template <class F>
+int guarded(chan_ctx* c, F&& f) {
+  if (!c) return fail(nullptr, "null context");
+  try {
+    f();
+    return 0;
+  } catch (const chan_error& e) {
+    return fail(c, e.msg);
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+}
+
+struct KTimer {
+  chan_ctx* c;
+  int kind;
+  hipEvent_t a{}, b{};
+  KTimer(chan_ctx* c_, int k) : c(c_), kind(k) {
+    if (c->timing) {
+      CHAN_CHECK(hipEventCreate(&a));
+      CHAN_CHECK(hipEventCreate(&b));
+      CHAN_CHECK(hipEventRecord(a, c->stream));
+    }
+  }
+  ~KTimer() {
+    if (c->timing) {
+      (void)hipEventRecord(b, c->stream);
+      c->ev.push_back(a);
+      c->ev.push_back(b);
+      c->ev_kind.push_back(kind);
+    }
+  }
+};
+
+bool smooth23(int n) {
+  if (n < 2) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  return n == 1;
+}
+
+int roundup(int a, int m) { return ((a + m - 1) / m) * m; }
+
+void* dalloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 8;
+  CHAN_CHECK(hipMalloc(&p, bytes));
+  return p;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+void* stage(chan_ctx* c, size_t bytes) {
+  if (bytes > c->stage_bytes) {
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    c->d_stage = dalloc(bytes);
+    c->stage_bytes = bytes;
+  }
+  return c->d_stage;
+}
+
+void h2d(chan_ctx* c, void* d, const void* h, size_t bytes) {
+  CHAN_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+}
+void d2h(chan_ctx* c, void* h, const void* d, size_t bytes) {
+  CHAN_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  CHAN_CHECK(hipStreamSynchronize(c->stream));
+}
+void d2d(chan_ctx* c, void* d, const void* s, size_t bytes) {
+  CHAN_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream));
+}
+
+void check_ops(chan_ctx* c) {
+  if (!c->d_ops) throw chan_error("operators not uploaded (chan_upload_operators / chan_build_operators)");
+}
+
+long nmodes(chan_ctx* c) { return (long)c->nxh * c->prm.nz; }
+size_t vel_bytes(chan_ctx* c) { return sizeof(double2) * 3 * c->nyp * nmodes(c); }
+size_t p_bytes(chan_ctx* c) { return sizeof(double2) * c->nyp * nmodes(c); }
+
+// -------- device pipelines --------
+void dev_c2r(chan_ctx* c, const XZGeom& g, const double2* in, int np, double* out) {
+  const long in_ps = (long)g.nxh * g.nz, zps = (long)g.nz2 * g.kxpad, ops_ = (long)g.nz2 * g.nx2;
+  launch_zinv(g, c->d_tw, in, in_ps, c->d_Z, zps, np, c->stream);
+  launch_xc2r(g, c->d_tw, c->d_Z, zps, out, ops_, np, c->stream);
+}
+
+void dev_r2c(chan_ctx* c, const XZGeom& g, const double* in, int np, double2* out) {
+  const long ips = (long)g.nz2 * g.nx2, aps = (long)g.nz2 * g.kxpad, ops_ = (long)g.nxh * g.nz;
+  launch_xr2c(g, c->d_tw, in, nullptr, c->ops.kx, ips, c->d_AB, aps, np, c->stream);
+  launch_zfwd(g, c->d_tw, c->d_AB, nullptr, aps, c->ops.kz, out, ops_, np, 0, c->stream);
+}
+
+void dev_div(chan_ctx* c, const double* fx, const double* fy, const double* fz, int np, double2* out) {
+  const XZGeom& g = c->gd;
+  const long ips = (long)g.nz2 * g.nx2, aps = (long)g.nz2 * g.kxpad, ops_ = (long)g.nxh * g.nz;
+  double2* A = c->d_AB;
+  double2* B = c->d_AB + (long)np * aps;
+  launch_xr2c(g, c->d_tw, fy, fx, c->ops.kx, ips, A, aps, np, c->stream);
+  launch_xr2c(g, c->d_tw, fz, nullptr, c->ops.kx, ips, B, aps, np, c->stream);
+  launch_zfwd(g, c->d_tw, A, B, aps, c->ops.kz, out, ops_, np, 0, c->stream);
+}
+
+// nonlinear / nonlinear_linearized: d_vel -> d_nl
+void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
+  const XZGeom& g = c->gd;
+  const int nyp = c->nyp, ny = c->prm.ny;
+  const long zps = (long)g.nz2 * g.kxpad;
+  if (lin && !c->d_base) throw chan_error("base flow not set (chan_set_baseflow)");
+  if (cfl) launch_fill_zero(c->d_cfl, sizeof(unsigned long long) * 3 * nyp, c->stream);
+  {
+    KTimer t(c, KT_ZINV);
+    launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream);
+  }
+  {
+    KTimer t(c, KT_XPASS);
+    XPassArgs a{};
+    a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
+    a.cflbits = cfl ? c->d_cfl : nullptr;
+    a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->xpass_chunk; a.rz = 0;
+    launch_xpass(g, c->d_tw, c->ops, a, c->stream);
+  }
+  {
+    KTimer t(c, KT_ZFWD);
+    launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * nyp, ny,
+                c->stream);
+  }
+  if (cfl) launch_cfl_finalize(c->ops, c->prm, c->d_cfl, c->d_sc, step, c->stream);
+}
+
+void solve_pass(chan_ctx* c, int rk) {
+  KTimer t(c, KT_SOLVE);
+  SolveArgs a{};
+  a.nl = c->d_nl; a.vel = c->d_vel; a.R = c->d_R; a.P = c->d_P; a.cbc = c->d_cbc; a.dudy = c->d_dudy;
+  a.sc = c->d_sc; a.scr = c->d_scr; a.rk = rk; a.ny = c->prm.ny; a.nxh = c->nxh; a.nz = c->prm.nz;
+  a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
+  a.fix_mass = c->prm.fix_mass;
+  launch_solve(c->ops, a, c->stream);
+}
+
+__global__ void k_advance_time(DevScalars* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->t_total = sc->t_total + sc->dt;
+}
+
+// boundary (PFC/read_write.f90:1465-1532)
+void boundary_dev(chan_ctx* c) {
+  KTimer t(c, KT_OTHER);
+  const long nm = nmodes(c);
+  launch_fill_zero(c->d_cbc, sizeof(double2) * 6 * nm, c->stream);
+  const double gains[2] = {c->prm.ctrlu, c->prm.ctrlw};
+  if (gains[0] == 0.0 && gains[1] == 0.0) return;
+  const XZGeom& g = c->gd;
+  const int ny = c->prm.ny;
+  const long np = (long)g.nz2 * g.nx2;
+  const double* dm = c->h_dm;
+  double* ph = (double*)stage(c, sizeof(double) * np * 6);
+  for (int q = 0; q < 2; ++q) {
+    if (gains[q] == 0.0) continue;
+    const int comp = (q == 0) ? 0 : 2;
+    const double add = (q == 0) ? 2.0 : 0.0;
+    const double2* base = c->d_vel + (long)comp * c->nyp * nm;
+    // planes 2,3 (0-based 1,2) and ny, ny-1 (0-based ny-1, ny-2)
+    dev_c2r(c, g, base + 1 * nm, 1, ph + 0 * np);
+    dev_c2r(c, g, base + 2 * nm, 1, ph + 1 * np);
+    dev_c2r(c, g, base + (long)(ny - 1) * nm, 1, ph + 2 * np);
+    dev_c2r(c, g, base + (long)(ny - 2) * nm, 1, ph + 3 * np);
+    // bottom: (dm(2,1) u2 + dm(3,1) u3 - add)/(gain - dm(1,1))
+    launch_ctrl_law((int)np, ph + 0 * np, ph + 1 * np, dm[1], dm[2], -add, 1.0, gains[q] - dm[0], ph + 4 * np,
+                    c->stream);
+    // top: -(dm(2,2) u_ny + dm(1,2) u_{ny-1} + add)/(gain + dm(3,2))
+    launch_ctrl_law((int)np, ph + 2 * np, ph + 3 * np, dm[4], dm[3], add, -1.0, gains[q] + dm[5], ph + 5 * np,
+                    c->stream);
+    double2* top = c->d_cbc + (0L * 3 + comp) * nm;
+    double2* bot = c->d_cbc + (1L * 3 + comp) * nm;
+    dev_r2c(c, g, ph + 4 * np, 1, bot);
+    dev_r2c(c, g, ph + 5 * np, 1, top);
+  }
+}
+
+void check_err(chan_ctx* c) {
+  DevScalars sc;
+  d2h(c, &sc, c->d_sc, sizeof(sc));
+  if (sc.err) {
+    int e = sc.err;
+    int zero = 0;
+    CHAN_CHECK(hipMemcpy(&c->d_sc->err, &zero, sizeof(int), hipMemcpyHostToDevice));
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%s (max CFL = %.6e)", e == 2 ? "DT IS TOO BIG" : "CFL IS TOO BIG", sc.cfl);
+    throw chan_error(buf);
+  }
+}
+
+void collect_times(chan_ctx* c) {
+  if (c->ev.empty()) return;
+  CHAN_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i + 1 < c->ev.size(); i += 2) {
+    float ms = 0.f;
+    CHAN_CHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    int k = c->ev_kind[i / 2];
+    c->ktime_ms[k] += ms;
+    c->kcount[k] += 1;
+    (void)hipEventDestroy(c->ev[i]);
+    (void)hipEventDestroy(c->ev[i + 1]);
+  }
+  c->ev.clear();
+  c->ev_kind.clear();
+}
+
+void ensure_nl_in_nl(chan_ctx* c) {
+  if (!c->data1_is_nl) d2d(c, c->d_nl, c->d_vel, vel_bytes(c));
+}
+
+void ensure_vel_in_vel(chan_ctx* c) {
+  if (c->data1_is_nl) {
+    d2d(c, c->d_vel, c->d_nl, vel_bytes(c));
+    c->data1_is_nl = false;
+  }
+}
+
+void init_flow(chan_ctx* c, bool lin) {
+  check_ops(c);
+  ensure_vel_in_vel(c);
+  const int ny = c->prm.ny, nxh = c->nxh, nz = c->prm.nz;
+  // remove_initial_divergence (PFC/init_vel.f90:518-536)
+  launch_project(c->ops, ny, nxh, nz, c->d_vel, c->d_vel, nullptr, c->d_scr, 1.0, 0, nullptr, c->prm.re,
+                 c->stream);
+  // d_um_dy (PFC/init_vel.f90:333-342)
+  launch_dudy(c->ops, ny, nxh, nz, c->d_vel, c->d_dudy, c->stream);
+  // data2 = data1 (:344-348)
+  d2d(c, c->d_R, c->d_vel, vel_bytes(c));
+  // compute_initial_pressure: nonlinear(step=-1, rk_step=3) + solve_continuous_poisson
+  nl_pass(c, -1, 3, lin, false);
+  launch_project(c->ops, ny, nxh, nz, c->d_nl, nullptr, c->d_P, c->d_scr, 1.0, 1, c->d_vel, c->prm.re,
+                 c->stream);
+  c->data1_is_nl = false;
+}
+
+void roots_for(std::vector<double2>& roots, std::map<int, int>& off, int n) {
+  if (off.count(n)) return;
+  off[n] = (int)roots.size();
+  for (int t = 0; t < n; ++t) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    roots.push_back(make_double2((double)cosl(ang), (double)sinl(ang)));
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+FFTPlan make_plan(int n, int tw_offset) {
+  FFTPlan p{};
+  p.n = n;
+  p.tw = tw_offset;
+  int a = 0, m = n;
+  while (m % 2 == 0) { m /= 2; ++a; }
+  int b = 0;
+  while (m % 3 == 0) { m /= 3; ++b; }
+  if (m != 1) throw chan_error("transform length " + std::to_string(n) + " is not of the form 2^a 3^b");
+  // powers of two as radix-8 stages, the remainder as 4, 4x4 or 2
+  std::vector<int> r;
+  const int n8 = a / 3, rem = a % 3;
+  if (rem == 0) { for (int i = 0; i < n8; ++i) r.push_back(8); }
+  else if (rem == 2) { for (int i = 0; i < n8; ++i) r.push_back(8); r.push_back(4); }
+  else if (n8 >= 1) { for (int i = 0; i < n8 - 1; ++i) r.push_back(8); r.push_back(4); r.push_back(4); }
+  else r.push_back(2);
+  for (int i = 0; i < b; ++i) r.push_back(3);
+  if ((int)r.size() > FFT_MAXSTAGES) throw chan_error("transform length too large");
+  p.nst = (int)r.size();
+  for (int i = 0; i < p.nst; ++i) p.rad[i] = r[i];
+  return p;
+}
+
+extern "C" {
+
+const char* chan_version(void) { return "chanflow 1 gfx950 fp64"; }
+
+const char* chan_last_error(const chan_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->last_error.c_str();
+}
+
+int chan_create(const chan_params* prm, chan_ctx** out) {
+  if (!prm || !out) return fail(nullptr, "null argument");
+  *out = nullptr;
+  const chan_params& p = *prm;
+  if (p.bl_grid != 0) return fail(nullptr, "bl_grid != 0 (boundary-layer grids) is not supported");
+  if (p.nx < 4 || p.nz < 4 || p.nx % 2 || p.nz % 2) return fail(nullptr, "nx, nz must be even and >= 4");
+  if (p.ny < 6) return fail(nullptr, "ny must be >= 6");
+  if (p.nx2 < p.nx || p.nz2 < p.nz || p.nx2 % 2) return fail(nullptr, "need nx2 >= nx (even), nz2 >= nz");
+  if (!smooth23(p.nx2 / 2) || !smooth23(p.nz2) || !smooth23(p.nz) || !smooth23(p.nx / 2))
+    return fail(nullptr, "transform lengths must be of the form 2^a 3^b");
+  if (p.re <= 0.0 || p.lx <= 0.0 || p.lz <= 0.0) return fail(nullptr, "invalid re/lx/lz");
+  chan_ctx* c = new chan_ctx();
+  c->prm = p;
+  try {
+    c->nxh = p.nx / 2 + 1;
+    c->nyp = p.ny + 1;
+    c->nmodes = c->nxh * p.nz;
+    std::vector<double2> roots;
+    std::map<int, int> off;
+    for (int n : {p.nz2, p.nx2 / 2, p.nx2, p.nz, p.nx / 2, p.nx}) roots_for(roots, off, n);
+    auto geo = [&](XZGeom& g, int nx2, int nz2, int small) {
+      g.nx = p.nx; g.nz = p.nz; g.nxh = c->nxh; g.nx2 = nx2; g.nz2 = nz2;
+      g.kxpad = roundup(c->nxh, 8);
+      g.small = small;
+      g.pz = make_plan(nz2, off[nz2]);
+      g.px = make_plan(nx2 / 2, off[nx2 / 2]);
+      g.twx = off[nx2];
+    };
+    geo(c->gd, p.nx2, p.nz2, 0);
+    geo(c->gs, p.nx, p.nz, 1);
+    c->kxpad = c->gd.kxpad;
+    CHAN_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+    c->d_tw = (double2*)dalloc(sizeof(double2) * roots.size());
+    CHAN_CHECK(hipMemcpy(c->d_tw, roots.data(), sizeof(double2) * roots.size(), hipMemcpyHostToDevice));
+    const long nm = c->nmodes, nyp = c->nyp;
+    c->d_vel = (double2*)dalloc(vel_bytes(c));
+    c->d_nl = (double2*)dalloc(vel_bytes(c));
+    c->d_R = (double2*)dalloc(vel_bytes(c));
+    c->d_P = (double2*)dalloc(p_bytes(c));
+    c->d_cbc = (double2*)dalloc(sizeof(double2) * 6 * nm);
+    c->d_dudy = (double*)dalloc(sizeof(double) * p.ny);
+    c->d_sc = (DevScalars*)dalloc(sizeof(DevScalars));
+    c->d_cfl = (unsigned long long*)dalloc(sizeof(unsigned long long) * 3 * nyp);
+    const long zsz = 3L * nyp * p.nz2 * c->kxpad;
+    c->d_Z = (double2*)dalloc(sizeof(double2) * zsz);
+    c->d_AB = (double2*)dalloc(sizeof(double2) * 2 * zsz);
+    c->d_scr = (double2*)dalloc((sizeof(double2) * 4 + sizeof(double) * 2) * nyp * nm);
+    CHAN_CHECK(hipMemset(c->d_vel, 0, vel_bytes(c)));
+    CHAN_CHECK(hipMemset(c->d_nl, 0, vel_bytes(c)));
+    CHAN_CHECK(hipMemset(c->d_R, 0, vel_bytes(c)));
+    CHAN_CHECK(hipMemset(c->d_P, 0, p_bytes(c)));
+    CHAN_CHECK(hipMemset(c->d_cbc, 0, sizeof(double2) * 6 * nm));
+    CHAN_CHECK(hipMemset(c->d_dudy, 0, sizeof(double) * p.ny));
+    DevScalars sc{};
+    sc.dt = 0.25 * p.lx / (double)p.nx;       // PFC/setup.f90:1377
+    sc.p_grad = -2.0 / p.re;                  // PFC/setup.f90:1384
+    CHAN_CHECK(hipMemcpy(c->d_sc, &sc, sizeof(sc), hipMemcpyHostToDevice));
+    CHAN_CHECK(hipDeviceSynchronize());
+  } catch (const chan_error& e) {
+    g_create_error = e.msg;
+    chan_destroy(c);
+    return 1;
+  }
+  *out = c;
+  return 0;
+}
+
+int chan_destroy(chan_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  dfree(c->d_tw); dfree(c->d_ops); dfree(c->d_vel); dfree(c->d_nl); dfree(c->d_R); dfree(c->d_P);
+  dfree(c->d_cbc); dfree(c->d_dudy); dfree(c->d_sc); dfree(c->d_cfl); dfree(c->d_Z); dfree(c->d_AB);
+  dfree(c->d_scr); dfree(c->d_base);
+  if (c->d_stage) (void)hipFree(c->d_stage);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int chan_upload_operators(chan_ctx* c, const chan_operators* o) {
+  return guarded(c, [&] {
+    if (!o || !o->d2h || !o->d2n || !o->dm || !o->dh || !o->dn || !o->hn || !o->iu || !o->dyh || !o->dy ||
+        !o->kx || !o->kz)
+      throw chan_error("missing operator array");
+    const int ny = c->prm.ny, nyp = c->nyp, nxh = c->nxh, nz = c->prm.nz;
+    const long nb = (long)c->prm.nx2 * c->prm.nz2;
+    bool hasbuf = false;
+    if (o->buffer)
+      for (long i = 0; i < nb; ++i)
+        if (o->buffer[i] != 0.0) { hasbuf = true; break; }
+    std::vector<double> h;
+    std::vector<size_t> at;
+    auto put = [&](const double* src, long n) {
+      at.push_back(h.size());
+      if (src) h.insert(h.end(), src, src + n);
+      else h.insert(h.end(), n, 0.0);
+      while (h.size() % 2) h.push_back(0.0);
+    };
+    put(o->d2h, 3L * nyp); put(o->d2n, 3L * ny); put(o->dm, 12); put(o->dh, nyp); put(o->dn, ny);
+    put(o->hn, ny); put(o->iu, 2L * ny); put(o->dyh, nyp); put(o->dy, nyp); put(o->kx, nxh); put(o->kz, nz);
+    put(o->target_inflow, 3L * nyp);
+    if (hasbuf) put(o->buffer, nb);
+    dfree(c->d_ops);
+    c->d_ops = (double*)dalloc(sizeof(double) * h.size());
+    CHAN_CHECK(hipMemcpy(c->d_ops, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    double* b = c->d_ops;
+    c->ops.d2h = b + at[0]; c->ops.d2n = b + at[1]; c->ops.dm = b + at[2]; c->ops.dh = b + at[3];
+    c->ops.dn = b + at[4]; c->ops.hn = b + at[5]; c->ops.iu = b + at[6]; c->ops.dyh = b + at[7];
+    c->ops.dy = b + at[8]; c->ops.kx = b + at[9]; c->ops.kz = b + at[10]; c->ops.target = b + at[11];
+    c->ops.buffer = hasbuf ? b + at[12] : nullptr;
+    c->has_buffer = hasbuf;
+    std::memcpy(c->h_dm, o->dm, sizeof(double) * 12);
+  });
+}
+
+// define_coordinates (PFC/setup.f90:541-754) and the wavenumbers of
+// define_parameters (PFC/setup.f90:1439-1466) for the channel grid.
+int chan_build_operators(chan_ctx* c, double stretch, double* out_y, double* out_yh) {
+  return guarded(c, [&] {
+    const int ny = c->prm.ny, nyp = ny + 1, nx = c->prm.nx, nz = c->prm.nz;
+    const double half = 0.5, one = 1.0, two = 2.0;
+    std::vector<double> y(ny), yh(nyp), hh(nyp), dh(nyp), hn(ny), dn(ny), iu(2 * ny), dm(12, 0.0),
+        d2n(3 * ny, 0.0), d2h(3 * nyp, 0.0), dyh(nyp, 0.0), dy(nyp, 0.0), kx(nx / 2 + 1), kz(nz, 0.0);
+    for (int p = 1; p <= ny; ++p) {
+      double xi = two * (double)(p - 1) / (double)(ny - 1) - one;
+      y[p - 1] = std::tanh(stretch * xi) / std::tanh(stretch);
+    }
+    yh[0] = y[0];
+    for (int j = 1; j < ny; ++j) yh[j] = half * (y[j - 1] + y[j]);
+    yh[ny] = y[ny - 1];
+    hh[0] = y[1] - y[0];
+    for (int j = 1; j < ny; ++j) hh[j] = y[j] - y[j - 1];
+    hh[ny] = y[ny - 1] - y[ny - 2];
+    for (int j = 0; j < nyp; ++j) dh[j] = one / hh[j];
+    for (int j = 0; j < ny; ++j) { hn[j] = yh[j + 1] - yh[j]; dn[j] = one / hn[j]; }
+    for (int j = 0; j < ny; ++j) { iu[2 * j] = half * hh[j] / hn[j]; iu[2 * j + 1] = half * hh[j + 1] / hn[j]; }
+    auto DM = [&](int i, int k) -> double& { return dm[(i - 1) + 3 * (k - 1)]; };
+    DM(1, 1) = -dn[0] * (two * hn[0] + hn[1]) / (hn[0] + hn[1]);
+    DM(2, 1) = dn[0] * dn[1] * (hn[0] + hn[1]);
+    DM(3, 1) = -dn[1] * hn[0] / (hn[0] + hn[1]);
+    DM(1, 2) = dn[ny - 2] * hn[ny - 1] / (hn[ny - 2] + hn[ny - 1]);
+    DM(2, 2) = -dn[ny - 1] * dn[ny - 2] * (hn[ny - 1] + hn[ny - 2]);
+    DM(3, 2) = dn[ny - 1] * (two * hn[ny - 1] + hn[ny - 2]) / (hn[ny - 1] + hn[ny - 2]);
+    DM(1, 3) = (y[0] - y[1] + y[0] - y[2]) / ((y[0] - y[1]) * (y[0] - y[2]));
+    DM(2, 3) = (y[0] - y[2]) / ((y[1] - y[0]) * (y[1] - y[2]));
+    DM(3, 3) = (y[0] - y[1]) / ((y[2] - y[0]) * (y[2] - y[1]));
+    DM(1, 4) = (y[ny - 1] - y[ny - 2]) / ((y[ny - 3] - y[ny - 1]) * (y[ny - 3] - y[ny - 2]));
+    DM(2, 4) = (y[ny - 1] - y[ny - 3]) / ((y[ny - 2] - y[ny - 1]) * (y[ny - 2] - y[ny - 3]));
+    DM(3, 4) = (y[ny - 1] - y[ny - 2] + y[ny - 1] - y[ny - 3]) / ((y[ny - 1] - y[ny - 2]) * (y[ny - 1] - y[ny - 3]));
+    for (int p = 2; p <= ny - 1; ++p) {
+      int a = p - 1;
+      d2n[3 * a + 0] = dn[a] * dh[a];
+      d2n[3 * a + 1] = -dn[a] * dh[a] - dn[a] * dh[a + 1];
+      d2n[3 * a + 2] = dn[a] * dh[a + 1];
+      d2h[3 * a + 0] = dh[a] * dn[a - 1];
+      d2h[3 * a + 1] = -dh[a] * dn[a - 1] - dh[a] * dn[a];
+      d2h[3 * a + 2] = dh[a] * dn[a];
+    }
+    d2h[3 * 1 + 0] = -dh[1] * DM(1, 1);
+    d2h[3 * 1 + 1] = -dh[1] * dn[1] - dh[1] * DM(2, 1);
+    d2h[3 * 1 + 2] = dh[1] * dn[1] - dh[1] * DM(3, 1);
+    d2h[3 * (ny - 1) + 0] = dh[ny - 1] * dn[ny - 2] + dh[ny - 1] * DM(1, 2);
+    d2h[3 * (ny - 1) + 1] = -dh[ny - 1] * dn[ny - 2] + dh[ny - 1] * DM(2, 2);
+    d2h[3 * (ny - 1) + 2] = dh[ny - 1] * DM(3, 2);
+    for (int i = 0; i < 3; ++i) {
+      d2h[i] = d2h[3 + i];
+      d2n[i] = d2n[3 + i];
+      d2n[3 * (ny - 1) + i] = d2n[3 * (ny - 2) + i];
+      d2h[3 * ny + i] = d2h[3 * (ny - 1) + i];
+    }
+    for (int j = 1; j < ny; ++j) dyh[j] = hh[j];
+    dy[0] = half * hh[1];
+    for (int j = 1; j < ny - 1; ++j) dy[j] = half * (hh[j] + hh[j + 1]);
+    dy[ny - 1] = half * hh[ny - 1];
+    const double pi = std::atan2(0.0, -1.0);
+    for (int x = 0; x <= nx / 2; ++x) kx[x] = (double)x * 2.0 * pi / c->prm.lx;
+    for (int z = 0; z <= nz / 2; ++z) kz[z] = (double)z * 2.0 * pi / c->prm.lz;
+    for (int z = nz; z >= nz / 2 + 2; --z) kz[z - 1] = -kz[nz + 2 - z - 1];
+    chan_operators o{};
+    o.d2h = d2h.data(); o.d2n = d2n.data(); o.dm = dm.data(); o.dh = dh.data(); o.dn = dn.data();
+    o.hn = hn.data(); o.iu = iu.data(); o.dyh = dyh.data(); o.dy = dy.data(); o.kx = kx.data(); o.kz = kz.data();
+    o.buffer = nullptr; o.target_inflow = nullptr;
+    if (chan_upload_operators(c, &o)) throw chan_error(c->last_error);
+    if (out_y) std::memcpy(out_y, y.data(), sizeof(double) * ny);
+    if (out_yh) std::memcpy(out_yh, yh.data(), sizeof(double) * nyp);
+  });
+}
+
+int chan_set_state(chan_ctx* c, const double* data1, const double* data2, const double* data4) {
+  return guarded(c, [&] {
+    const int ny = c->prm.ny;
+    if (data1) { h2d(c, c->d_vel, data1, vel_bytes(c)); c->data1_is_nl = false; }
+    if (data2) {
+      void* s = stage(c, vel_bytes(c));
+      h2d(c, s, data2, vel_bytes(c));
+      launch_layout(0, ny, c->nxh, c->prm.nz, (const double2*)s, c->d_R, c->stream);
+    }
+    if (data4) {
+      void* s = stage(c, vel_bytes(c));
+      h2d(c, s, data4, p_bytes(c));
+      launch_layout(2, ny, c->nxh, c->prm.nz, (const double2*)s, c->d_P, c->stream);
+    }
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_get_state(chan_ctx* c, double* data1, double* data2, double* data4) {
+  return guarded(c, [&] {
+    const int ny = c->prm.ny;
+    if (data1) d2h(c, data1, c->data1_is_nl ? c->d_nl : c->d_vel, vel_bytes(c));
+    if (data2) {
+      void* s = stage(c, vel_bytes(c));
+      launch_layout(1, ny, c->nxh, c->prm.nz, c->d_R, (double2*)s, c->stream);
+      d2h(c, data2, s, vel_bytes(c));
+    }
+    if (data4) {
+      void* s = stage(c, vel_bytes(c));
+      launch_layout(3, ny, c->nxh, c->prm.nz, c->d_P, (double2*)s, c->stream);
+      d2h(c, data4, s, p_bytes(c));
+    }
+  });
+}
+
+int chan_get_planes(chan_ctx* c, int comp, int k0, int k1, double* out) {
+  return guarded(c, [&] {
+    if (comp < 1 || comp > 3 || k0 < 1 || k1 > c->nyp || k1 < k0) throw chan_error("bad plane range");
+    const double2* src = (c->data1_is_nl ? c->d_nl : c->d_vel) + ((long)(comp - 1) * c->nyp + (k0 - 1)) * nmodes(c);
+    d2h(c, out, src, sizeof(double2) * nmodes(c) * (k1 - k0 + 1));
+  });
+}
+
+int chan_set_scalars(chan_ctx* c, const double* dt, const double* t_total, const double* p_grad,
+                     const double* d_um_dy) {
+  return guarded(c, [&] {
+    DevScalars sc;
+    d2h(c, &sc, c->d_sc, sizeof(sc));
+    if (dt) sc.dt = *dt;
+    if (t_total) sc.t_total = *t_total;
+    if (p_grad) sc.p_grad = *p_grad;
+    h2d(c, c->d_sc, &sc, sizeof(sc));
+    if (d_um_dy) h2d(c, c->d_dudy, d_um_dy, sizeof(double) * c->prm.ny);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_get_scalars(chan_ctx* c, double* dt, double* t_total, double* p_grad, double* d_um_dy) {
+  return guarded(c, [&] {
+    DevScalars sc;
+    d2h(c, &sc, c->d_sc, sizeof(sc));
+    if (dt) *dt = sc.dt;
+    if (t_total) *t_total = sc.t_total;
+    if (p_grad) *p_grad = sc.p_grad;
+    if (d_um_dy) d2h(c, d_um_dy, c->d_dudy, sizeof(double) * c->prm.ny);
+  });
+}
+
+int chan_set_wall_bc(chan_ctx* c, const double* top, const double* bot) {
+  return guarded(c, [&] {
+    const size_t b = sizeof(double2) * 3 * nmodes(c);
+    if (top) h2d(c, c->d_cbc, top, b);
+    if (bot) h2d(c, c->d_cbc + 3 * nmodes(c), bot, b);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+static int nonlinear_common(chan_ctx* c, int step, int rk, double* dt_inout, double* cfl_out, bool lin) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (dt_inout) {
+      DevScalars sc;
+      d2h(c, &sc, c->d_sc, sizeof(sc));
+      sc.dt = *dt_inout;
+      h2d(c, c->d_sc, &sc, sizeof(sc));
+    }
+    ensure_vel_in_vel(c);
+    nl_pass(c, step, rk, lin, rk == 1);
+    c->data1_is_nl = true;
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    if (rk == 1) check_err(c);
+    DevScalars sc;
+    d2h(c, &sc, c->d_sc, sizeof(sc));
+    if (dt_inout) *dt_inout = sc.dt;
+    if (cfl_out) *cfl_out = (rk == 1) ? sc.cfl : 0.0;
+  });
+}
+
+int chan_nonlinear(chan_ctx* c, int step, int rk, double* dt_inout, double* cfl_out) {
+  return nonlinear_common(c, step, rk, dt_inout, cfl_out, false);
+}
+
+int chan_nonlinear_linearized(chan_ctx* c, int step, int rk, double* dt_inout, double* cfl_out) {
+  return nonlinear_common(c, step, rk, dt_inout, cfl_out, true);
+}
+
+int chan_set_baseflow(chan_ctx* c, const double* base) {
+  return guarded(c, [&] {
+    const size_t b = sizeof(double) * 3 * c->nyp * (size_t)c->prm.nx2 * c->prm.nz2;
+    if (!c->d_base) c->d_base = (double*)dalloc(b);
+    h2d(c, c->d_base, base, b);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_rhs_solve(chan_ctx* c, int step, int rk, double* p_grad_out) {
+  (void)step;
+  return guarded(c, [&] {
+    check_ops(c);
+    if (rk < 1 || rk > 3) throw chan_error("rk_step must be 1..3");
+    ensure_nl_in_nl(c);
+    solve_pass(c, rk);
+    c->data1_is_nl = false;
+    if (p_grad_out) {
+      DevScalars sc;
+      d2h(c, &sc, c->d_sc, sizeof(sc));
+      *p_grad_out = sc.p_grad;
+    }
+  });
+}
+
+int chan_step(chan_ctx* c, int step0, int nsteps, int lin) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    for (int s = 0; s < nsteps; ++s) {
+      boundary_dev(c);
+      for (int rk = 1; rk <= 3; ++rk) {
+        nl_pass(c, step0 + s, rk, lin != 0, rk == 1);
+        solve_pass(c, rk);
+      }
+      hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, c->stream, c->d_sc);
+      CHAN_CHECK(hipGetLastError());
+    }
+    c->data1_is_nl = false;
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    check_err(c);
+  });
+}
+
+int chan_boundary(chan_ctx* c) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    boundary_dev(c);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_initialize_flow(chan_ctx* c, int lin) {
+  return guarded(c, [&] {
+    init_flow(c, lin != 0);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_remove_divergence(chan_ctx* c) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    launch_project(c->ops, c->prm.ny, c->nxh, c->prm.nz, c->d_vel, c->d_vel, nullptr, c->d_scr, 1.0, 0,
+                   nullptr, c->prm.re, c->stream);
+    launch_dudy(c->ops, c->prm.ny, c->nxh, c->prm.nz, c->d_vel, c->d_dudy, c->stream);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_compute_initial_pressure(chan_ctx* c, int lin) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    d2d(c, c->d_R, c->d_vel, vel_bytes(c));
+    nl_pass(c, -1, 3, lin != 0, false);
+    launch_project(c->ops, c->prm.ny, c->nxh, c->prm.nz, c->d_nl, nullptr, c->d_P, c->d_scr, 1.0, 1,
+                   c->d_vel, c->prm.re, c->stream);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+static void check_np(chan_ctx* c, int np) {
+  if (np < 1 || np > 3 * c->nyp) throw chan_error("nplanes must be in 1..3*(ny+1)");
+}
+
+int chan_xz_fft_complex_to_real(chan_ctx* c, const double* cdata, double* data, int np) {
+  return guarded(c, [&] {
+    check_ops(c); check_np(c, np);
+    const XZGeom& g = c->gd;
+    const size_t cb = sizeof(double2) * (size_t)np * g.nxh * g.nz, rb = sizeof(double) * (size_t)np * g.nx2 * g.nz2;
+    char* s = (char*)stage(c, cb + rb);
+    h2d(c, s, cdata, cb);
+    dev_c2r(c, g, (const double2*)s, np, (double*)(s + cb));
+    d2h(c, data, s + cb, rb);
+  });
+}
+
+int chan_xz_fft_real_to_complex(chan_ctx* c, const double* data, double* cdata, int np) {
+  return guarded(c, [&] {
+    check_ops(c); check_np(c, np);
+    const XZGeom& g = c->gd;
+    const size_t cb = sizeof(double2) * (size_t)np * g.nxh * g.nz, rb = sizeof(double) * (size_t)np * g.nx2 * g.nz2;
+    char* s = (char*)stage(c, cb + rb);
+    h2d(c, s + cb, data, rb);
+    dev_r2c(c, g, (const double*)(s + cb), np, (double2*)s);
+    d2h(c, cdata, s, cb);
+  });
+}
+
+int chan_compute_divergence_of_fluxes(chan_ctx* c, const double* fx, const double* fy, const double* fz,
+                                      double* cdata, int np) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (np < 1 || 2 * np > 3 * c->nyp) throw chan_error("nplanes too large for the staging buffers");
+    const XZGeom& g = c->gd;
+    const size_t cb = sizeof(double2) * (size_t)np * g.nxh * g.nz, rb = sizeof(double) * (size_t)np * g.nx2 * g.nz2;
+    char* s = (char*)stage(c, cb + 3 * rb);
+    h2d(c, s + cb, fx, rb);
+    h2d(c, s + cb + rb, fy, rb);
+    h2d(c, s + cb + 2 * rb, fz, rb);
+    dev_div(c, (const double*)(s + cb), (const double*)(s + cb + rb), (const double*)(s + cb + 2 * rb), np,
+            (double2*)s);
+    d2h(c, cdata, s, cb);
+  });
+}
+
+int chan_small_fft_complex_to_real(chan_ctx* c, const double* cdata, double* data, int np) {
+  return guarded(c, [&] {
+    check_ops(c); check_np(c, np);
+    const XZGeom& g = c->gs;
+    const size_t cb = sizeof(double2) * (size_t)np * g.nxh * g.nz, rb = sizeof(double) * (size_t)np * g.nx2 * g.nz2;
+    char* s = (char*)stage(c, cb + rb);
+    h2d(c, s, cdata, cb);
+    dev_c2r(c, g, (const double2*)s, np, (double*)(s + cb));
+    d2h(c, data, s + cb, rb);
+  });
+}
+
+int chan_small_fft_real_to_complex(chan_ctx* c, const double* data, double* cdata, int np) {
+  return guarded(c, [&] {
+    check_ops(c); check_np(c, np);
+    const XZGeom& g = c->gs;
+    const size_t cb = sizeof(double2) * (size_t)np * g.nxh * g.nz, rb = sizeof(double) * (size_t)np * g.nx2 * g.nz2;
+    char* s = (char*)stage(c, cb + rb);
+    h2d(c, s + cb, data, rb);
+    dev_r2c(c, g, (const double*)(s + cb), np, (double2*)s);
+    d2h(c, cdata, s, cb);
+  });
+}
+
+int chan_solve_helmholtz(chan_ctx* c, int n, int nsys, double* f, const double* alpha, double diff,
+                         const double* d2, const double* topbc, const double* botbc, int top_neumann,
+                         int bot_neumann) {
+  return guarded(c, [&] {
+    if (top_neumann || bot_neumann)
+      throw chan_error("Neumann wall rows (bl_grid != 0) are not supported");
+    if (n < 3 || nsys < 1) throw chan_error("bad system size");
+    const size_t fb = sizeof(double2) * (size_t)n * nsys, ab = sizeof(double) * nsys, db = sizeof(double) * 3 * n,
+                 bb = sizeof(double2) * nsys, sb = sizeof(double) * (size_t)n * nsys;
+    char* s = (char*)stage(c, fb + ab + db + 2 * bb + sb);
+    char* p = s;
+    double2* df = (double2*)p; p += fb;
+    double* da = (double*)p; p += ab;
+    double* dd = (double*)p; p += db;
+    double2* dt = (double2*)p; p += bb;
+    double2* db_ = (double2*)p; p += bb;
+    double* scr = (double*)p;
+    h2d(c, df, f, fb); h2d(c, da, alpha, ab); h2d(c, dd, d2, db); h2d(c, dt, topbc, bb); h2d(c, db_, botbc, bb);
+    launch_helmholtz_batch(n, nsys, df, da, diff, dd, dt, db_, scr, c->stream);
+    d2h(c, f, df, fb);
+  });
+}
+
+int chan_solve_p(chan_ctx* c, int nsys, double* pp, const double* alpha) {
+  return guarded(c, [&] {
+    check_ops(c);
+    const int n = c->nyp;
+    const size_t fb = sizeof(double2) * (size_t)n * nsys, ab = sizeof(double) * nsys, sb = sizeof(double) * (size_t)n * nsys;
+    char* s = (char*)stage(c, fb + ab + sb);
+    h2d(c, s, pp, fb);
+    h2d(c, s + fb, alpha, ab);
+    launch_solvep_batch(c->prm.ny, nsys, (double2*)s, (const double*)(s + fb), c->ops, (double*)(s + fb + ab),
+                        c->stream);
+    d2h(c, pp, s, fb);
+  });
+}
+
+int chan_strip_divergence(chan_ctx* c, double* cu, double* cp, double coef) {
+  return guarded(c, [&] {
+    check_ops(c);
+    const size_t vb = vel_bytes(c), pb = p_bytes(c);
+    char* s = (char*)stage(c, vb + 2 * pb);
+    h2d(c, s, cu, vb);
+    launch_project(c->ops, c->prm.ny, c->nxh, c->prm.nz, (const double2*)s, (double2*)s, (double2*)(s + vb),
+                   c->d_scr, coef, 2, nullptr, c->prm.re, c->stream);
+    d2h(c, cu, s, vb);
+    if (cp) {
+      launch_layout(3, c->prm.ny, c->nxh, c->prm.nz, (const double2*)(s + vb), (double2*)(s + vb + pb), c->stream);
+      d2h(c, cp, s + vb + pb, pb);
+    }
+  });
+}
+
+int chan_linearized_matvec(chan_ctx* c, const double* v, double* w, int tend, double* t_out) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (!c->d_base) throw chan_error("base flow not set (chan_set_baseflow)");
+    if (c->prm.fix_mass != 0) throw chan_error("the linearized matvec needs fix_mass = 0");
+    if (tend < 1) throw chan_error("tend must be >= 1");
+    const XZGeom& g = c->gs;
+    const int nyp = c->nyp, ny = c->prm.ny;
+    const long plane_r = (long)g.nx2 * g.nz2;          // nx * nz reals
+    const long plane_c = (long)g.nxh * g.nz;
+    const size_t rb = sizeof(double) * 3 * nyp * plane_r;
+    char* s = (char*)stage(c, rb + 16);
+    double* dv = (double*)s;
+    unsigned long long* vmax = (unsigned long long*)(s + rb);
+    h2d(c, dv, v, rb);
+    launch_fill_zero(vmax, 8, c->stream);
+    launch_absmax(3L * nyp * plane_r, dv, vmax, c->stream);
+    dev_r2c(c, g, dv, 3 * nyp, c->d_vel);
+    launch_arnoldi_scale(ny, plane_c, c->d_vel, c->ops.dy, c->ops.dyh, vmax, 1.0e-8, 0, c->stream);
+    c->data1_is_nl = false;
+    d2d(c, c->d_R, c->d_vel, vel_bytes(c));
+    launch_fill_zero(c->d_cbc, sizeof(double2) * 6 * plane_c, c->stream);
+    DevScalars sc;
+    d2h(c, &sc, c->d_sc, sizeof(sc));
+    const double dt = sc.dt;
+    sc.p_grad = 0.0;   // init_baseflow (PFC/pass1_linearized.f90:126)
+    sc.t_total = 0.0;
+    h2d(c, c->d_sc, &sc, sizeof(sc));
+    // initial pressure (PFC/arnoldi.f90:528-535); the stale rk_step is never 1
+    nl_pass(c, -1, 0, true, false);
+    launch_project(c->ops, ny, c->nxh, c->prm.nz, c->d_nl, nullptr, c->d_P, c->d_scr, 1.0, 1, c->d_vel, c->prm.re,
+                   c->stream);
+    double t_total = 0.0;
+    for (int st = 1; st <= tend; ++st) {
+      for (int rk = 1; rk <= 3; ++rk) {
+        nl_pass(c, st, rk, true, rk == 1);
+        solve_pass(c, rk);
+      }
+      t_total = t_total + dt;
+    }
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    check_err(c);
+    launch_arnoldi_scale(ny, plane_c, c->d_vel, c->ops.dy, c->ops.dyh, vmax, 1.0e-8, 1, c->stream);
+    dev_c2r(c, g, c->d_vel, 3 * nyp, dv);
+    d2h(c, w, dv, rb);
+    if (t_out) *t_out = t_total;
+  });
+}
+
+int chan_divergence_max(chan_ctx* c, double* out) {
+  return guarded(c, [&] {
+    check_ops(c);
+    unsigned long long* d = (unsigned long long*)stage(c, 8);
+    launch_fill_zero(d, 8, c->stream);
+    launch_divmax(c->ops, c->prm.ny, c->nxh, c->prm.nz, c->data1_is_nl ? c->d_nl : c->d_vel, d, c->stream);
+    unsigned long long h;
+    d2h(c, &h, d, 8);
+    std::memcpy(out, &h, 8);
+  });
+}
+
+int chan_mass_flux(chan_ctx* c, double* out) {
+  return guarded(c, [&] {
+    check_ops(c);
+    const int nyp = c->nyp;
+    std::vector<double2> col(nyp);
+    std::vector<double> dyh(nyp);
+    const double2* src = c->data1_is_nl ? c->d_nl : c->d_vel;
+    CHAN_CHECK(hipMemcpy2DAsync(col.data(), sizeof(double2), src, sizeof(double2) * nmodes(c), sizeof(double2), nyp,
+                                hipMemcpyDeviceToHost, c->stream));
+    d2h(c, dyh.data(), c->ops.dyh, sizeof(double) * nyp);
+    double q = 0.0;
+    for (int j = 0; j < nyp; ++j) q += col[j].x * dyh[j];
+    *out = q;
+  });
+}
+
+int chan_set_timing(chan_ctx* c, int enable) {
+  return guarded(c, [&] {
+    collect_times(c);
+    c->timing = enable;
+    for (int k = 0; k < 8; ++k) { c->ktime_ms[k] = 0; c->kcount[k] = 0; }
+  });
+}
+
+int chan_get_kernel_times(chan_ctx* c, double* ms, long* counts, int nk) {
+  return guarded(c, [&] {
+    collect_times(c);
+    for (int k = 0; k < nk && k < 8; ++k) {
+      if (ms) ms[k] = c->ktime_ms[k];
+      if (counts) counts[k] = c->kcount[k];
+    }
+  });
+}
+
+int chan_sync(chan_ctx* c) {
+  return guarded(c, [&] {
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    check_err(c);
+  });
+}
+
+}  // extern "C"

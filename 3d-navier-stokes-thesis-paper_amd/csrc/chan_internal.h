@@ -1,0 +1,172 @@
+// Internal declarations shared by the HIP translation units of libchanflow.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "fft_device.h"
+#include "../../include/chanflow.h"
+
+#define CHAN_CHECK(x)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      throw chan_error(std::string(#x) + ": " + hipGetErrorString(e_));            \
+    }                                                                              \
+  } while (0)
+
+struct chan_error {
+  std::string msg;
+  explicit chan_error(std::string m) : msg(std::move(m)) {}
+};
+
+// Geometry of one dealiased (or small) x-z transform family.
+struct XZGeom {
+  int nx, nz, nxh;    // spectral sizes: nxh = nx/2+1 stored kx, nz kz
+  int nx2, nz2;       // physical sizes
+  int kxpad;          // row length (complex) of the [zz][kx] intermediate
+  int small;          // 1: "small" transforms (no padding, split, fold or doubling)
+  FFTPlan pz;         // complex length nz2
+  FFTPlan px;         // complex length nx2/2 (half-length real transform)
+  int twx;            // root-table offset for length nx2 (real pre/post twiddles)
+};
+
+// Device-resident scalars of one channel instance.
+struct DevScalars {
+  double dt;
+  double p_grad;
+  double t_total;
+  double cfl;          // last max CFL (rk_step==1)
+  int err;             // nonzero: CFL abort (1: CFL IS TOO BIG, 2: DT IS TOO BIG)
+  int pad;
+};
+
+// Operators uploaded from the driver (layouts of the reference's globals).
+struct DevOps {
+  const double* d2h;   // d2h(3, ny+1)
+  const double* d2n;   // d2n(3, ny)
+  const double* dm;    // dm(3, 4)
+  const double* dh;    // dh(ny+1)
+  const double* dn;    // dn(ny)
+  const double* hn;    // hn(ny)
+  const double* iu;    // iu(2, ny)
+  const double* dyh;   // dyh(ny+1)
+  const double* dy;    // dy(ny+1)
+  const double* kx;    // kx(nx/2+1)
+  const double* kz;    // kz(nz)
+  const double* buffer;   // buffer(nx2, nz2) or nullptr when identically zero
+  const double* target;   // target_inflow(ny+1, 3)
+};
+
+// Parameters for one nonlinear x-pass launch.
+struct XPassArgs {
+  const double2* Z;       // [3][ny+1][nz2][kxpad]
+  double2* AB;            // [3][ny+1][2][nz2][kxpad]
+  const double* base;     // linearized base flow [3][ny+1][nz2][nx2] or nullptr
+  unsigned long long* cflbits;  // [ny+1][3] or nullptr
+  int ny;
+  int ka, kb;             // reference planes computed: NL(k) for ka <= k <= kb
+  int chunk;              // planes per workgroup
+  int rz;                 // rows per workgroup
+};
+
+struct chan_ctx {
+  chan_params prm;
+  int nxh, nyp, nmodes, kxpad;
+  XZGeom gd;     // dealiased transforms
+  XZGeom gs;     // small transforms (nx x nz grid)
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+
+  // twiddle roots
+  double2* d_tw = nullptr;
+  // operators
+  double* d_ops = nullptr;  // packed
+  DevOps ops{};
+  double h_dm[12] = {0};         // host copy of dm(3,4) for the feedback law
+  bool has_buffer = false;
+  // state (plane-major, kz fastest): vel and nl are [3][ny+1][nxh][nz]
+  double2* d_vel = nullptr;
+  double2* d_nl = nullptr;
+  double2* d_R = nullptr;
+  double2* d_P = nullptr;        // [ny+1][nxh][nz]
+  double2* d_cbc = nullptr;      // [2][3][nxh][nz]: top then bottom
+  double* d_dudy = nullptr;      // d_um_dy(ny)
+  DevScalars* d_sc = nullptr;
+  unsigned long long* d_cfl = nullptr;  // [ny+1][3]
+  // intermediates
+  double2* d_Z = nullptr;
+  double2* d_AB = nullptr;
+  double2* d_scr = nullptr;      // solve scratch: 4 complex + 2 real per (row, mode)
+  double* d_base = nullptr;      // linearized base flow (physical)
+  // generic staging buffers for host-pointer entry points
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
+  // options
+  int xpass_chunk = 8;
+  int timing = 0;
+  std::vector<hipEvent_t> ev;    // timing events (pairs)
+  std::vector<int> ev_kind;
+  double ktime_ms[8] = {0};
+  long kcount[8] = {0};
+};
+
+// kernel-kind indices for timing
+enum { KT_ZINV = 0, KT_XPASS, KT_ZFWD, KT_SOLVE, KT_OTHER, KT_N };
+
+// plan helpers (host)
+FFTPlan make_plan(int n, int tw_offset);
+void xz_geom_init(XZGeom& g, int nx, int nz, int nx2, int nz2, int small, std::vector<double2>& roots);
+
+// launchers (k_transform.hip)
+void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
+                 long out_ps, int nplanes, hipStream_t s);
+void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps, double* out,
+                 long out_ps, int nplanes, hipStream_t s);
+void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const double* fx,
+                 const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s);
+void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
+                 const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
+                 hipStream_t s);
+// (k_nonlinear.hip)
+void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
+                  hipStream_t s);
+// (k_solve.hip)
+struct SolveArgs {
+  const double2* nl;  // [3][ny+1][nxh][nz]
+  double2* vel;       // output velocity
+  double2* R;
+  double2* P;
+  const double2* cbc; // [2][3][nxh][nz]
+  double* dudy;
+  DevScalars* sc;
+  double2* scr;
+  int rk;             // 1..3
+  int ny, nxh, nz;
+  double re, cosb, sinb;
+  int fix_mass;
+};
+void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
+void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
+                    double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
+                    double re, hipStream_t s);
+void launch_cfl_finalize(const DevOps& ops, chan_params prm, unsigned long long* cfl, DevScalars* sc,
+                         int step, hipStream_t s);
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+void launch_divmax(const DevOps& ops, int ny, int nxh, int nz, const double2* vel,
+                   unsigned long long* out, hipStream_t s);
+void launch_dudy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel, double* dudy,
+                 hipStream_t s);
+void launch_layout(int kind, int ny, int nxh, int nz, const double2* in, double2* out, hipStream_t s);
+// (k_misc.hip)
+void launch_helmholtz_batch(int n, int nsys, double2* f, const double* alpha, double diff,
+                            const double* d2, const double2* topbc, const double2* botbc, double* scr,
+                            hipStream_t s);
+void launch_solvep_batch(int ny, int nsys, double2* p, const double* alpha, const DevOps& ops,
+                         double* scr, hipStream_t s);
+void launch_ctrl_law(int npts, const double* p2, const double* p3, double ca, double cb, double add,
+                     double sgn, double den, double* out, hipStream_t s);
+void launch_absmax(long n, const double* x, unsigned long long* out, hipStream_t s);
+void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, const double* dyh,
+                          const unsigned long long* vmax, double eps, int dir, hipStream_t s);

@@ -1,0 +1,279 @@
+// Physical-space pass of the nonlinear term (PFC/pass1.f90:26-252) and of
+// its linearized variant (PFC/pass1_linearized.f90:135-509), fused per z-row.
+//
+// A workgroup owns `rz` physical z-rows and a chunk of y-planes.  Thread
+// (r, n) owns the two physical points x = 2n, 2n+1 of row r, which are the
+// real and imaginary parts of point n of the half-length complex FFT used
+// for every x transform.  Walking up the chunk, it keeps in registers the
+// velocity at planes k and k+1 and the wall-normal fluxes at y(k-1), yh(k),
+// exactly the sliding window of the reference (u = up at :81), so each plane
+// is inverse transformed once per chunk plus one halo plane.
+//
+// Per plane k the workgroup
+//   1. x-c2r's the z-inverse rows of u, v, w at plane k+1 (three half-length
+//      backward FFTs per row),
+//   2. forms the flux products and y-differences (:138-195),
+//   3. forward x-transforms the 8 distinct real fields (u^2, d/dy flux_u,
+//      w u, d/dy flux_w, w^2, the two interpolated v fluxes, d/dy flux_v),
+//   4. combines A = F(fy) + i kx F(fx) and B = F(fz) for each component
+//      (PFC/transform.f90:613) and writes them for the z-forward pass.
+// At rk_step == 1 it also accumulates the per-plane CFL maxima (:202-217).
+#include "chan_internal.h"
+
+__device__ __forceinline__ double2 c2r_pack_nl(const double2* __restrict__ row, int nxh, int M,
+                                               int n, double2 wconj) {
+  double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+  if (n < nxh) { a = row[n]; if (n == 0 || n == M) a.y = 0.0; }
+  int m = M - n;
+  if (m < nxh) { b = row[m]; if (m == 0 || m == M) b.y = 0.0; b.y = -b.y; }
+  double2 s = cadd(a, b), d = csub(a, b);
+  double2 wd = cmul(wconj, d);
+  return make_double2(s.x - wd.y, s.y + wd.x);
+}
+
+__device__ __forceinline__ double2 r2c_post_nl(const double2* __restrict__ Zs, int M, int k,
+                                               double2 wk) {
+  double2 a = Zs[k % M], b = cconj(Zs[(M - k) % M]);
+  double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+  double2 d = csub(a, b);
+  double2 O = make_double2(0.5 * d.y, -0.5 * d.x);
+  return cadd(E, cmul(wk, O));
+}
+
+// elementwise helpers on the pair of points (x = 2n, 2n+1)
+__device__ __forceinline__ double2 pmul(double2 a, double2 b) { return make_double2(a.x * b.x, a.y * b.y); }
+__device__ __forceinline__ double2 padd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 psub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 psc(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double pabsmax(double2 a) { return fmax(fabs(a.x), fabs(a.y)); }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <bool LIN>
+__global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restrict__ tw, DevOps ops,
+                                                XPassArgs a) {
+  extern __shared__ double2 lds[];
+  const int M = g.nx2 / 2, SS = M + 1, RZ = a.rz;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int r = tid / M, n = tid - r * M;
+  const bool act = r < RZ;
+  const int zz = blockIdx.x * RZ + (act ? r : 0);
+  const int ny = a.ny, nyp = ny + 1;
+  const int ka = a.ka + blockIdx.y * a.chunk;
+  const int kb = min(a.kb, ka + a.chunk - 1);
+  const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
+  const double2* rootx = tw + g.twx;
+  const double2 wconj = cconj(rootx[n < M ? n : 0]);
+  const long rowoff = (long)zz * kxp;
+
+  double2 buf = make_double2(0.0, 0.0);
+  const bool hasbuf = ops.buffer != nullptr;
+  if (hasbuf && act) buf = *reinterpret_cast<const double2*>(ops.buffer + (long)zz * g.nx2 + 2 * n);
+
+  // c2r of plane p (1-based reference plane) for the components in cmask
+  auto load_plane = [&](int p, double2& U, double2& V, double2& W, bool needv) {
+    const int nc = 3;
+    for (int c = 0; c < nc; ++c) {
+      double2 v = make_double2(0.0, 0.0);
+      if (act && (c != 1 || needv)) {
+        const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
+        v = c2r_pack_nl(row, nxh, M, n, wconj);
+      }
+      if (act) lds[(c * RZ + r) * SS + n] = v;
+    }
+    __syncthreads();
+    fft_batch<1, 8>(lds, SS, 3 * RZ, g.px, tw, tid, nthr);
+    if (act) {
+      U = lds[(0 * RZ + r) * SS + n];
+      V = lds[(1 * RZ + r) * SS + n];
+      W = lds[(2 * RZ + r) * SS + n];
+    }
+    __syncthreads();
+  };
+  auto load_base = [&](int p, double2& U, double2& V, double2& W) {
+    if (!LIN || !act) return;
+    const double* b = a.base;
+    long o = ((long)(p - 1) * nz2 + zz) * g.nx2 + 2 * n;
+    long cs = (long)nyp * nz2 * g.nx2;
+    U = *reinterpret_cast<const double2*>(b + o);
+    V = *reinterpret_cast<const double2*>(b + cs + o);
+    W = *reinterpret_cast<const double2*>(b + 2 * cs + o);
+  };
+
+  double2 U0 = {0, 0}, V0 = {0, 0}, W0 = {0, 0}, U1 = {0, 0}, V1 = {0, 0}, W1 = {0, 0};
+  double2 BU0 = {0, 0}, BV0 = {0, 0}, BW0 = {0, 0}, BU1 = {0, 0}, BV1 = {0, 0}, BW1 = {0, 0};
+  double2 FU, FW, GV;
+  const double half = 0.5, two = 2.0;
+
+  // prologue: planes ka-1 and ka, fluxes at y(ka-1) and yh(ka)
+  load_plane(ka - 1, U0, V0, W0, true);
+  load_base(ka - 1, BU0, BV0, BW0);
+  load_plane(ka, U1, V1, W1, true);
+  load_base(ka, BU1, BV1, BW1);
+  if (!LIN) {
+    if (ka - 1 == 1) { FU = pmul(V0, U0); FW = pmul(V0, W0); }                 // :53,56
+    else { FU = pmul(V0, psc(half, padd(U0, U1))); FW = pmul(V0, psc(half, padd(W0, W1))); }
+    double2 t = psc(half, padd(V0, V1)); GV = pmul(t, t);                        // :54-55
+  } else {
+    if (ka - 1 == 1) {                                                          // lin :160-170
+      FU = padd(pmul(BV0, U0), pmul(V0, BU0));
+      FW = padd(pmul(BV0, W0), pmul(V0, BW0));
+    } else {                                                                    // lin :265-273
+      FU = padd(pmul(psc(half, BV0), padd(U0, U1)), pmul(psc(half, V0), padd(BU0, BU1)));
+      FW = padd(pmul(psc(half, BV0), padd(W0, W1)), pmul(psc(half, V0), padd(BW0, BW1)));
+    }
+    double2 t = psc(half, padd(V0, V1)), tb = psc(half, padd(BV0, BV1));
+    GV = pmul(psc(two, t), tb);
+  }
+  U0 = U1; V0 = V1; W0 = W1; BU0 = BU1; BV0 = BV1; BW0 = BW1;
+
+  for (int k = ka; k <= kb; ++k) {
+    const bool top = (k == ny);
+    load_plane(k + 1, U1, V1, W1, !top);
+    load_base(k + 1, BU1, BV1, BW1);
+    const int kk = k - 1;  // 0-based plane index
+    const double dh = ops.dh[kk];
+    // wall-parallel fluxes at y(k) (:138-146)
+    double2 Fu, Fw;
+    if (!LIN) {
+      if (!top) { Fu = pmul(V0, psc(half, padd(U0, U1))); Fw = pmul(V0, psc(half, padd(W0, W1))); }
+      else { Fu = pmul(V0, U1); Fw = pmul(V0, W1); }
+    } else {
+      if (!top) {
+        Fu = padd(pmul(psc(half, BV0), padd(U0, U1)), pmul(psc(half, V0), padd(BU0, BU1)));
+        Fw = padd(pmul(psc(half, BV0), padd(W0, W1)), pmul(psc(half, V0), padd(BW0, BW1)));
+      } else {
+        Fu = padd(pmul(BV0, U1), pmul(V0, BU1));
+        Fw = padd(pmul(BV0, W1), pmul(V0, BW1));
+      }
+    }
+    double2 yfu = psc(dh, psub(Fu, FU));
+    double2 yfw = psc(dh, psub(Fw, FW));
+    if (hasbuf) {
+      double tu = ops.target[0 * nyp + kk], tw_ = ops.target[2 * nyp + kk];
+      yfu = padd(yfu, pmul(buf, make_double2(U0.x - tu, U0.y - tu)));
+      yfw = padd(yfw, pmul(buf, make_double2(W0.x - tw_, W0.y - tw_)));
+    }
+    const int nf = top ? 5 : 8;
+    double2 fld[8];
+    if (!LIN) {
+      fld[0] = pmul(U0, U0);          // x-flux of u
+      fld[2] = pmul(W0, U0);          // z-flux of u == x-flux of w
+      fld[4] = pmul(W0, W0);          // z-flux of w
+    } else {
+      fld[0] = pmul(psc(two, BU0), U0);
+      fld[2] = padd(pmul(BW0, U0), pmul(W0, BU0));
+      fld[4] = pmul(psc(two, BW0), W0);
+    }
+    fld[1] = yfu;
+    fld[3] = yfw;
+    double2 G1 = {0, 0};
+    if (!top) {
+      const double i1 = ops.iu[2 * kk], i2 = ops.iu[2 * kk + 1];
+      if (!LIN) {
+        double2 t = psc(half, padd(V0, V1)); G1 = pmul(t, t);
+        fld[5] = pmul(V0, padd(psc(i1, U0), psc(i2, U1)));
+        fld[7] = pmul(V0, padd(psc(i1, W0), psc(i2, W1)));
+      } else {
+        double2 t = psc(half, padd(V0, V1)), tb = psc(half, padd(BV0, BV1));
+        G1 = pmul(psc(two, t), tb);
+        fld[5] = padd(pmul(BV0, padd(psc(i1, U0), psc(i2, U1))), pmul(V0, padd(psc(i1, BU0), psc(i2, BU1))));
+        fld[7] = padd(pmul(BV0, padd(psc(i1, W0), psc(i2, W1))), pmul(V0, padd(psc(i1, BW0), psc(i2, BW1))));
+      }
+      double2 yfv = psc(ops.dn[kk], psub(G1, GV));                                 // :188-189
+      if (hasbuf) {
+        double tv = ops.target[1 * nyp + kk];
+        yfv = padd(yfv, pmul(buf, make_double2(V0.x - tv, V0.y - tv)));
+      }
+      fld[6] = yfv;
+    }
+    // CFL maxima for this plane (check_cfl :211-213)
+    if (a.cflbits) {
+      double au = act ? pabsmax(padd(U0, U1)) : 0.0;
+      double av = act ? pabsmax(V0) : 0.0;
+      double aw = act ? pabsmax(padd(W0, W1)) : 0.0;
+      if (LIN && act) {
+        au = fmax(au, pabsmax(padd(BU0, BU1)));
+        av = fmax(av, pabsmax(BV0));
+        aw = fmax(aw, pabsmax(padd(BW0, BW1)));
+      }
+      au = wave_max(au); av = wave_max(av); aw = wave_max(aw);
+      if ((tid & 63) == 0) {
+        unsigned long long* cb = a.cflbits + 3 * kk;
+        atomicMax(cb + 0, (unsigned long long)__double_as_longlong(au));
+        atomicMax(cb + 1, (unsigned long long)__double_as_longlong(av));
+        atomicMax(cb + 2, (unsigned long long)__double_as_longlong(aw));
+      }
+    }
+    // forward x transforms of the nf fields
+    if (act) {
+#pragma unroll
+      for (int f = 0; f < 8; ++f)
+        if (f < nf) lds[(f * RZ + r) * SS + n] = fld[f];
+    }
+    __syncthreads();
+    fft_batch<-1, 8>(lds, SS, nf * RZ, g.px, tw, tid, nthr);
+    if (act) {
+      // outputs kx = n (and kx = M for an undealiased x grid)
+      for (int kq = n; kq < nxh; kq += M) {
+        const double2 wk = rootx[kq];
+        const double kxv = ops.kx[kq];
+        auto X = [&](int f) { return r2c_post_nl(lds + (f * RZ + r) * SS, M, kq, wk); };
+        auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
+        double2 X0 = X(0), X1 = X(1), X2 = X(2), X3 = X(3), X4 = X(4);
+        double2 Au = cadd(X1, ikxm(X0));
+        double2 Aw = cadd(X3, ikxm(X2));
+        const long ab0 = ((long)((0 * nyp + kk) * 2) * nz2 + zz) * kxp + kq;
+        const long ab2 = ((long)((2 * nyp + kk) * 2) * nz2 + zz) * kxp + kq;
+        const long bst = (long)nz2 * kxp;
+        a.AB[ab0] = Au;
+        a.AB[ab0 + bst] = X2;
+        a.AB[ab2] = Aw;
+        a.AB[ab2 + bst] = X4;
+        if (!top) {
+          double2 X5 = X(5), X6 = X(6), X7 = X(7);
+          const long ab1 = ((long)((1 * nyp + kk) * 2) * nz2 + zz) * kxp + kq;
+          a.AB[ab1] = cadd(X6, ikxm(X5));
+          a.AB[ab1 + bst] = X7;
+        }
+      }
+    }
+    __syncthreads();
+    // slide the window (:81, :170-171, :198)
+    FU = Fu; FW = Fw; GV = G1;
+    U0 = U1; V0 = V1; W0 = W1; BU0 = BU1; BV0 = BV1; BW0 = BW1;
+  }
+}
+
+void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a0,
+                  hipStream_t s) {
+  XPassArgs a = a0;
+  const int M = g.nx2 / 2;
+  if (a.kb < a.ka) return;
+  // rows per workgroup: largest divisor of nz2 with rz*M <= 640
+  int rz = 1;
+  for (int d = 1; d <= g.nz2; ++d)
+    if (g.nz2 % d == 0 && d * M <= 640) rz = d;
+  a.rz = rz;
+  int nthr = ((rz * M + 63) / 64) * 64;
+  if (nthr > 1024) throw chan_error("x-pass workgroup too large");
+  // staging bound of the in-place FFT: 8*rz sequences of length M
+  for (int i = 0; i < g.px.nst; ++i) {
+    int R = g.px.rad[i];
+    long nbf = (long)8 * rz * (M / R);
+    if (((nbf + nthr - 1) / nthr) * R > 8) throw chan_error("x-pass FFT staging bound");
+  }
+  int nplanes = a.kb - a.ka + 1;
+  int nch = (nplanes + a.chunk - 1) / a.chunk;
+  dim3 grid(g.nz2 / rz, nch);
+  size_t lds = sizeof(double2) * 8 * rz * (M + 1);
+  if (a.base)
+    hipLaunchKernelGGL(k_xpass<true>, grid, dim3(nthr), lds, s, g, tw, ops, a);
+  else
+    hipLaunchKernelGGL(k_xpass<false>, grid, dim3(nthr), lds, s, g, tw, ops, a);
+  CHAN_CHECK(hipGetLastError());
+}

@@ -1,0 +1,584 @@
+// Fourier-space part of the step: the per-(kx,kz) low-storage RK3 /
+// Crank-Nicolson update with fractional-step projection (PFC/pass2.f90:29-352),
+// the Helmholtz/Poisson banded solves (PFC/solve.f90:14-338) and the
+// mean-mode constant-mass-flux pressure gradient (PFC/pass2.f90:356-375).
+//
+// One thread owns one (kx,kz) mode; all state arrays are plane-major with
+// kz fastest, so a wavefront reads 64 consecutive modes of one plane at a
+// time (coalesced).  The y-recurrences stream through three passes:
+//   A (ascending)  : assemble the RHS (explicit diffusion and pressure
+//                    gradient at rk_step 1) and eliminate the u, w (D2H) and
+//                    v (D2N) Helmholtz systems bottom-up,
+//   B (descending) : back-substitute, form the divergence of the provisional
+//                    velocity and eliminate the Neumann Poisson system
+//                    top-down,
+//   C (ascending)  : back-substitute the pressure correction, apply the wall
+//                    extrapolation, project, update the pressure
+//                    (Brown-Cortez-Minion form, :307-309) and assemble the
+//                    next RK register.
+// The eliminated coefficients live in a plane-major scratch array.
+#include "chan_internal.h"
+
+// RK3 coefficients: default-REAL literal quotients widened to double
+// (PFC/pass2.f90:14-18).
+__constant__ double c_rka[3] = {0.0, (double)(-17.0f / 60.0f), (double)(-5.0f / 12.0f)};
+__constant__ double c_rkb[3] = {(double)(8.0f / 15.0f), (double)(5.0f / 12.0f), (double)(3.0f / 4.0f)};
+__constant__ double c_rkc[3] = {(double)(29.0f / 96.0f), (double)(-3.0f / 40.0f), (double)(1.0f / 6.0f)};
+__constant__ double c_rkd[3] = {(double)(37.0f / 160.0f), (double)(5.0f / 24.0f), (double)(1.0f / 6.0f)};
+
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 rmul(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// i*s*a
+__device__ __forceinline__ double2 imul(double s, double2 a) { return make_double2(-s * a.y, s * a.x); }
+
+
+__global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
+  const long nm = (long)a.nxh * a.nz;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int ix = (int)(m / a.nz), iz = (int)(m - (long)ix * a.nz);
+  const int ny = a.ny, nyp = ny + 1;
+  const long MS = nm;
+  const double kxv = ops.kx[ix], kzv = ops.kz[iz];
+  const double k2 = kxv * kxv + kzv * kzv;
+  const bool mean = (k2 == 0.0);
+  const int r = a.rk - 1;
+  const double dt = a.sc->dt;
+  const double re = a.re;
+
+  const double2* __restrict__ NL = a.nl;
+  double2* __restrict__ R = a.R;
+  double2* __restrict__ P = a.P;
+  double2* __restrict__ U = a.vel;
+  double2* __restrict__ S = a.scr;
+  double* __restrict__ SR = reinterpret_cast<double*>(a.scr + 4L * nyp * MS);
+  auto nl = [&](int c, int j) { return NL[((long)c * nyp + j) * MS + m]; };
+  auto rr = [&](int c, int j) { return R[((long)c * nyp + j) * MS + m]; };
+  auto pp = [&](int j) { return P[(long)j * MS + m]; };
+  auto sidx = [&](int c, int j) { return ((long)c * nyp + j) * MS + m; };
+  const double* d2h = ops.d2h;
+  const double* d2n = ops.d2n;
+  const double* dm = ops.dm;
+
+  const double vis_imp = c_rkd[r] * (dt / re);
+  const double rk_press = (c_rkc[r] + c_rkd[r]) * dt;
+  const double coef = c_rkb[r] * dt;
+  const double vis_exp1 = c_rkc[r] * (dt / re);
+  const double alpha = 1.0 + vis_imp * k2;
+
+  const double2 top[3] = {a.cbc[(0L * 3 + 0) * MS + m], a.cbc[(0L * 3 + 1) * MS + m], a.cbc[(0L * 3 + 2) * MS + m]};
+  const double2 bot[3] = {a.cbc[(1L * 3 + 0) * MS + m], a.cbc[(1L * 3 + 1) * MS + m], a.cbc[(1L * 3 + 2) * MS + m]};
+
+  // mean-mode pressure gradient (compute_p_grad, :369-373)
+  double p_grad = a.sc->p_grad;
+  if (mean && a.fix_mass == 1) {
+    double tmp = 0.0;
+    for (int j = 1; j < ny; ++j) tmp = tmp + ops.dyh[j] * (-nl(0, j).x);
+    p_grad = -tmp + 0.5 * (a.dudy[ny - 1] - a.dudy[0]) / re;
+    a.sc->p_grad = p_grad;
+  }
+
+  // ---------------- pass A: RHS and forward elimination ----------------
+  double cH = 0.0, cN = 0.0;
+  double2 fu = cz(), fw = cz(), fv = cz();
+  for (int j = 0; j <= ny; ++j) {
+    double2 cu0 = nl(0, j), cu1 = nl(1, j), cu2 = nl(2, j);
+    cu0 = make_double2(-cu0.x, -cu0.y); cu1 = make_double2(-cu1.x, -cu1.y); cu2 = make_double2(-cu2.x, -cu2.y);
+    double2 R0 = rr(0, j), R1 = rr(1, j), R2 = rr(2, j);
+    double2 v0 = cadd(R0, rmul(coef, cu0));
+    double2 v1 = cadd(R1, rmul(coef, cu1));
+    double2 v2 = cadd(R2, rmul(coef, cu2));
+    if (r == 0) {
+      // add_explicit_diffusive_terms(cv, cru) (:339-351)
+      int j0 = min(max(j - 1, 0), nyp - 3);
+      double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
+      double2 du = cadd(cadd(rmul(a0, rr(0, j0)), rmul(a1, rr(0, j0 + 1))), rmul(a2, rr(0, j0 + 2)));
+      v0 = cadd(v0, rmul(vis_exp1, csub(du, rmul(k2, R0))));
+      double2 dw = cadd(cadd(rmul(a0, rr(2, j0)), rmul(a1, rr(2, j0 + 1))), rmul(a2, rr(2, j0 + 2)));
+      v2 = cadd(v2, rmul(vis_exp1, csub(dw, rmul(k2, R2))));
+      if (j < ny) {
+        int jv = min(max(j - 1, 0), ny - 3);
+        double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+        double2 dv = cadd(cadd(rmul(b0, rr(1, jv)), rmul(b1, rr(1, jv + 1))), rmul(b2, rr(1, jv + 2)));
+        v1 = cadd(v1, rmul(vis_exp1, csub(dv, rmul(k2, R1))));
+      }
+      if (!mean) {  // subtract_pressure_gradient(cv, cp, rk_press) (:54-77)
+        if (j >= 1 && j <= ny - 1) {
+          double2 pj = pp(j);
+          v0 = csub(v0, imul(rk_press * kxv, pj));
+          v2 = csub(v2, imul(rk_press * kzv, pj));
+        }
+        if (j >= 1 && j <= ny - 2) v1 = csub(v1, rmul(rk_press * ops.dn[j], csub(pp(j + 1), pp(j))));
+      }
+    }
+    if (mean) {  // mean pressure gradient (:220-221)
+      v0 = csub(v0, make_double2(a.cosb * rk_press * p_grad, 0.0));
+      v2 = csub(v2, make_double2(a.sinb * rk_press * p_grad, 0.0));
+    }
+    // Helmholtz rows (SOLVE_HELMHOLTZ, Dirichlet walls, :174-213)
+    double ah, bh, ch;
+    if (j == 0) { ah = 0.0; bh = 1.0; ch = 0.0; v0 = bot[0]; v2 = bot[2]; }
+    else if (j == ny) { ah = 0.0; bh = 1.0; ch = 0.0; v0 = top[0]; v2 = top[2]; }
+    else {
+      ah = -vis_imp * d2h[3 * j]; bh = -vis_imp * d2h[3 * j + 1] + alpha; ch = -vis_imp * d2h[3 * j + 2];
+    }
+    double bp = (j == 0) ? bh : bh - ah * cH;
+    double2 gu = (j == 0) ? v0 : csub(v0, rmul(ah, fu));
+    double2 gw = (j == 0) ? v2 : csub(v2, rmul(ah, fw));
+    double rb = 1.0 / bp;
+    cH = ch * rb;
+    fu = rmul(rb, gu);
+    fw = rmul(rb, gw);
+    S[sidx(0, j)] = fu;
+    S[sidx(1, j)] = fw;
+    SR[sidx(0, j)] = cH;
+    if (j < ny && !mean) {
+      double an, bn, cn;
+      if (j == 0) { an = 0.0; bn = 1.0; cn = 0.0; v1 = bot[1]; }
+      else if (j == ny - 1) { an = 0.0; bn = 1.0; cn = 0.0; v1 = top[1]; }
+      else { an = -vis_imp * d2n[3 * j]; bn = -vis_imp * d2n[3 * j + 1] + alpha; cn = -vis_imp * d2n[3 * j + 2]; }
+      double bq = (j == 0) ? bn : bn - an * cN;
+      double2 gv = (j == 0) ? v1 : csub(v1, rmul(an, fv));
+      double rq = 1.0 / bq;
+      cN = cn * rq;
+      fv = rmul(rq, gv);
+      S[sidx(2, j)] = fv;
+      SR[sidx(1, j)] = cN;
+    }
+  }
+
+  // ---------------- pass B: back substitution + Poisson elimination ----------------
+  const double tmpc = 1.0 / rk_press;
+  const double alp = -k2;
+  double2 xu1 = cz(), xw1 = cz(), xv1 = cz();  // values at j+1
+  double ap = 0.0;                              // a'_{i+1} of the reversed Poisson elimination
+  double2 fp = cz();
+  auto poisson_row = [&](int i, double& a_, double& b_, double& c_) {
+    const double* dh = ops.dh; const double* dn = ops.dn;
+    if (i == 0) { a_ = 0.0; b_ = -dn[0]; c_ = dn[0]; }
+    else if (i == 1) { a_ = dh[1] * dn[0]; b_ = -dh[1] * dn[0] - dh[1] * dn[1] + alp; c_ = dh[1] * dn[1]; }
+    else if (i == ny - 1) {
+      a_ = dh[ny - 1] * dn[ny - 2]; b_ = -dh[ny - 1] * dn[ny - 2] - dh[ny - 1] * dn[ny - 1] + alp;
+      c_ = dh[ny - 1] * dn[ny - 1];
+    } else if (i == ny) { a_ = -dn[ny - 1]; b_ = dn[ny - 1]; c_ = 0.0; }
+    else { a_ = d2h[3 * i]; b_ = d2h[3 * i + 1] + alp; c_ = d2h[3 * i + 2]; }
+  };
+  auto poisson_elim = [&](int i, double2 rhs) {  // reversed Thomas, row i (descending)
+    double a_, b_, c_;
+    poisson_row(i, a_, b_, c_);
+    double bp2 = (i == ny) ? b_ : b_ - c_ * ap;
+    double2 g = (i == ny) ? rhs : csub(rhs, rmul(c_, fp));
+    double rb2 = 1.0 / bp2;
+    ap = a_ * rb2;
+    fp = rmul(rb2, g);
+    S[sidx(3, i)] = fp;
+    SR[sidx(0, i)] = ap;
+  };
+  for (int j = ny; j >= 0; --j) {
+    double2 xu, xw, xv;
+    if (j == ny) { xu = S[sidx(0, j)]; xw = S[sidx(1, j)]; }
+    else {
+      double c = SR[sidx(0, j)];
+      xu = csub(S[sidx(0, j)], rmul(c, xu1));
+      xw = csub(S[sidx(1, j)], rmul(c, xw1));
+    }
+    if (mean) xv = (j < ny) ? bot[1] : cz();  // (:261-263)
+    else if (j == ny) xv = cz();
+    else if (j == ny - 1) xv = S[sidx(2, j)];
+    else xv = csub(S[sidx(2, j)], rmul(SR[sidx(1, j)], xv1));
+    S[sidx(0, j)] = xu;
+    S[sidx(1, j)] = xw;
+    S[sidx(2, j)] = xv;
+    if (!mean && j + 1 <= ny) {
+      int i = j + 1;
+      double2 rhs = cz();
+      if (i <= ny - 1) {  // strip_divergence rhs (:28-31)
+        double2 d = cadd(cadd(imul(kxv, xu1), rmul(ops.dh[i], csub(xv1, xv))), imul(kzv, xw1));
+        rhs = rmul(tmpc, d);
+      }
+      poisson_elim(i, rhs);
+    }
+    xu1 = xu; xw1 = xw; xv1 = xv;
+  }
+  if (!mean) poisson_elim(0, cz());
+
+  // ---------------- pass C1: pressure-correction back substitution ----------------
+  if (!mean) {
+    double2 ph = S[sidx(3, 0)];
+    S[sidx(3, 0)] = ph;
+    double2 p1 = cz(), p2 = cz(), pm2 = cz(), pm1 = cz();
+    for (int i = 1; i <= ny; ++i) {
+      ph = csub(S[sidx(3, i)], rmul(SR[sidx(0, i)], ph));
+      S[sidx(3, i)] = ph;
+      if (i == 1) p1 = ph;
+      if (i == 2) p2 = ph;
+      if (i == ny - 2) pm2 = ph;
+      if (i == ny - 1) pm1 = ph;
+    }
+    // wall re-extrapolation (SOLVE_P :335-336), zero wall data
+    double2 pb = make_double2((0.0 - dm[1] * p1.x - dm[2] * p2.x) / dm[0], (0.0 - dm[1] * p1.y - dm[2] * p2.y) / dm[0]);
+    double2 pt = make_double2((0.0 - dm[3] * pm2.x - dm[4] * pm1.x) / dm[5],
+                              (0.0 - dm[3] * pm2.y - dm[4] * pm1.y) / dm[5]);
+    S[sidx(3, 0)] = pb;
+    S[sidx(3, ny)] = pt;
+  }
+
+  // ---------------- pass C2: projection, pressure update, next RK register ----------------
+  const bool more = (a.rk < 3);
+  const double coef2 = more ? dt * c_rka[r + 1] : 0.0;
+  const double vis_exp2 = more ? (dt / re) * c_rkc[r + 1] : 0.0;
+  const double rk_press2 = more ? (c_rkc[r + 1] + c_rkd[r + 1]) * dt : 0.0;
+  double2 w0[4][3];   // cv at rows i-3..i
+  double2 Pw[2];      // P_new at rows i-1, i
+  for (int t = 0; t < 4; ++t) for (int c = 0; c < 3; ++c) w0[t][c] = cz();
+  Pw[0] = Pw[1] = cz();
+  auto phi = [&](int i) { return S[sidx(3, i)]; };
+  auto buildR = [&](int j, const double2* cvj, const double2 (*st)[3], const double2 (*stv)[3], double2 Pj,
+                    double2 Pj1) {
+    // cvj: cv at row j; st: three rows for the D2H stencil; stv: three rows for D2N
+    double2 cu[3];
+    for (int c = 0; c < 3; ++c) { double2 t = nl(c, j); cu[c] = make_double2(-t.x, -t.y); }
+    double2 out[3];
+    for (int c = 0; c < 3; ++c) out[c] = cadd(cvj[c], rmul(coef2, cu[c]));
+    double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
+    double2 du = cadd(cadd(rmul(a0, st[0][0]), rmul(a1, st[1][0])), rmul(a2, st[2][0]));
+    double2 dw = cadd(cadd(rmul(a0, st[0][2]), rmul(a1, st[1][2])), rmul(a2, st[2][2]));
+    out[0] = cadd(out[0], rmul(vis_exp2, csub(du, rmul(k2, cvj[0]))));
+    out[2] = cadd(out[2], rmul(vis_exp2, csub(dw, rmul(k2, cvj[2]))));
+    if (j < ny) {
+      double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+      double2 dv = cadd(cadd(rmul(b0, stv[0][1]), rmul(b1, stv[1][1])), rmul(b2, stv[2][1]));
+      out[1] = cadd(out[1], rmul(vis_exp2, csub(dv, rmul(k2, cvj[1]))));
+    }
+    if (!mean) {
+      if (j >= 1 && j <= ny - 1) {
+        out[0] = csub(out[0], imul(rk_press2 * kxv, Pj));
+        out[2] = csub(out[2], imul(rk_press2 * kzv, Pj));
+      }
+      if (j >= 1 && j <= ny - 2) out[1] = csub(out[1], rmul(rk_press2 * ops.dn[j], csub(Pj1, Pj)));
+    }
+    for (int c = 0; c < 3; ++c) R[((long)c * nyp + j) * MS + m] = out[c];
+  };
+  const double alpha2 = 1.0 + vis_imp * k2;
+  for (int i = 0; i <= ny; ++i) {
+    double2 cv[3];
+    cv[0] = S[sidx(0, i)];
+    cv[1] = S[sidx(1, i)];  // w stored in slot 1
+    cv[2] = S[sidx(2, i)];  // v stored in slot 2
+    // reorder to (u, v, w)
+    double2 cu_ = cv[0], cw_ = cv[1], cvv = cv[2];
+    cv[0] = cu_; cv[1] = cvv; cv[2] = cw_;
+    double2 pn;
+    if (!mean) {
+      double2 f0 = phi(i);
+      if (i >= 1 && i <= ny - 1) {
+        cv[0] = csub(cv[0], imul(rk_press * kxv, f0));
+        cv[2] = csub(cv[2], imul(rk_press * kzv, f0));
+      }
+      if (i >= 1 && i <= ny - 2) cv[1] = csub(cv[1], rmul(rk_press * ops.dn[i], csub(phi(i + 1), f0)));
+      int j0 = min(max(i - 1, 0), nyp - 3);
+      double2 d2 = cadd(cadd(rmul(d2h[3 * i], phi(j0)), rmul(d2h[3 * i + 1], phi(j0 + 1))),
+                        rmul(d2h[3 * i + 2], phi(j0 + 2)));
+      pn = csub(cadd(pp(i), rmul(alpha2, f0)), rmul(vis_imp, d2));
+    } else {
+      pn = cz();
+    }
+    for (int c = 0; c < 3; ++c) U[((long)c * nyp + i) * MS + m] = cv[c];
+    P[(long)i * MS + m] = pn;
+    // slide windows
+    for (int t = 0; t < 3; ++t) for (int c = 0; c < 3; ++c) w0[t][c] = w0[t + 1][c];
+    for (int c = 0; c < 3; ++c) w0[3][c] = cv[c];
+    Pw[0] = Pw[1]; Pw[1] = pn;
+    if (mean) {  // d_um_dy update from the new mean profile (:84-89)
+      if (i >= 2 && i <= ny - 1) a.dudy[i - 1] = (ops.dn[i - 1] * (w0[3][0].x - w0[2][0].x));
+      if (i == 2) a.dudy[0] = dm[0] * w0[1][0].x + dm[1] * w0[2][0].x + dm[2] * w0[3][0].x;
+      if (i == ny) a.dudy[ny - 1] = dm[3] * w0[1][0].x + dm[4] * w0[2][0].x + dm[5] * w0[3][0].x;
+    }
+    if (!more) {
+      for (int c = 0; c < 3; ++c) R[((long)c * nyp + i) * MS + m] = cv[c];
+      continue;
+    }
+    if (i >= 2) {
+      if (i == 2) {  // row 0 (clamped stencils on rows 0..2, no gradient)
+        buildR(0, w0[1], &w0[1], &w0[1], cz(), cz());
+      }
+      // row j = i-1: centred D2H on rows i-2..i; D2N clamped at the last v row
+      const int j = i - 1;
+      const double2 (*stv)[3] = (j == ny - 1) ? &w0[0] : &w0[1];
+      buildR(j, w0[2], &w0[1], stv, Pw[0], Pw[1]);
+    }
+  }
+  if (more) buildR(ny, w0[3], &w0[1], &w0[1], Pw[1], cz());
+}
+
+void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
+  long nm = (long)a.nxh * a.nz;
+  dim3 grid((unsigned)((nm + 63) / 64));
+  hipLaunchKernelGGL(k_solve, grid, dim3(64), 0, s, ops, a);
+  CHAN_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Projection for the initial field (mode 0: remove_initial_divergence,
+// PFC/init_vel.f90:518-536) and the initial pressure (mode 1:
+// solve_continuous_poisson, PFC/init_vel.f90:477-513).  Solves the Poisson
+// system in the reference's direction (SOLVE_P, PFC/solve.f90:269-338).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_project(DevOps ops, int ny, int nxh, int nz,
+                                                const double2* __restrict__ in, double2* out_vel,
+                                                double2* out_p, double2* scr, double coef, int mode,
+                                                const double2* __restrict__ vel, double re) {
+  const long nm = (long)nxh * nz;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
+  const int nyp = ny + 1;
+  const long MS = nm;
+  const double kxv = ops.kx[ix], kzv = ops.kz[iz];
+  const double k2 = kxv * kxv + kzv * kzv;
+  const bool mean = (k2 == 0.0);
+  const double* d2h = ops.d2h; const double* d2n = ops.d2n; const double* dh = ops.dh;
+  const double* dn = ops.dn; const double* dm = ops.dm;
+  auto X = [&](int c, int j) -> double2 {
+    if (mode == 0) return in[((long)c * nyp + j) * MS + m];
+    // mode 1: cru = -NL + (1/Re)(D2 u - k2 u)  (:488-501)
+    double2 t = in[((long)c * nyp + j) * MS + m];
+    double2 v = make_double2(-t.x, -t.y);
+    const double visc = 1.0 / re;
+    auto uu = [&](int cc, int jj) { return vel[((long)cc * nyp + jj) * MS + m]; };
+    if (c != 1) {
+      int j0 = min(max(j - 1, 0), nyp - 3);
+      double2 du = cadd(cadd(rmul(d2h[3 * j], uu(c, j0)), rmul(d2h[3 * j + 1], uu(c, j0 + 1))),
+                        rmul(d2h[3 * j + 2], uu(c, j0 + 2)));
+      v = cadd(v, rmul(visc, csub(du, rmul(k2, uu(c, j)))));
+    } else if (j < ny) {
+      int j0 = min(max(j - 1, 0), ny - 3);
+      double2 du = cadd(cadd(rmul(d2n[3 * j], uu(1, j0)), rmul(d2n[3 * j + 1], uu(1, j0 + 1))),
+                        rmul(d2n[3 * j + 2], uu(1, j0 + 2)));
+      v = cadd(v, rmul(visc, csub(du, rmul(k2, uu(1, j)))));
+    }
+    return v;
+  };
+  double2* F = scr;                                        // f' then phi
+  double* C = reinterpret_cast<double*>(scr + (long)nyp * MS);  // c'
+  if (mean) {
+    if (mode == 0) {
+      // set the mean wall-normal velocity to its bottom-wall value (:526-528)
+      double2 v0 = in[((long)1 * nyp + 0) * MS + m];
+      for (int c = 0; c < 3; ++c)
+        for (int j = 0; j <= ny; ++j) {
+          double2 v = in[((long)c * nyp + j) * MS + m];
+          if (c == 1 && j >= 1 && j <= ny - 1) v = v0;
+          out_vel[((long)c * nyp + j) * MS + m] = v;
+        }
+    } else if (mode == 2 && out_vel && out_vel != in) {
+      for (int c = 0; c < 3; ++c)
+        for (int j = 0; j <= ny; ++j)
+          out_vel[((long)c * nyp + j) * MS + m] = in[((long)c * nyp + j) * MS + m];
+    }
+    if (out_p) for (int j = 0; j <= ny; ++j) out_p[(long)j * MS + m] = cz();
+    return;
+  }
+  const double tmp = 1.0 / coef, alp = -k2;
+  // forward elimination (ascending), rows as in SOLVE_P
+  double cprev = 0.0; double2 fprev = cz();
+  for (int i = 0; i <= ny; ++i) {
+    double a_, b_, c_;
+    if (i == 0) { a_ = 0.0; b_ = -dn[0]; c_ = dn[0]; }
+    else if (i == 1) { a_ = dh[1] * dn[0]; b_ = -dh[1] * dn[0] - dh[1] * dn[1] + alp; c_ = dh[1] * dn[1]; }
+    else if (i == ny - 1) {
+      a_ = dh[ny - 1] * dn[ny - 2]; b_ = -dh[ny - 1] * dn[ny - 2] - dh[ny - 1] * dn[ny - 1] + alp;
+      c_ = dh[ny - 1] * dn[ny - 1];
+    } else if (i == ny) { a_ = -dn[ny - 1]; b_ = dn[ny - 1]; c_ = 0.0; }
+    else { a_ = d2h[3 * i]; b_ = d2h[3 * i + 1] + alp; c_ = d2h[3 * i + 2]; }
+    double2 rhs = cz();
+    if (i >= 1 && i <= ny - 1) {
+      double2 d = cadd(cadd(imul(kxv, X(0, i)), rmul(dh[i], csub(X(1, i), X(1, i - 1)))), imul(kzv, X(2, i)));
+      rhs = rmul(tmp, d);
+    }
+    double bp = (i == 0) ? b_ : b_ - a_ * cprev;
+    double2 g = (i == 0) ? rhs : csub(rhs, rmul(a_, fprev));
+    double rb = 1.0 / bp;
+    cprev = c_ * rb;
+    fprev = rmul(rb, g);
+    F[(long)i * MS + m] = fprev;
+    C[(long)i * MS + m] = cprev;
+  }
+  // back substitution (descending)
+  double2 x1 = F[(long)ny * MS + m];
+  double2 p1 = cz(), p2 = cz(), pm1 = cz(), pm2 = cz();
+  for (int i = ny - 1; i >= 0; --i) {
+    double2 x = csub(F[(long)i * MS + m], rmul(C[(long)i * MS + m], x1));
+    F[(long)i * MS + m] = x;
+    if (i == 1) p1 = x;
+    if (i == 2) p2 = x;
+    if (i == ny - 2) pm2 = x;
+    if (i == ny - 1) pm1 = x;
+    x1 = x;
+  }
+  double2 pb = make_double2((0.0 - dm[1] * p1.x - dm[2] * p2.x) / dm[0], (0.0 - dm[1] * p1.y - dm[2] * p2.y) / dm[0]);
+  double2 pt = make_double2((0.0 - dm[3] * pm2.x - dm[4] * pm1.x) / dm[5], (0.0 - dm[3] * pm2.y - dm[4] * pm1.y) / dm[5]);
+  F[0 * MS + m] = pb;
+  F[(long)ny * MS + m] = pt;
+  if (out_p) for (int j = 0; j <= ny; ++j) out_p[(long)j * MS + m] = F[(long)j * MS + m];
+  if ((mode == 0 || mode == 2) && out_vel) {
+    // subtract_pressure_gradient(cu, cp, coef) (:72-75)
+    for (int j = 0; j <= ny; ++j) {
+      double2 u = X(0, j), v = X(1, j), w = X(2, j);
+      double2 f0 = F[(long)j * MS + m];
+      if (j >= 1 && j <= ny - 1) {
+        u = csub(u, imul(coef * kxv, f0));
+        w = csub(w, imul(coef * kzv, f0));
+      }
+      if (j >= 1 && j <= ny - 2) v = csub(v, rmul(coef * dn[j], csub(F[(long)(j + 1) * MS + m], f0)));
+      out_vel[((long)0 * nyp + j) * MS + m] = u;
+      out_vel[((long)1 * nyp + j) * MS + m] = v;
+      out_vel[((long)2 * nyp + j) * MS + m] = w;
+    }
+  }
+}
+
+void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
+                    double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
+                    double re, hipStream_t s) {
+  long nm = (long)nxh * nz;
+  dim3 grid((unsigned)((nm + 63) / 64));
+  hipLaunchKernelGGL(k_project, grid, dim3(64), 0, s, ops, ny, nxh, nz, in, out_vel, out_p, scr, coef,
+                     mode, vel_for_visc, re);
+  CHAN_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// CFL number and dt adaptation (check_cfl PFC/pass1.f90:202-250, :106-124)
+// ---------------------------------------------------------------------------
+__global__ void k_cfl_finalize(DevOps ops, chan_params prm, const unsigned long long* cfl,
+                               DevScalars* sc, int step) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ny = prm.ny;
+  const double dt = sc->dt;
+  double cmax = 0.0;
+  int err = 0;
+  for (int k = 2; k <= ny; ++k) {
+    const int kk = k - 1;
+    double amax = __longlong_as_double((long long)cfl[3 * kk + 0]);
+    double vmax = __longlong_as_double((long long)cfl[3 * kk + 1]);
+    double wmax = __longlong_as_double((long long)cfl[3 * kk + 2]);
+    double tempa = 0.5 * amax * (double)prm.nx / (0.55 * prm.lx);
+    double tempb = vmax / (1.73 * fmin(ops.hn[k - 1], ops.hn[k - 2]));
+    double tempc = 0.5 * wmax * (double)prm.nz / (0.55 * prm.lz);
+    double c = (tempa + tempb + tempc) * dt;
+    if (prm.dt_flag == 1) { if (c >= 3.0e2 && !err) err = 2; }
+    else if (c >= 1.15 && !err) err = 1;
+    if (k == 2 || c > cmax) cmax = c;
+  }
+  sc->cfl = cmax;
+  if (err && !sc->err) sc->err = err;
+  if (step >= 0 && prm.dt_flag == 1) {
+    double dt_temp = dt * prm.cfl_max / cmax;
+    sc->dt = fmin(dt_temp, dt + (dt_temp - dt) * dt / (1.0 + dt));
+  }
+}
+
+void launch_cfl_finalize(const DevOps& ops, chan_params prm, unsigned long long* cfl, DevScalars* sc,
+                         int step, hipStream_t s) {
+  hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(64), 0, s, ops, prm, cfl, sc, step);
+  CHAN_CHECK(hipGetLastError());
+}
+
+__global__ void k_fill_zero(unsigned long long* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = 0ull;
+}
+
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+  size_t n = bytes / 8;
+  if (n == 0) return;
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, (unsigned long long*)p, n);
+  CHAN_CHECK(hipGetLastError());
+}
+
+// max |ikx u + dh (v_j - v_{j-1}) + ikz w| over non-mean modes, j = 2..ny
+__global__ __launch_bounds__(256) void k_divmax(DevOps ops, int ny, int nxh, int nz,
+                                                const double2* __restrict__ vel,
+                                                unsigned long long* out) {
+  const long nm = (long)nxh * nz;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0;
+  if (m < nm) {
+    const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
+    const double kxv = ops.kx[ix], kzv = ops.kz[iz];
+    const int nyp = ny + 1;
+    const long MS = nm;
+    if (kxv * kxv + kzv * kzv != 0.0) {
+      for (int j = 1; j <= ny - 1; ++j) {
+        double2 u = vel[((long)0 * nyp + j) * MS + m];
+        double2 v = vel[((long)1 * nyp + j) * MS + m];
+        double2 vm = vel[((long)1 * nyp + j - 1) * MS + m];
+        double2 w = vel[((long)2 * nyp + j) * MS + m];
+        double2 d = cadd(cadd(imul(kxv, u), rmul(ops.dh[j], csub(v, vm))), imul(kzv, w));
+        mx = fmax(mx, sqrt(d.x * d.x + d.y * d.y));
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+void launch_divmax(const DevOps& ops, int ny, int nxh, int nz, const double2* vel,
+                   unsigned long long* out, hipStream_t s) {
+  long nm = (long)nxh * nz;
+  hipLaunchKernelGGL(k_divmax, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, ops, ny, nxh, nz,
+                     vel, out);
+  CHAN_CHECK(hipGetLastError());
+}
+
+// d_um_dy from the mean profile (PFC/init_vel.f90:333-342)
+__global__ void k_dudy(DevOps ops, int ny, int nz, const double2* __restrict__ vel, double* dudy,
+                       long MS) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* dm = ops.dm;
+  auto u = [&](int j) { return vel[(long)j * MS].x; };
+  dudy[0] = dm[0] * u(0) + dm[1] * u(1) + dm[2] * u(2);
+  for (int j = 1; j <= ny - 2; ++j) dudy[j] = ops.dn[j] * (u(j + 1) - u(j));
+  dudy[ny - 1] = dm[3] * u(ny - 2) + dm[4] * u(ny - 1) + dm[5] * u(ny);
+  (void)nz;
+}
+
+void launch_dudy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel, double* dudy,
+                 hipStream_t s) {
+  // vel points at mode (0,0) of component u; planes are nxh*nz modes apart
+  long MS = (long)nxh * nz;
+  hipLaunchKernelGGL(k_dudy, dim3(1), dim3(64), 0, s, ops, ny, nz, vel, dudy, MS);
+  CHAN_CHECK(hipGetLastError());
+}
+
+// Layout conversion between the reference's data2/data4 and the plane-major
+// device arrays.  kind 0: data2(ny+1,3,nz,nxh) -> R[3][ny+1][nxh][nz];
+// kind 1: inverse; kind 2: data4(ny+1,nz,nxh) -> P[ny+1][nxh][nz]; kind 3: inverse.
+__global__ void k_layout(int kind, int ny, int nxh, int nz, const double2* __restrict__ in,
+                         double2* __restrict__ out) {
+  const int nyp = ny + 1;
+  const long nm = (long)nxh * nz;
+  const long ncomp = (kind <= 1) ? 3 : 1;
+  const long tot = ncomp * nyp * nm;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    // e indexes the plane-major array: ((c*nyp + j)*nxh + x)*nz + z
+    long z = e % nz, t = e / nz;
+    long x = t % nxh; t /= nxh;
+    long j = t % nyp; long c = t / nyp;
+    long f;  // index in the Fortran layout
+    if (kind <= 1) f = (((x * nz + z) * 3 + c) * nyp + j);
+    else f = ((x * nz + z) * nyp + j);
+    if (kind == 0 || kind == 2) out[e] = in[f];
+    else out[f] = in[e];
+  }
+}
+
+void launch_layout(int kind, int ny, int nxh, int nz, const double2* in, double2* out, hipStream_t s) {
+  long tot = (long)((kind <= 1) ? 3 : 1) * (ny + 1) * nxh * nz;
+  unsigned blocks = (unsigned)std::min<long>((tot + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_layout, dim3(blocks), dim3(256), 0, s, kind, ny, nxh, nz, in, out);
+  CHAN_CHECK(hipGetLastError());
+}

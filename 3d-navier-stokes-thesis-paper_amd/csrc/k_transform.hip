@@ -1,0 +1,283 @@
+// Batched dealiased x-z transforms on the device (PFC/transform.f90 semantics).
+//
+//  k_zinv : spectral plane (nz, nx/2+1) -> z-inverse rows [nz2][kxpad]
+//           (z padding with the Nyquist split and the x-Nyquist halving of
+//           PFC/transform.f90:345-357, backward c2c of length nz2)
+//  k_xc2r : rows [nz2][kxpad] -> physical real (nx2, nz2) via a half-length
+//           complex FFT (c2r of PFC/transform.f90:396-405)
+//  k_xr2c : physical real rows -> x spectra kept for kx < nx/2+1, optionally
+//           combined as F(fy) + i kx F(fx) (PFC/transform.f90:265-276,581-617)
+//  k_zfwd : x spectra -> forward c2c of length nz2, truncation with the
+//           Nyquist fold, 1/(nx2 nz2) scaling, doubling of the last kx
+//           column and optional + i kz * (second field)
+//           (PFC/transform.f90:316-333, 641-687)
+#include "chan_internal.h"
+
+#define ZT_KX 8
+
+__global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
+                                              const double2* __restrict__ in, long in_ps,
+                                              double2* __restrict__ out, long out_ps) {
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int kx0 = blockIdx.x * ZT_KX;
+  const long P = blockIdx.y;
+  const int nz = g.nz, nz2 = g.nz2, ss = nz2 + 1;
+  const int ncol = min(ZT_KX, g.nxh - kx0);
+  const double2* src = in + P * in_ps;
+  const bool pad = (nz2 > nz) && !g.small;
+  const int h = nz / 2;
+  for (int e = tid; e < ZT_KX * nz2; e += nthr) {
+    int s = e / nz2, t = e - s * nz2;
+    double2 v = make_double2(0.0, 0.0);
+    if (s < ncol) {
+      const double2* col = src + (long)(kx0 + s) * nz;
+      if (pad) {
+        if (t < h) v = col[t];
+        else if (t == h) v = cscale(0.5, col[h]);
+        else if (t == nz2 - h) v = cscale(0.5, cconj(col[h]));
+        else if (t > nz2 - h) v = col[t - (nz2 - nz)];
+        if (g.nx2 > g.nx && kx0 + s == g.nxh - 1) v = cscale(0.5, v);
+      } else {
+        v = col[t];
+      }
+    }
+    lds[s * ss + t] = v;
+  }
+  __syncthreads();
+  fft_batch<1>(lds, ss, ZT_KX, g.pz, tw, tid, nthr);
+  double2* dst = out + P * out_ps;
+  for (int e = tid; e < nz2 * ZT_KX; e += nthr) {
+    int t = e / ZT_KX, s = e - t * ZT_KX;
+    if (s < ncol) dst[(long)t * g.kxpad + kx0 + s] = lds[s * ss + t];
+  }
+}
+
+// Hermitian half-length packing for an unnormalised c2r of length nx2 = 2M:
+// Zc[n] = (Y[n] + conj(Y[M-n])) + i w^n (Y[n] - conj(Y[M-n])), w = exp(2 pi i/nx2),
+// with Im Y[0] and Im Y[M] dropped (FFTW c2r semantics).
+__device__ __forceinline__ double2 c2r_pack(const double2* __restrict__ row, int nxh, int M, int n,
+                                            double2 wn_conjroot) {
+  auto Y = [&](int k) -> double2 {
+    if (k >= nxh) return make_double2(0.0, 0.0);
+    double2 v = row[k];
+    if (k == 0 || k == M) v.y = 0.0;
+    return v;
+  };
+  double2 a = Y(n), b = cconj(Y(M - n));
+  double2 s = cadd(a, b), d = csub(a, b);
+  double2 wd = cmul(wn_conjroot, d);  // w^n * d
+  return make_double2(s.x - wd.y, s.y + wd.x);
+}
+
+// Post-processing of a half-length forward FFT Z (length M) into the real
+// transform X_k = E_k + w^-k O_k, E = (Z_k + conj Z_{M-k})/2, O = -i (Z_k - conj Z_{M-k})/2.
+__device__ __forceinline__ double2 r2c_post(const double2* __restrict__ Zs, int M, int k, double2 wk) {
+  double2 a = Zs[k % M], b = cconj(Zs[(M - k) % M]);
+  double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+  double2 d = csub(a, b);
+  double2 O = make_double2(0.5 * d.y, -0.5 * d.x);
+  return cadd(E, cmul(wk, O));
+}
+
+__global__ __launch_bounds__(512) void k_xc2r(XZGeom g, const double2* __restrict__ tw,
+                                               const double2* __restrict__ Z, long z_ps,
+                                               double* __restrict__ out, long out_ps, int rows) {
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = g.nx2 / 2, ss = M + 1;
+  const long P = blockIdx.y;
+  const int z0 = blockIdx.x * rows;
+  const int nr = min(rows, g.nz2 - z0);
+  const double2* rootx = tw + g.twx;
+  for (int e = tid; e < rows * M; e += nthr) {
+    int r = e / M, n = e - r * M;
+    double2 v = make_double2(0.0, 0.0);
+    if (r < nr) {
+      const double2* row = Z + P * z_ps + (long)(z0 + r) * g.kxpad;
+      v = c2r_pack(row, g.nxh, M, n, cconj(rootx[n]));
+    }
+    lds[r * ss + n] = v;
+  }
+  __syncthreads();
+  fft_batch<1>(lds, ss, rows, g.px, tw, tid, nthr);
+  for (int e = tid; e < nr * M; e += nthr) {
+    int r = e / M, n = e - r * M;
+    double2 v = lds[r * ss + n];
+    double* o = out + P * out_ps + (long)(z0 + r) * g.nx2 + 2 * n;
+    *reinterpret_cast<double2*>(o) = v;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_xr2c(XZGeom g, const double2* __restrict__ tw,
+                                               const double* __restrict__ fy,
+                                               const double* __restrict__ fx,
+                                               const double* __restrict__ kxv, long in_ps,
+                                               double2* __restrict__ out, long out_ps, int rows) {
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = g.nx2 / 2, ss = M + 1;
+  const long P = blockIdx.y;
+  const int z0 = blockIdx.x * rows;
+  const int nr = min(rows, g.nz2 - z0);
+  const int nf = fx ? 2 : 1;
+  const double2* rootx = tw + g.twx;
+  for (int e = tid; e < nf * rows * M; e += nthr) {
+    int q = e / M, n = e - q * M;
+    int f = q / rows, r = q - f * rows;
+    double2 v = make_double2(0.0, 0.0);
+    if (r < nr) {
+      const double* src = (f == 0 ? fy : fx) + P * in_ps + (long)(z0 + r) * g.nx2;
+      v = *reinterpret_cast<const double2*>(src + 2 * n);
+    }
+    lds[q * ss + n] = v;
+  }
+  __syncthreads();
+  fft_batch<-1>(lds, ss, nf * rows, g.px, tw, tid, nthr);
+  for (int e = tid; e < nr * g.nxh; e += nthr) {
+    int r = e / g.nxh, k = e - r * g.nxh;
+    double2 wk = rootx[k];
+    double2 X = r2c_post(lds + r * ss, M, k, wk);
+    if (fx) {
+      double2 Xx = r2c_post(lds + (rows + r) * ss, M, k, wk);
+      double kk = kxv[k];
+      X = make_double2(X.x - kk * Xx.y, X.y + kk * Xx.x);
+    }
+    out[P * out_ps + (long)(z0 + r) * g.kxpad + k] = X;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restrict__ tw,
+                                              const double2* __restrict__ A,
+                                              const double2* __restrict__ B, long ab_ps,
+                                              const double* __restrict__ kzv,
+                                              double2* __restrict__ out, long out_ps,
+                                              int nl_zero_ny) {
+  extern __shared__ double2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int kx0 = blockIdx.x * ZT_KX;
+  const long P = blockIdx.y;
+  const int nz = g.nz, nz2 = g.nz2, ss = nz2 + 1;
+  const int ncol = min(ZT_KX, g.nxh - kx0);
+  double2* dst = out + P * out_ps;
+  if (nl_zero_ny > 0) {
+    // NL planes that the reference leaves zero: plane 1, plane ny+1, and v at
+    // plane ny (PFC/pass1.f90:62,96-99).
+    const int ny = nl_zero_ny;
+    int c = (int)(P / (ny + 1)), j = (int)(P % (ny + 1));
+    if (j == 0 || j == ny || (c == 1 && j == ny - 1)) {
+      for (int e = tid; e < ncol * nz; e += nthr) {
+        int s = e / nz, q = e - s * nz;
+        dst[(long)(kx0 + s) * nz + q] = make_double2(0.0, 0.0);
+      }
+      return;
+    }
+  }
+  const int nf = B ? 2 : 1;
+  for (int e = tid; e < nf * nz2 * ZT_KX; e += nthr) {
+    int f = e / (nz2 * ZT_KX);
+    int e2 = e - f * nz2 * ZT_KX;
+    int t = e2 / ZT_KX, s = e2 - t * ZT_KX;
+    double2 v = make_double2(0.0, 0.0);
+    if (s < ncol) v = (f == 0 ? A : B)[P * ab_ps + (long)t * g.kxpad + kx0 + s];
+    lds[(f * ZT_KX + s) * ss + t] = v;
+  }
+  __syncthreads();
+  fft_batch<-1>(lds, ss, nf * ZT_KX, g.pz, tw, tid, nthr);
+  const int h = nz / 2, off = nz2 - nz;
+  for (int e = tid; e < ncol * nz; e += nthr) {
+    int s = e / nz, q = e - s * nz;
+    const double2* a = lds + s * ss;
+    double2 val;
+    if (g.small) {
+      double sc = 1.0 / (double)(g.nx * g.nz);
+      val = cscale(sc, a[q]);
+    } else {
+      const double scale = 1.0 / (double)(g.nx2 * g.nz2);
+      const double f = (kx0 + s == g.nxh - 1) ? 2.0 * scale : scale;
+      double2 fa;
+      if (q < h) fa = a[q];
+      else if (q == h) fa = cadd(a[h], cconj(a[off + h]));
+      else fa = a[q + off];
+      val = cscale(f, fa);
+      if (B) {
+        const double2* b = lds + (ZT_KX + s) * ss;
+        double2 fb;
+        if (q < h) fb = b[q];
+        else if (q == h) fb = cadd(b[h], cconj(b[off + h]));
+        else fb = b[q + off];
+        fb = cscale(f, fb);
+        double kz = kzv[q];
+        val = make_double2(val.x - kz * fb.y, val.y + kz * fb.x);
+      }
+    }
+    dst[(long)(kx0 + s) * nz + q] = val;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static void check_stage_fit(const FFTPlan& p, int nseq, int nthr) {
+  for (int i = 0; i < p.nst; ++i) {
+    int R = p.rad[i];
+    long nbf = (long)nseq * (p.n / R);
+    long per = (nbf + nthr - 1) / nthr;
+    if (per * R > FFT_MAXV) throw chan_error("FFT batch too large for workgroup");
+  }
+}
+
+void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
+                 long out_ps, int nplanes, hipStream_t s) {
+  dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
+  const int nthr = 256;
+  check_stage_fit(g.pz, ZT_KX, nthr);
+  size_t lds = sizeof(double2) * ZT_KX * (g.nz2 + 1);
+  hipLaunchKernelGGL(k_zinv, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps);
+  CHAN_CHECK(hipGetLastError());
+}
+
+static int rows_for(int M, int nz2, int nf) {
+  int best = 1;
+  for (int d = 1; d <= nz2; ++d)
+    if (nz2 % d == 0 && d * M <= 512 / nf) best = d;
+  return best;
+}
+
+void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps, double* out,
+                 long out_ps, int nplanes, hipStream_t s) {
+  const int M = g.nx2 / 2;
+  int rows = rows_for(M, g.nz2, 1);
+  int nthr = std::min(512, std::max(64, ((rows * M + 63) / 64) * 64));
+  check_stage_fit(g.px, rows, nthr);
+  dim3 grid(g.nz2 / rows, nplanes);
+  size_t lds = sizeof(double2) * rows * (M + 1);
+  hipLaunchKernelGGL(k_xc2r, grid, dim3(nthr), lds, s, g, tw, Z, z_ps, out, out_ps, rows);
+  CHAN_CHECK(hipGetLastError());
+}
+
+void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const double* fx,
+                 const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s) {
+  const int M = g.nx2 / 2;
+  const int nf = fx ? 2 : 1;
+  int rows = rows_for(M, g.nz2, nf);
+  int nthr = std::min(512, std::max(64, ((nf * rows * M + 63) / 64) * 64));
+  check_stage_fit(g.px, nf * rows, nthr);
+  dim3 grid(g.nz2 / rows, nplanes);
+  size_t lds = sizeof(double2) * nf * rows * (M + 1);
+  hipLaunchKernelGGL(k_xr2c, grid, dim3(nthr), lds, s, g, tw, fy, fx, kx, in_ps, out, out_ps, rows);
+  CHAN_CHECK(hipGetLastError());
+}
+
+void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
+                 const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
+                 hipStream_t s) {
+  dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
+  const int nf = B ? 2 : 1;
+  const int nthr = 256 * nf;
+  check_stage_fit(g.pz, nf * ZT_KX, nthr);
+  size_t lds = sizeof(double2) * nf * ZT_KX * (g.nz2 + 1);
+  hipLaunchKernelGGL(k_zfwd, grid, dim3(nthr), lds, s, g, tw, A, B, ab_ps, kz, out, out_ps,
+                     nl_zero_ny);
+  CHAN_CHECK(hipGetLastError());
+}

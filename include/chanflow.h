@@ -1,0 +1,184 @@
+/*
+ * chanflow.h -- C-ABI of the MI355X-native channel-flow time step.
+ *
+ * Drop-in boundary for the pseudo-spectral RK3/Crank-Nicolson step of the
+ * MPI-Fortran channel solver.  The kept Fortran driver (main_mpi.f90, setup,
+ * read_write, init_vel, init_jet_buffer) binds these entry points through
+ * iso_c_binding in replacement modules that keep the reference's module names
+ * and PUBLIC lists (transform, pass1, pass2, solve, pass1_linearized).
+ *
+ * Conventions
+ *  - Every entry point returns int status: 0 = OK, nonzero = error; the
+ *    message is available from chan_last_error().  The reference has no
+ *    error codes and STOPs (e.g. PFC/pass1.f90:243); the Fortran wrappers
+ *    print chan_last_error() and STOP.
+ *  - Complex arrays are interleaved double pairs (Fortran COMPLEX(8)); all
+ *    host arrays use the reference's column-major layouts.  Calls copy host
+ *    arrays and never retain host pointers.
+ *  - One HIP stream per context; calls are stream-ordered and synchronise
+ *    before returning host-visible outputs.  A context is not thread-safe.
+ *  - PFC/ abbreviates ParallelFortran-Code/ of the reference.
+ */
+#ifndef CHANFLOW_H
+#define CHANFLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chan_ctx chan_ctx;
+
+/* Run parameters: the subset of runparms/gridparms the step reads
+ * (PFC/runparms.f90:22-34, PFC/gridparms.f90:82-85). */
+typedef struct chan_params {
+  int nx, ny, nz;        /* spectral grid: nx/2+1 stored kx, ny+1 planes, nz kz  */
+  int nx2, nz2;          /* dealiased physical grid (3/2 rule)                     */
+  double lx, lz;         /* box lengths (already multiplied by pi if scaled)       */
+  double re;             /* Reynolds number                                        */
+  double beta;           /* mean pressure-gradient angle in radians                */
+  int fix_mass;          /* 1: constant mass flux (PFC/pass2.f90:219)              */
+  int dt_flag;           /* 1: CFL-adaptive dt (PFC/pass1.f90:120-123)             */
+  double cfl_max;        /* target CFL for dt_flag == 1 (PFC/setup.f90:1378)       */
+  double ctrlu, ctrlw;   /* wall feedback gains (PFC/setup.f90:61-62)              */
+  int bl_grid;           /* must be 0: channel (Dirichlet) walls                   */
+} chan_params;
+
+/* Operators produced by define_coordinates/define_parameters
+ * (PFC/setup.f90:541-754, 1440-1466), passed verbatim from the Fortran
+ * globals so both sides use bit-identical operators.  buffer and
+ * target_inflow may be NULL (fringe off: both identically zero). */
+typedef struct chan_operators {
+  const double* d2h;            /* d2h(3, ny+1)           */
+  const double* d2n;            /* d2n(3, ny)             */
+  const double* dm;             /* dm(3, 4)               */
+  const double* dh;             /* dh(ny+1)               */
+  const double* dn;             /* dn(ny)                 */
+  const double* hn;             /* hn(ny)                 */
+  const double* iu;             /* iu(2, ny)              */
+  const double* dyh;            /* dyh(ny+1)              */
+  const double* dy;             /* dy(ny+1)               */
+  const double* kx;             /* kx(nx/2+1)             */
+  const double* kz;             /* kz(nz)                 */
+  const double* buffer;         /* buffer(nx2, nz2)       */
+  const double* target_inflow;  /* target_inflow(ny+1, 3) */
+} chan_operators;
+
+/* Context lifetime.  Replaces the allocation of data1/data2/data4 and the
+ * FFTW plans (PFC/setup.f90:441-459, PFC/transform.f90:60-160). */
+int chan_create(const chan_params* prm, chan_ctx** out);
+int chan_destroy(chan_ctx* ctx);
+const char* chan_last_error(const chan_ctx* ctx);
+/* Version / capability string, e.g. "chanflow 1 gfx950 fp64". */
+const char* chan_version(void);
+
+/* Upload operators (see chan_operators). */
+int chan_upload_operators(chan_ctx* ctx, const chan_operators* ops);
+/* Build the channel operators on the host exactly as define_coordinates /
+ * define_parameters do (tanh grid with the given stretching, which the
+ * reference hard-codes as 2.75 at PFC/setup.f90:545) and upload them.  The
+ * arrays are also returned when out_* is non-NULL (sizes as in
+ * chan_operators; y has ny entries, yh ny+1). */
+int chan_build_operators(chan_ctx* ctx, double stretch, double* out_y, double* out_yh);
+
+/* State transfer in the reference's layouts (PFC/data.f90:140-141, shapes
+ * PFC/setup.f90:447-449): data1 complex(nz, nx/2+1, ny+1, 3), data2
+ * complex(ny+1, 3, nz, nx/2+1), data4 complex(ny+1, nz, nx/2+1).  NULL
+ * pointers are skipped.  chan_get_state's data1 is the NL term between
+ * chan_nonlinear and chan_rhs_solve, the velocity otherwise (as in the
+ * reference, where nonlinear overwrites data1 in place). */
+int chan_set_state(chan_ctx* ctx, const double* data1, const double* data2, const double* data4);
+int chan_get_state(chan_ctx* ctx, double* data1, double* data2, double* data4);
+/* Partial download of planes k0..k1 (1-based, inclusive) of data1 component
+ * comp (1..3) into out(nz, nx/2+1, k1-k0+1). */
+int chan_get_planes(chan_ctx* ctx, int comp, int k0, int k1, double* out);
+
+/* Module scalars dt, t_total, p_grad and d_um_dy(ny) (PFC/runparms.f90:23,
+ * PFC/gridparms.f90:90). NULL pointers are skipped. */
+int chan_set_scalars(chan_ctx* ctx, const double* dt, const double* t_total, const double* p_grad,
+                     const double* d_um_dy);
+int chan_get_scalars(chan_ctx* ctx, double* dt, double* t_total, double* p_grad, double* d_um_dy);
+
+/* Wall boundary data cbc_top/cbc_bot complex(nz, nx/2+1, 3) (PFC/gridparms.f90:113). */
+int chan_set_wall_bc(chan_ctx* ctx, const double* cbc_top, const double* cbc_bot);
+
+/* pass1::nonlinear (PFC/pass1.f90:26-252): data1 (velocity) -> data1 (NL
+ * term).  At rk_step == 1 the CFL number is checked (abort -> error status)
+ * and, with dt_flag == 1 and step >= 0, dt is adapted.  dt_inout and
+ * cfl_out may be NULL. */
+int chan_nonlinear(chan_ctx* ctx, int step, int rk_step, double* dt_inout, double* cfl_out);
+
+/* pass1_linearized::nonlinear_linearized (PFC/pass1_linearized.f90:135-509)
+ * about the base flow set by chan_set_baseflow. */
+int chan_nonlinear_linearized(chan_ctx* ctx, int step, int rk_step, double* dt_inout, double* cfl_out);
+/* baseflow real(nx2, nz2, ny+1, 3) (PFC/data.f90:149, PFC/pass1_linearized.f90:34). */
+int chan_set_baseflow(chan_ctx* ctx, const double* baseflow);
+
+/* pass2::rhs_solve (PFC/pass2.f90:29-180): per-(kx,kz) RK3/CN substep with
+ * projection, pressure update and the constant-mass-flux mean mode.  Uses
+ * the wall data of the last chan_set_wall_bc / chan_boundary call.
+ * p_grad_out may be NULL. */
+int chan_rhs_solve(chan_ctx* ctx, int step, int rk_step, double* p_grad_out);
+
+/* One call = nsteps iterations of the timestep loop (PFC/main_mpi.f90:70-87):
+ * boundary (zero wall data or feedback control), then three
+ * (nonlinear | nonlinear_linearized, rhs_solve) substeps and t_total += dt,
+ * all device-resident with no host synchronisation.  step0 is the step
+ * number of the first iteration. linearized != 0 uses the base flow. */
+int chan_step(chan_ctx* ctx, int step0, int nsteps, int linearized);
+
+/* read_write::boundary (PFC/read_write.f90:1465-1532) on the device. */
+int chan_boundary(chan_ctx* ctx);
+
+/* init_vel::remove_initial_divergence (PFC/init_vel.f90:518-536) and the
+ * d_um_dy initialisation (:333-342), then data2 = data1 and
+ * compute_initial_pressure (:437-514). linearized != 0 uses the base flow. */
+int chan_initialize_flow(chan_ctx* ctx, int linearized);
+int chan_remove_divergence(chan_ctx* ctx);
+int chan_compute_initial_pressure(chan_ctx* ctx, int linearized);
+
+/* Host-pointer variants of the transform module (PFC/transform.f90:6-13)
+ * for nplanes consecutive planes.  cdata complex(nz, nx/2+1, nplanes),
+ * data real(nx2, nz2, nplanes). */
+int chan_xz_fft_complex_to_real(chan_ctx* ctx, const double* cdata, double* data, int nplanes);
+int chan_xz_fft_real_to_complex(chan_ctx* ctx, const double* data, double* cdata, int nplanes);
+int chan_compute_divergence_of_fluxes(chan_ctx* ctx, const double* datax, const double* datay,
+                                      const double* dataz, double* cdata, int nplanes);
+/* small_fft_* (PFC/transform.f90:950-1000, 1071-1142): data real(nx, nz, nplanes). */
+int chan_small_fft_complex_to_real(chan_ctx* ctx, const double* cdata, double* data, int nplanes);
+int chan_small_fft_real_to_complex(chan_ctx* ctx, const double* data, double* cdata, int nplanes);
+
+/* Host-pointer variants of the solve module (PFC/solve.f90:7-8), run on
+ * the device for nsys independent systems sharing alpha-independent data.
+ * solve_helmholtz: f complex(n, nsys) in/out, alpha(nsys), d2(3, n),
+ * topbc/botbc complex(nsys), n in {ny, ny+1}. */
+int chan_solve_helmholtz(chan_ctx* ctx, int n, int nsys, double* f, const double* alpha, double diff,
+                         const double* d2, const double* topbc, const double* botbc, int top_neumann,
+                         int bot_neumann);
+/* solve_p: p complex(ny+1, nsys) in/out, alpha(nsys), zero wall data. */
+int chan_solve_p(chan_ctx* ctx, int nsys, double* p, const double* alpha);
+/* strip_divergence for every (kx,kz) of a field cu complex(nz, nx/2+1, ny+1, 3)
+ * (data1 layout) with scale coef; cp complex(ny+1, nz, nx/2+1) receives the
+ * pressure correction (may be NULL). */
+int chan_strip_divergence(chan_ctx* ctx, double* cu, double* cp, double coef);
+
+/* arnoldi::compute_time_deriv (PFC/arnoldi.f90:485-567): w = W P_T W^-1 v
+ * with v, w real(nx, nz, ny+1, 3), P_T = tend linearized RK3 steps.  Needs
+ * chan_set_baseflow.  t_total_out (may be NULL) receives T. */
+int chan_linearized_matvec(chan_ctx* ctx, const double* v, double* w, int tend, double* t_total_out);
+
+/* Diagnostics: max discrete divergence over non-mean modes, j = 2..ny
+ * (PFC/pass2.f90:380-401); mass flux Q (PFC/pass2.f90:106-108). */
+int chan_divergence_max(chan_ctx* ctx, double* out);
+int chan_mass_flux(chan_ctx* ctx, double* out);
+
+/* Per-kernel timing with HIP events on the context stream.
+ * kinds: 0 z-inverse, 1 x-pass, 2 z-forward, 3 solve, 4 other. */
+int chan_set_timing(chan_ctx* ctx, int enable);
+int chan_get_kernel_times(chan_ctx* ctx, double* ms_total, long* counts, int nkinds);
+/* Synchronise the context stream and report a deferred device error. */
+int chan_sync(chan_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHANFLOW_H */

@@ -1,0 +1,240 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Floating-point tolerances (FP64 throughout):
+  * single transforms / solves / one nonlinear or rhs_solve call: relative
+    L2 <= 1e-12 (both sides are round-off different restatements of the same
+    arithmetic);
+  * 10 RK3 steps of config 0: relative L2 of u, v, w and of the mean profile
+    <= 1e-10; discrete divergence <= 1e-12.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import chanflow, grid_cfg0, init_cfg0, make_channel, rel_l2
+from oracle import chan_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def g0():
+    return grid_cfg0(O)
+
+
+def _rand_spec(rng, g, npl):
+    C = rng.standard_normal((npl, g.nxh, g.nz)) + 1j * rng.standard_normal((npl, g.nxh, g.nz))
+    return C
+
+
+def test_version_and_create(g0):
+    cf = chanflow()
+    assert "gfx950" in cf.version()
+    ch = make_channel(g0)
+    ch.close()
+
+
+@pytest.mark.parametrize("dims", [(32, 48, 32, 48), (16, 24, 16, 24), (32, 32, 32, 32), (24, 36, 12, 18)])
+def test_xz_transforms(dims):
+    nx, nx2, nz, nz2 = dims
+    g = O.make_grid(nx, 9, nz, nx2, nz2, 2.0, 2.0, 1000.0, stretch=2.75)
+    ch = make_channel(g)
+    rng = np.random.default_rng(1)
+    C = _rand_spec(rng, g, 5)
+    got = ch.xz_c2r(C)
+    ref = O.xz_c2r(g, C)
+    assert rel_l2(got, ref) < TOL1
+    f = rng.standard_normal((5, g.nz2, g.nx2))
+    assert rel_l2(ch.xz_r2c(f), O.xz_r2c(g, f)) < TOL1
+    fx, fy, fz = (rng.standard_normal((3, g.nz2, g.nx2)) for _ in range(3))
+    assert rel_l2(ch.div_fluxes(fx, fy, fz), O.div_fluxes(g, fx, fy, fz)) < TOL1
+    Cs = _rand_spec(rng, g, 4)
+    assert rel_l2(ch.small_c2r(Cs), O.small_c2r(g, Cs)) < TOL1
+    fs = rng.standard_normal((4, g.nz, g.nx))
+    assert rel_l2(ch.small_r2c(fs), O.small_r2c(g, fs)) < TOL1
+    ch.close()
+
+
+def test_solvers(g0):
+    ch = make_channel(g0)
+    rng = np.random.default_rng(2)
+    ns = 37
+    n = g0.ny + 1
+    f = rng.standard_normal((ns, n)) + 1j * rng.standard_normal((ns, n))
+    alpha = 1.0 + rng.random(ns) * 3.0
+    top = rng.standard_normal(ns) + 1j * rng.standard_normal(ns)
+    bot = rng.standard_normal(ns) + 1j * rng.standard_normal(ns)
+    diff = 3.7e-4
+    got = ch.solve_helmholtz(f, alpha, diff, g0.d2h, top, bot)
+    ref = O.solve_helmholtz(g0, n, f.T, alpha, diff, g0.d2h, top, bot).T
+    assert rel_l2(got, ref) < TOL1
+    fv = f[:, :g0.ny]
+    got = ch.solve_helmholtz(fv, alpha, diff, g0.d2n, top, bot)
+    ref = O.solve_helmholtz(g0, g0.ny, fv.T, alpha, diff, g0.d2n, top, bot).T
+    assert rel_l2(got, ref) < TOL1
+    p = rng.standard_normal((ns, n)) + 1j * rng.standard_normal((ns, n))
+    al = -(0.5 + rng.random(ns) * 40.0)
+    got = ch.solve_p(p, al)
+    ref = O.solve_p(g0, p.T, al).T
+    assert rel_l2(got, ref) < 1e-11
+    ch.close()
+
+
+def test_strip_divergence(g0):
+    ch = make_channel(g0)
+    U = init_cfg0(O, g0, seed=3, amp=1e-2)
+    got, cp = ch.strip_divergence(U, 0.37)
+    ref, cpr = O.strip_divergence(g0, U, 0.37)
+    assert rel_l2(got, ref) < TOL1
+    assert rel_l2(cp, O.to_fortran_data4(cpr)) < 1e-11
+    ch.close()
+
+
+def test_nonlinear_single(g0):
+    ch = make_channel(g0)
+    U = O.remove_initial_divergence(g0, init_cfg0(O, g0, seed=4, amp=1e-2))
+    ch.set_state(data1=U)
+    dt0 = 0.01
+    dt, cfl = ch.nonlinear(1, 1, dt=dt0)
+    got = ch.get_data1()
+    ref, dtr, cflr = O.nonlinear(g0, U, step=1, rk_step=1, dt=dt0, return_cfl=True)
+    assert rel_l2(got, ref) < TOL1
+    assert dt == dtr
+    assert abs(cfl - cflr) <= 1e-14 * abs(cflr)
+    ch.close()
+
+
+def test_nonlinear_dt_flag(g0):
+    g = grid_cfg0(O, dt_flag=1)
+    ch = make_channel(g)
+    U = O.remove_initial_divergence(g, init_cfg0(O, g, seed=5, amp=1e-2))
+    ch.set_state(data1=U)
+    dt, cfl = ch.nonlinear(1, 1, dt=0.02)
+    _, dtr, cflr = O.nonlinear(g, U, step=1, rk_step=1, dt=0.02, return_cfl=True)
+    assert abs(dt - dtr) <= 1e-14 * dtr
+    ch.close()
+
+
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_rhs_solve_single(g0, rk):
+    ch = make_channel(g0)
+    rng = np.random.default_rng(10 + rk)
+    st = O.initial_state(g0, init_cfg0(O, g0, seed=6, amp=1e-2), dt=0.01)
+    NL, _ = O.nonlinear(g0, st.U, step=1, rk_step=rk, dt=0.01)
+    st.U = NL
+    st.R = st.R + 1e-3 * (rng.standard_normal(st.R.shape) + 1j * rng.standard_normal(st.R.shape))
+    st.R[1, g0.ny] = 0.0
+    ch.set_state(data1=NL, data2=O.to_fortran_data2(st.R), data4=O.to_fortran_data4(st.P))
+    ch.set_scalars(dt=st.dt, p_grad=st.p_grad, d_um_dy=st.d_um_dy)
+    pg = ch.rhs_solve(1, rk)
+    ref = O.rhs_solve(g0, st.copy(), rk)
+    d1, d2, d4 = ch.get_state()
+    assert rel_l2(d1, ref.U) < TOL1
+    assert rel_l2(d2, O.to_fortran_data2(ref.R)) < TOL1
+    assert rel_l2(d4, O.to_fortran_data4(ref.P)) < 1e-11
+    assert abs(pg - ref.p_grad) <= 1e-12 * abs(ref.p_grad)
+    sc = ch.get_scalars()
+    assert rel_l2(sc["d_um_dy"], ref.d_um_dy) < TOL1
+    ch.close()
+
+
+def test_initialize_flow(g0):
+    ch = make_channel(g0)
+    U0 = init_cfg0(O, g0, seed=7)
+    ch.set_state(data1=U0)
+    ch.initialize_flow()
+    st = O.initial_state(g0, U0, dt=0.01)
+    d1, d2, d4 = ch.get_state()
+    assert rel_l2(d1, st.U) < TOL1
+    assert rel_l2(d2, O.to_fortran_data2(st.R)) < TOL1
+    assert rel_l2(d4, O.to_fortran_data4(st.P)) < 1e-11
+    assert rel_l2(ch.get_scalars()["d_um_dy"], st.d_um_dy) < TOL1
+    ch.close()
+
+
+def test_ten_steps_config0(g0):
+    """Config 0 parity: 10 RK3 steps, rel L2 <= 1e-10, divergence <= 1e-12."""
+    ch = make_channel(g0)
+    U0 = init_cfg0(O, g0, seed=0)
+    dt = 0.01
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=dt, p_grad=-2.0 / g0.re)
+    ch.initialize_flow()
+    st = O.initial_state(g0, U0, dt=dt)
+    ch.step(10, step0=1)
+    for s in range(1, 11):
+        O.rk3_step(g0, st, s)
+    d1 = ch.get_data1()
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
+    assert rel_l2(d1[0, :, 0, 0].real, st.U[0, :, 0, 0].real) <= 1e-10
+    assert ch.divergence_max() <= 1e-12
+    sc = ch.get_scalars()
+    assert abs(sc["t_total"] - st.t_total) <= 1e-14
+    assert abs(sc["p_grad"] - st.p_grad) <= 1e-9 * abs(st.p_grad)
+    ch.close()
+
+
+def test_step_api_matches_phase_calls(g0):
+    """chan_step == the driver's boundary/nonlinear/rhs_solve call sequence."""
+    U0 = init_cfg0(O, g0, seed=8, amp=1e-2)
+    a = make_channel(g0)
+    b = make_channel(g0)
+    for ch in (a, b):
+        ch.set_state(data1=U0)
+        ch.set_scalars(dt=0.01)
+        ch.initialize_flow()
+    a.step(2, step0=1)
+    for s in (1, 2):
+        b.boundary()
+        for rk in (1, 2, 3):
+            b.nonlinear(s, rk)
+            b.rhs_solve(s, rk)
+    assert rel_l2(a.get_data1(), b.get_data1()) < 1e-14
+    a.close(); b.close()
+
+
+def test_feedback_control(g0):
+    """boundary() with nonzero gains (PFC/read_write.f90:1465-1532) inside the step."""
+    g = grid_cfg0(O, ctrlu=0.5, ctrlw=0.3)
+    ch = make_channel(g)
+    U0 = init_cfg0(O, g, seed=9, amp=1e-2)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.01)
+    ch.initialize_flow()
+    st = O.initial_state(g, U0, dt=0.01)
+    ch.step(3, step0=1)
+    for s in (1, 2, 3):
+        O.rk3_step(g, st, s)
+    assert rel_l2(ch.get_data1(), st.U) < 1e-11
+    ch.close()
+
+
+def test_linearized_matvec():
+    """Arnoldi matvec compute_time_deriv on a small grid."""
+    g = O.make_grid(16, 33, 16, 24, 24, 2.0, 2.0, 2800.0, stretch=2.75, fix_mass=0)
+    ch = make_channel(g)
+    base = O.laminar_baseflow(g)
+    ch.set_baseflow(base)
+    dt = O.linearized_dt(g)
+    ch.set_scalars(dt=dt)
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal((3, g.ny + 1, g.nz, g.nx))
+    w, T = ch.linearized_matvec(v, 3)
+    wr, Tr = O.compute_time_deriv(g, v, base, dt, 3)
+    assert rel_l2(w, wr) < 1e-11
+    assert T == Tr
+    ch.close()
+
+
+def test_cfl_abort(g0):
+    """A too-large dt with dt_flag == 0 is reported as an error (PFC/pass1.f90:235-244)."""
+    cf = chanflow()
+    ch = make_channel(g0)
+    ch.set_state(data1=init_cfg0(O, g0, seed=1))
+    with pytest.raises(cf.ChanError, match="CFL IS TOO BIG"):
+        ch.nonlinear(1, 1, dt=1.0)
+    ch.close()
