@@ -77,6 +77,8 @@ void dfree(T*& p) {
 
 void* stage(chan_ctx* c, size_t bytes) {
   if (bytes > c->stage_bytes) {
+    // queued work may still read the old buffer
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
     if (c->d_stage) (void)hipFree(c->d_stage);
     c->d_stage = nullptr;
     c->stage_bytes = 0;
@@ -86,12 +88,39 @@ void* stage(chan_ctx* c, size_t bytes) {
   return c->d_stage;
 }
 
+// All host transfers go through a pinned buffer owned by the context
+// (hipHostMalloc): user arrays are never handed to the DMA engines directly,
+// so the runtime never has to pin pageable memory on the fly.
+void* pinned(chan_ctx* c, size_t bytes) {
+  if (bytes > c->pin_bytes) {
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    c->h_pin = nullptr;
+    c->pin_bytes = 0;
+    size_t want = std::max<size_t>(bytes, 1 << 20);
+    CHAN_CHECK(hipHostMalloc(&c->h_pin, want, hipHostMallocDefault));
+    c->pin_bytes = want;
+  }
+  return c->h_pin;
+}
 void h2d(chan_ctx* c, void* d, const void* h, size_t bytes) {
-  CHAN_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+  if (bytes == 0) return;
+  void* p = pinned(c, bytes);
+  std::memcpy(p, h, bytes);
+  CHAN_CHECK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c->stream));
+  CHAN_CHECK(hipStreamSynchronize(c->stream));
 }
 void d2h(chan_ctx* c, void* h, const void* d, size_t bytes) {
-  CHAN_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (bytes == 0) return;
+  void* p = pinned(c, bytes);
+  CHAN_CHECK(hipMemcpyAsync(p, d, bytes, hipMemcpyDeviceToHost, c->stream));
   CHAN_CHECK(hipStreamSynchronize(c->stream));
+  std::memcpy(h, p, bytes);
+}
+// carve 256-byte aligned sub-buffers out of a staging allocation
+inline char* carve(char*& cur, size_t bytes) {
+  char* r = cur;
+  cur += (bytes + 255) & ~size_t(255);
+  return r;
 }
 void d2d(chan_ctx* c, void* d, const void* s, size_t bytes) {
   CHAN_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -209,8 +238,8 @@ void check_err(chan_ctx* c) {
   d2h(c, &sc, c->d_sc, sizeof(sc));
   if (sc.err) {
     int e = sc.err;
-    int zero = 0;
-    CHAN_CHECK(hipMemcpy(&c->d_sc->err, &zero, sizeof(int), hipMemcpyHostToDevice));
+    sc.err = 0;
+    h2d(c, c->d_sc, &sc, sizeof(sc));
     char buf[160];
     snprintf(buf, sizeof(buf), "%s (max CFL = %.6e)", e == 2 ? "DT IS TOO BIG" : "CFL IS TOO BIG", sc.cfl);
     throw chan_error(buf);
@@ -340,7 +369,7 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     CHAN_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     c->d_tw = (double2*)dalloc(sizeof(double2) * roots.size());
-    CHAN_CHECK(hipMemcpy(c->d_tw, roots.data(), sizeof(double2) * roots.size(), hipMemcpyHostToDevice));
+    h2d(c, c->d_tw, roots.data(), sizeof(double2) * roots.size());
     const long nm = c->nmodes, nyp = c->nyp;
     c->d_vel = (double2*)dalloc(vel_bytes(c));
     c->d_nl = (double2*)dalloc(vel_bytes(c));
@@ -363,7 +392,7 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     DevScalars sc{};
     sc.dt = 0.25 * p.lx / (double)p.nx;       // PFC/setup.f90:1377
     sc.p_grad = -2.0 / p.re;                  // PFC/setup.f90:1384
-    CHAN_CHECK(hipMemcpy(c->d_sc, &sc, sizeof(sc), hipMemcpyHostToDevice));
+    h2d(c, c->d_sc, &sc, sizeof(sc));
     CHAN_CHECK(hipDeviceSynchronize());
   } catch (const chan_error& e) {
     g_create_error = e.msg;
@@ -382,6 +411,7 @@ int chan_destroy(chan_ctx* c) {
   dfree(c->d_cbc); dfree(c->d_dudy); dfree(c->d_sc); dfree(c->d_cfl); dfree(c->d_Z); dfree(c->d_AB);
   dfree(c->d_scr); dfree(c->d_base);
   if (c->d_stage) (void)hipFree(c->d_stage);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -412,7 +442,7 @@ int chan_upload_operators(chan_ctx* c, const chan_operators* o) {
     if (hasbuf) put(o->buffer, nb);
     dfree(c->d_ops);
     c->d_ops = (double*)dalloc(sizeof(double) * h.size());
-    CHAN_CHECK(hipMemcpy(c->d_ops, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    h2d(c, c->d_ops, h.data(), sizeof(double) * h.size());
     double* b = c->d_ops;
     c->ops.d2h = b + at[0]; c->ops.d2n = b + at[1]; c->ops.dm = b + at[2]; c->ops.dh = b + at[3];
     c->ops.dn = b + at[4]; c->ops.hn = b + at[5]; c->ops.iu = b + at[6]; c->ops.dyh = b + at[7];
@@ -638,7 +668,7 @@ int chan_step(chan_ctx* c, int step0, int nsteps, int lin) {
         solve_pass(c, rk);
       }
       hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, c->stream, c->d_sc);
-      CHAN_CHECK(hipGetLastError());
+      LAUNCH_CHECK("k_advance_time");
     }
     c->data1_is_nl = false;
     CHAN_CHECK(hipStreamSynchronize(c->stream));
@@ -763,14 +793,13 @@ int chan_solve_helmholtz(chan_ctx* c, int n, int nsys, double* f, const double* 
     if (n < 3 || nsys < 1) throw chan_error("bad system size");
     const size_t fb = sizeof(double2) * (size_t)n * nsys, ab = sizeof(double) * nsys, db = sizeof(double) * 3 * n,
                  bb = sizeof(double2) * nsys, sb = sizeof(double) * (size_t)n * nsys;
-    char* s = (char*)stage(c, fb + ab + db + 2 * bb + sb);
-    char* p = s;
-    double2* df = (double2*)p; p += fb;
-    double* da = (double*)p; p += ab;
-    double* dd = (double*)p; p += db;
-    double2* dt = (double2*)p; p += bb;
-    double2* db_ = (double2*)p; p += bb;
-    double* scr = (double*)p;
+    char* p = (char*)stage(c, fb + ab + db + 2 * bb + sb + 6 * 256);
+    double2* df = (double2*)carve(p, fb);
+    double* da = (double*)carve(p, ab);
+    double* dd = (double*)carve(p, db);
+    double2* dt = (double2*)carve(p, bb);
+    double2* db_ = (double2*)carve(p, bb);
+    double* scr = (double*)carve(p, sb);
     h2d(c, df, f, fb); h2d(c, da, alpha, ab); h2d(c, dd, d2, db); h2d(c, dt, topbc, bb); h2d(c, db_, botbc, bb);
     launch_helmholtz_batch(n, nsys, df, da, diff, dd, dt, db_, scr, c->stream);
     d2h(c, f, df, fb);
@@ -782,11 +811,13 @@ int chan_solve_p(chan_ctx* c, int nsys, double* pp, const double* alpha) {
     check_ops(c);
     const int n = c->nyp;
     const size_t fb = sizeof(double2) * (size_t)n * nsys, ab = sizeof(double) * nsys, sb = sizeof(double) * (size_t)n * nsys;
-    char* s = (char*)stage(c, fb + ab + sb);
+    char* cur = (char*)stage(c, fb + ab + sb + 3 * 256);
+    char* s = carve(cur, fb);
+    double* da = (double*)carve(cur, ab);
+    double* scr = (double*)carve(cur, sb);
     h2d(c, s, pp, fb);
-    h2d(c, s + fb, alpha, ab);
-    launch_solvep_batch(c->prm.ny, nsys, (double2*)s, (const double*)(s + fb), c->ops, (double*)(s + fb + ab),
-                        c->stream);
+    h2d(c, da, alpha, ab);
+    launch_solvep_batch(c->prm.ny, nsys, (double2*)s, da, c->ops, scr, c->stream);
     d2h(c, pp, s, fb);
   });
 }
@@ -875,8 +906,11 @@ int chan_mass_flux(chan_ctx* c, double* out) {
     std::vector<double2> col(nyp);
     std::vector<double> dyh(nyp);
     const double2* src = c->data1_is_nl ? c->d_nl : c->d_vel;
-    CHAN_CHECK(hipMemcpy2DAsync(col.data(), sizeof(double2), src, sizeof(double2) * nmodes(c), sizeof(double2), nyp,
+    double2* pc = (double2*)pinned(c, sizeof(double2) * nyp);
+    CHAN_CHECK(hipMemcpy2DAsync(pc, sizeof(double2), src, sizeof(double2) * nmodes(c), sizeof(double2), nyp,
                                 hipMemcpyDeviceToHost, c->stream));
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(col.data(), pc, sizeof(double2) * nyp);
     d2h(c, dyh.data(), c->ops.dyh, sizeof(double) * nyp);
     double q = 0.0;
     for (int j = 0; j < nyp; ++j) q += col[j].x * dyh[j];

@@ -14,6 +14,12 @@
     }                                                                              \
   } while (0)
 
+#define LAUNCH_CHECK(name)                                                         \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) throw chan_error(std::string(name) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
 struct chan_error {
   std::string msg;
   explicit chan_error(std::string m) : msg(std::move(m)) {}
@@ -102,6 +108,8 @@ struct chan_ctx {
   // generic staging buffers for host-pointer entry points
   void* d_stage = nullptr;
   size_t stage_bytes = 0;
+  void* h_pin = nullptr;         // pinned host staging (hipHostMalloc)
+  size_t pin_bytes = 0;
   bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
   // options
   int xpass_chunk = 8;

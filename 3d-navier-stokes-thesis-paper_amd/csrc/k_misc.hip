@@ -115,27 +115,27 @@ void launch_helmholtz_batch(int n, int nsys, double2* f, const double* alpha, do
                             hipStream_t s) {
   hipLaunchKernelGGL(k_helmholtz_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, n, nsys, f, alpha, diff,
                      d2, topbc, botbc, scr);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_helmholtz_batch");
 }
 
 void launch_solvep_batch(int ny, int nsys, double2* p, const double* alpha, const DevOps& ops,
                          double* scr, hipStream_t s) {
   hipLaunchKernelGGL(k_solvep_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, ny, nsys, p, alpha, ops.d2h,
                      ops.dh, ops.dn, ops.dm, scr);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_solvep_batch");
 }
 
 void launch_ctrl_law(int npts, const double* p2, const double* p3, double ca, double cb, double add,
                      double sgn, double den, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_ctrl_law, dim3((npts + 255) / 256), dim3(256), 0, s, npts, p2, p3, ca, cb, add,
                      sgn, den, out);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_ctrl_law");
 }
 
 void launch_absmax(long n, const double* x, unsigned long long* out, hipStream_t s) {
   unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 2048);
   hipLaunchKernelGGL(k_absmax, dim3(blocks), dim3(256), 0, s, n, x, out);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_absmax");
 }
 
 void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, const double* dyh,
@@ -143,5 +143,5 @@ void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, cons
   long tot = 3L * (ny + 1) * plane;
   unsigned blocks = (unsigned)std::min<long>((tot + 255) / 256, 4096);
   hipLaunchKernelGGL(k_arnoldi_scale, dim3(blocks), dim3(256), 0, s, ny, plane, x, dy, dyh, vmax, eps, dir);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_arnoldi_scale");
 }

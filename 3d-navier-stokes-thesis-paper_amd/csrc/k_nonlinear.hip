@@ -86,7 +86,7 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
       if (act) lds[(c * RZ + r) * SS + n] = v;
     }
     __syncthreads();
-    fft_batch<1, 8>(lds, SS, 3 * RZ, g.px, tw, tid, nthr);
+    fft_batch<1, 12>(lds, SS, 3 * RZ, g.px, tw, tid, nthr);
     if (act) {
       U = lds[(0 * RZ + r) * SS + n];
       V = lds[(1 * RZ + r) * SS + n];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
         if (f < nf) lds[(f * RZ + r) * SS + n] = fld[f];
     }
     __syncthreads();
-    fft_batch<-1, 8>(lds, SS, nf * RZ, g.px, tw, tid, nthr);
+    fft_batch<-1, 12>(lds, SS, nf * RZ, g.px, tw, tid, nthr);
     if (act) {
       // outputs kx = n (and kx = M for an undealiased x grid)
       for (int kq = n; kq < nxh; kq += M) {
@@ -265,7 +265,7 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
   for (int i = 0; i < g.px.nst; ++i) {
     int R = g.px.rad[i];
     long nbf = (long)8 * rz * (M / R);
-    if (((nbf + nthr - 1) / nthr) * R > 8) throw chan_error("x-pass FFT staging bound");
+    if (((nbf + nthr - 1) / nthr) * R > 12) throw chan_error("x-pass FFT staging bound");
   }
   int nplanes = a.kb - a.ka + 1;
   int nch = (nplanes + a.chunk - 1) / a.chunk;
@@ -275,5 +275,5 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     hipLaunchKernelGGL(k_xpass<true>, grid, dim3(nthr), lds, s, g, tw, ops, a);
   else
     hipLaunchKernelGGL(k_xpass<false>, grid, dim3(nthr), lds, s, g, tw, ops, a);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_xpass<false>");
 }

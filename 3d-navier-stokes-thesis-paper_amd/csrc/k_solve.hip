@@ -315,7 +315,7 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));
   hipLaunchKernelGGL(k_solve, grid, dim3(64), 0, s, ops, a);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_solve");
 }
 
 // ---------------------------------------------------------------------------
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64) void k_project(DevOps ops, int ny, int nxh, int
   const double* d2h = ops.d2h; const double* d2n = ops.d2n; const double* dh = ops.dh;
   const double* dn = ops.dn; const double* dm = ops.dm;
   auto X = [&](int c, int j) -> double2 {
-    if (mode == 0) return in[((long)c * nyp + j) * MS + m];
+    if (mode != 1) return in[((long)c * nyp + j) * MS + m];
     // mode 1: cru = -NL + (1/Re)(D2 u - k2 u)  (:488-501)
     double2 t = in[((long)c * nyp + j) * MS + m];
     double2 v = make_double2(-t.x, -t.y);
@@ -442,10 +442,13 @@ void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* i
                     double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
                     double re, hipStream_t s) {
   long nm = (long)nxh * nz;
+  if (mode < 0 || mode > 2) throw chan_error("k_project: bad mode");
+  if (mode == 1 && !vel_for_visc) throw chan_error("k_project: mode 1 needs the velocity");
+  if (mode != 1 && !out_vel) throw chan_error("k_project: projection needs an output field");
   dim3 grid((unsigned)((nm + 63) / 64));
   hipLaunchKernelGGL(k_project, grid, dim3(64), 0, s, ops, ny, nxh, nz, in, out_vel, out_p, scr, coef,
                      mode, vel_for_visc, re);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_project");
 }
 
 // ---------------------------------------------------------------------------
@@ -482,7 +485,7 @@ __global__ void k_cfl_finalize(DevOps ops, chan_params prm, const unsigned long 
 void launch_cfl_finalize(const DevOps& ops, chan_params prm, unsigned long long* cfl, DevScalars* sc,
                          int step, hipStream_t s) {
   hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(64), 0, s, ops, prm, cfl, sc, step);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_cfl_finalize");
 }
 
 __global__ void k_fill_zero(unsigned long long* p, size_t n) {
@@ -496,7 +499,7 @@ void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
   if (n == 0) return;
   unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, (unsigned long long*)p, n);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_fill_zero");
 }
 
 // max |ikx u + dh (v_j - v_{j-1}) + ikz w| over non-mean modes, j = 2..ny
@@ -531,7 +534,7 @@ void launch_divmax(const DevOps& ops, int ny, int nxh, int nz, const double2* ve
   long nm = (long)nxh * nz;
   hipLaunchKernelGGL(k_divmax, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, ops, ny, nxh, nz,
                      vel, out);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_divmax");
 }
 
 // d_um_dy from the mean profile (PFC/init_vel.f90:333-342)
@@ -551,7 +554,7 @@ void launch_dudy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel,
   // vel points at mode (0,0) of component u; planes are nxh*nz modes apart
   long MS = (long)nxh * nz;
   hipLaunchKernelGGL(k_dudy, dim3(1), dim3(64), 0, s, ops, ny, nz, vel, dudy, MS);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_dudy");
 }
 
 // Layout conversion between the reference's data2/data4 and the plane-major
@@ -580,5 +583,5 @@ void launch_layout(int kind, int ny, int nxh, int nz, const double2* in, double2
   long tot = (long)((kind <= 1) ? 3 : 1) * (ny + 1) * nxh * nz;
   unsigned blocks = (unsigned)std::min<long>((tot + 255) / 256, 8192);
   hipLaunchKernelGGL(k_layout, dim3(blocks), dim3(256), 0, s, kind, ny, nxh, nz, in, out);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_layout");
 }

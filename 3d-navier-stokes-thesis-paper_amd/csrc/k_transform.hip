@@ -234,7 +234,7 @@ void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_
   check_stage_fit(g.pz, ZT_KX, nthr);
   size_t lds = sizeof(double2) * ZT_KX * (g.nz2 + 1);
   hipLaunchKernelGGL(k_zinv, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_zinv");
 }
 
 static int rows_for(int M, int nz2, int nf) {
@@ -253,7 +253,7 @@ void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps
   dim3 grid(g.nz2 / rows, nplanes);
   size_t lds = sizeof(double2) * rows * (M + 1);
   hipLaunchKernelGGL(k_xc2r, grid, dim3(nthr), lds, s, g, tw, Z, z_ps, out, out_ps, rows);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_xc2r");
 }
 
 void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const double* fx,
@@ -266,7 +266,7 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
   dim3 grid(g.nz2 / rows, nplanes);
   size_t lds = sizeof(double2) * nf * rows * (M + 1);
   hipLaunchKernelGGL(k_xr2c, grid, dim3(nthr), lds, s, g, tw, fy, fx, kx, in_ps, out, out_ps, rows);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_xr2c");
 }
 
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
@@ -279,5 +279,5 @@ void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const dou
   size_t lds = sizeof(double2) * nf * ZT_KX * (g.nz2 + 1);
   hipLaunchKernelGGL(k_zfwd, grid, dim3(nthr), lds, s, g, tw, A, B, ab_ps, kz, out, out_ps,
                      nl_zero_ny);
-  CHAN_CHECK(hipGetLastError());
+  LAUNCH_CHECK("k_zfwd");
 }

@@ -1,0 +1,173 @@
+"""Benchmark: FP64 RK3 steps/s of the channel step at 192 x 129 x 192.
+
+Workload (the headline turbulent-channel configuration): Re_c = 3300
+(Re_tau = 180), box 4pi x 2 x 4pi/3, 192 x 129 x 192 Fourier/grid points,
+3/2-dealiased to 288 x 288, tanh-stretched y (2.75), dt = 0.005, constant
+mass flux.  Synthetic seeded input: Reichardt mean profile plus solenoidal
+k^-5/3 fluctuations of rms 0.1.  One "step" = boundary + three
+(nonlinear, rhs_solve) substeps, all device resident.
+
+Prints one JSON line.  `value` is RK3 steps/s of the whole job (one GPU).
+`roofline` is computed for the dominant part of the step from HIP events
+recorded on the library's stream; `cpu_baseline` times the numpy oracle on a
+bounded sample (2 steps) of the same workload on one host core.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "3d-navier-stokes-thesis-paper_amd")
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def load_pkg():
+    if "chanflow" in sys.modules:
+        return sys.modules["chanflow"]
+    spec = importlib.util.spec_from_file_location(
+        "chanflow", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["chanflow"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+HEADLINE = dict(nx=192, ny=129, nz=192, nx2=288, nz2=288, lx_over_pi=4.0, aspect=3.0, re=3300.0,
+                dt=0.005, stretch=2.75, seed=0, rms=0.1, re_tau=180.0)
+
+
+def make_channel(cf, c):
+    import math
+    pi = math.atan2(0.0, -1.0)
+    lx = c["lx_over_pi"] * pi
+    lz = lx / c["aspect"]
+    ch = cf.Channel(c["nx"], c["ny"], c["nz"], c["nx2"], c["nz2"], lx, lz, c["re"], fix_mass=1, dt_flag=0)
+    y, yh = ch.build_operators(c["stretch"])
+    return ch, y, yh, lx, lz
+
+
+def cpu_baseline(c, U0, nsteps=2):
+    """Time the oracle (numpy, one core) on nsteps RK3 steps of the same workload."""
+    sys.path.insert(0, ROOT)
+    from oracle import chan_oracle as O
+    g = O.make_grid(c["nx"], c["ny"], c["nz"], c["nx2"], c["nz2"], c["lx_over_pi"], c["aspect"], c["re"],
+                    stretch=c["stretch"], fix_mass=1)
+    st = O.initial_state(g, U0, dt=c["dt"])
+    t0 = time.perf_counter()
+    for s in range(1, nsteps + 1):
+        O.rk3_step(g, st, s)
+    t1 = time.perf_counter()
+    return nsteps / (t1 - t0), t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run warmup + this many steps (for rocprof), print no JSON")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(the multi-GPU sharding is a design note, not implemented)"}))
+        sys.exit(2)
+
+    cf = load_pkg()
+    import chanflow.synth as synth
+    c = HEADLINE
+    ch, y, yh, lx, lz = make_channel(cf, c)
+    U0 = synth.turbulent_like(c["nx"], c["ny"], c["nz"], yh, y, seed=c["seed"], rms=c["rms"], re_tau=c["re_tau"])
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=c["dt"], t_total=0.0, p_grad=-2.0 / c["re"])
+    ch.initialize_flow()
+    ch.step(args.warmup, step0=1)
+    ch.sync()
+    if args.profile_steps:
+        ch.step(args.profile_steps, step0=args.warmup + 1)
+        ch.sync()
+        print(f"profiled {args.profile_steps} steps", flush=True)
+        return
+
+    # ---- timed region: exactly K steps, synchronised on both sides
+    ch.sync()
+    t0 = time.perf_counter()
+    ch.step(args.steps, step0=args.warmup + 1)
+    ch.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    div = ch.divergence_max()
+
+    # ---- per-kernel durations with HIP events on the library's stream
+    nmeas = max(5, min(20, args.steps // 10))
+    ch.set_timing(True)
+    ch.step(nmeas, step0=args.warmup + args.steps + 1)
+    kt = ch.kernel_times()
+    ch.set_timing(False)
+
+    nyp = c["ny"] + 1
+    nxh = c["nx"] // 2 + 1
+    n_c = c["nz"] * nxh * nyp                      # complex coefficients per component
+    alg_step = 960.0 * n_c                         # bytes per RK3 step (state traffic)
+    alg = {"z-inverse": 48.0 * n_c, "x-pass": 0.0, "z-forward": 48.0 * n_c, "solve": 224.0 * n_c}
+    per_launch = {k: (ms / cnt if cnt else 0.0) for k, (ms, cnt) in kt.items()}
+    nl_ms = per_launch["z-inverse"] + per_launch["x-pass"] + per_launch["z-forward"]
+    parts = {"nl_pass": nl_ms, "solve": per_launch["solve"]}
+    dom = max(parts, key=parts.get)
+    if dom == "nl_pass":
+        bytes_launch, dur_ms, name = 96.0 * n_c, nl_ms, "nl_pass (z-inverse + x-pass + z-forward)"
+    else:
+        bytes_launch, dur_ms, name = alg["solve"], per_launch["solve"], "solve"
+    achieved = bytes_launch / (dur_ms * 1e-3) / 1e9
+    step_gbs = alg_step * steps_per_s / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sps, secs = cpu_baseline(c, U0, args.cpu_steps)
+        cpu = {"value": sps, "unit": "RK3 steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} RK3 steps of the 192x129x192 workload by the numpy oracle "
+                         f"({secs:.1f} s, single-threaded numpy FFT)"}
+
+    out = {
+        "metric": "FP64 RK3 steps/s at 192x129x192 (%HBM roofline)",
+        "value": steps_per_s,
+        "unit": "RK3 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Reichardt profile + k^-5/3 solenoidal fluctuations)",
+        "config": {"workload": "turbulent channel Re_c=3300 (Re_tau=180), box 4pi x 2 x 4pi/3, "
+                               "192x129x192, 3/2 dealiased to 288x288, dt=0.005, constant mass flux",
+                   "nx": 192, "ny": 129, "nz": 192, "nx2": 288, "nz2": 288, "dt": 0.005},
+        "grid_point_steps_per_s": steps_per_s * c["nx"] * c["ny"] * c["nz"],
+        "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_step": alg_step},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": dur_ms},
+        "kernels_ms_per_launch": per_launch,
+        "kernel_launches_measured": {k: v[1] for k, v in kt.items()},
+        "max_divergence": div,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

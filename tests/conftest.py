@@ -13,3 +13,4 @@ if HERE not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libchanflow.so")
+    config.addinivalue_line("markers", "slow: long-running CPU check (oracle known answers)")
