@@ -155,3 +155,188 @@ __device__ void fft_batch(double2* lds, int sstride, int B, const FFTPlan& pl,
     p *= R;
   }
 }
+
+
+// ---------------------------------------------------------------------------
+// Two-pass (four-step) FFT with a compile-time length N = N1 * N2.
+//
+// A sequence is held in LDS as an N1 x N2 array with row stride N2 + 1.
+// Input x[n], n = a*N2 + b, sits at row a, column b.  Pass 1 transforms each
+// column b over a (length N1, in registers) and multiplies by w_N^{b*k1};
+// pass 2 transforms each row k1 over b (length N2).  Both passes read and
+// write the same LDS cells, so the transform is in place with one barrier
+// between the passes.  The result X[k], k = k1 + N1*k2, ends at row k1,
+// column k2 (fft2_out_pos).  With N2 even the odd row stride keeps the
+// 16-byte row-wise accesses of pass 2 free of bank conflicts.
+//
+// The in-register DFTs of length L (4, 6, 8, 9, 12, 16, 18, 24) are built the
+// same way from the radix-2/3/4/8 kernels above; their twiddles w_L^e are read
+// from the length-N root table at the compile-time index e*N/L (uniform
+// scalar loads).
+// ---------------------------------------------------------------------------
+struct Fft2Split { int n1, n2; };
+
+__host__ __device__ constexpr Fft2Split fft2_split(int n) {
+  switch (n) {
+    case 16: return {4, 4};
+    case 24: return {4, 6};
+    case 32: return {4, 8};
+    case 36: return {6, 6};
+    case 48: return {6, 8};
+    case 64: return {8, 8};
+    case 72: return {9, 8};
+    case 96: return {8, 12};
+    case 144: return {12, 12};
+    case 192: return {12, 16};
+    case 288: return {18, 16};
+    default: return {0, 0};
+  }
+}
+
+// LDS cells one sequence occupies: N1*(N2+1) for a compile-time length, n+1
+// for the runtime-planned transform.
+__host__ __device__ constexpr int fft_seq_cells(int n) {
+  return fft2_split(n).n1 ? fft2_split(n).n1 * (fft2_split(n).n2 + 1) : n + 1;
+}
+
+__host__ __device__ constexpr int rdft_first(int L) {
+  return (L % 4 == 0 && L > 8) ? 4 : (L % 3 == 0 ? 3 : 2);
+}
+
+// x * w_L^e (SGN = -1: w = exp(-2 pi i/L); SGN = +1: conjugate), e a constant
+// after unrolling; rt[e*S] = exp(-2 pi i e/L).
+template <int L, int SGN, int S>
+__device__ __forceinline__ double2 rtwiddle(double2 x, int e, const double2* __restrict__ rt) {
+  e %= L;
+  if (e == 0) return x;
+  if (2 * e == L) return make_double2(-x.x, -x.y);
+  if (4 * e == L) return cmuli(x, SGN);
+  if (4 * e == 3 * L) return cmuli(x, -SGN);
+  double2 w = rt[e * S];
+  if (SGN > 0) w = cconj(w);
+  return cmul(x, w);
+}
+
+template <int L, int SGN, int S>
+__device__ __forceinline__ void rdft(double2* v, const double2* __restrict__ rt) {
+  if constexpr (L == 1) {
+  } else if constexpr (L == 2 || L == 3 || L == 4 || L == 8) {
+    dft_r<L, SGN>(v);
+  } else {
+    constexpr int A = rdft_first(L), B = L / A;
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      double2 t[A];
+#pragma unroll
+      for (int a = 0; a < A; ++a) t[a] = v[n2 + B * a];
+      dft_r<A, SGN>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) v[n2 + B * k1] = rtwiddle<L, SGN, S>(t[k1], n2 * k1, rt);
+    }
+    double2 o[L];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      rdft<B, SGN, S * A>(v + B * k1, rt);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) o[k1 + A * k2] = v[B * k1 + k2];
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) v[k] = o[k];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ int fft2_in_pos(int n) {
+  constexpr int N2 = fft2_split(N).n2;
+  return n + n / N2;
+}
+template <int N>
+__device__ __forceinline__ int fft2_out_pos(int k) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2;
+  return (k % N1) * (N2 + 1) + k / N1;
+}
+
+// In-place two-pass FFT of B sequences (sequence s at lds + s*sstride).
+// rl: root table exp(-2 pi i t/N), t < N, in LDS; rg: the same table in
+// global memory (scalar-loaded constants of the register DFTs).  Call with
+// the input visible to all threads (a barrier before); ends with a barrier.
+template <int N, int SGN>
+__device__ __forceinline__ void fft2p(double2* lds, int sstride, int B, const double2* rl,
+                                      const double2* __restrict__ rg, int tid, int nthr) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2, RS = N2 + 1;
+  static_assert(N1 * N2 == N && N1 > 0, "unsupported two-pass length");
+  for (int g = tid; g < B * N2; g += nthr) {
+    const int s = g / N2, b = g - s * N2;
+    double2* p = lds + s * sstride + b;
+    double2 v[N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) v[a] = p[a * RS];
+    rdft<N1, SGN, N2>(v, rg);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      double2 w = rl[b * k1];
+      if (SGN > 0) w = cconj(w);
+      v[k1] = cmul(v[k1], w);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) p[k1 * RS] = v[k1];
+  }
+  __syncthreads();
+  for (int g = tid; g < B * N1; g += nthr) {
+    const int s = g / N1, k1 = g - s * N1;
+    double2* p = lds + s * sstride + k1 * RS;
+    double2 v[N2];
+#pragma unroll
+    for (int b = 0; b < N2; ++b) v[b] = p[b];
+    rdft<N2, SGN, N1>(v, rg);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) p[k2] = v[k2];
+  }
+  __syncthreads();
+}
+
+// Position of input element n / output element k of a sequence: the two-pass
+// layout for NC > 0, natural order for the runtime-planned transform (NC = 0).
+template <int NC>
+__device__ __forceinline__ int fft_in_pos(int n) {
+  if constexpr (NC > 0) return fft2_in_pos<NC>(n);
+  else return n;
+}
+template <int NC>
+__device__ __forceinline__ int fft_out_pos(int k) {
+  if constexpr (NC > 0) return fft2_out_pos<NC>(k);
+  else return k;
+}
+
+// Transform of B sequences: two-pass for NC > 0, runtime plan otherwise.
+template <int NC, int SGN, int MAXV>
+__device__ __forceinline__ void fft_run(double2* lds, int sstride, int B, const FFTPlan& pl,
+                                        const double2* __restrict__ twbase, const double2* rl, int tid,
+                                        int nthr) {
+  if constexpr (NC > 0) fft2p<NC, SGN>(lds, sstride, B, rl, twbase + pl.tw, tid, nthr);
+  else fft_batch<SGN, MAXV>(lds, sstride, B, pl, twbase, tid, nthr);
+}
+
+// Copy the length-n root table into LDS (whole workgroup; caller synchronises).
+__device__ __forceinline__ void load_roots(double2* dst, const double2* __restrict__ src, int n, int tid,
+                                           int nthr) {
+  for (int i = tid; i < n; i += nthr) dst[i] = src[i];
+}
+
+// Dispatch a launcher body over the compile-time lengths: FFT_DISPATCH(n, F)
+// expands to F(n) for a supported n and F(0) otherwise.
+#define FFT_DISPATCH(n, F)        \
+  switch (n) {                    \
+    case 16: F(16); break;        \
+    case 24: F(24); break;        \
+    case 32: F(32); break;        \
+    case 36: F(36); break;        \
+    case 48: F(48); break;        \
+    case 64: F(64); break;        \
+    case 72: F(72); break;        \
+    case 96: F(96); break;        \
+    case 144: F(144); break;      \
+    case 192: F(192); break;      \
+    case 288: F(288); break;      \
+    default: F(0); break;         \
+  }

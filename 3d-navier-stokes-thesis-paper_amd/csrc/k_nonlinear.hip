@@ -31,9 +31,9 @@ __device__ __forceinline__ double2 c2r_pack_nl(const double2* __restrict__ row, 
   return make_double2(s.x - wd.y, s.y + wd.x);
 }
 
-__device__ __forceinline__ double2 r2c_post_nl(const double2* __restrict__ Zs, int M, int k,
-                                               double2 wk) {
-  double2 a = Zs[k % M], b = cconj(Zs[(M - k) % M]);
+template <int MC>
+__device__ __forceinline__ double2 r2c_post_nl(const double2* __restrict__ Zs, int M, int k, double2 wk) {
+  double2 a = Zs[fft_out_pos<MC>(k % M)], b = cconj(Zs[fft_out_pos<MC>((M - k) % M)]);
   double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
   double2 d = csub(a, b);
   double2 O = make_double2(0.5 * d.y, -0.5 * d.x);
@@ -53,11 +53,11 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-template <bool LIN>
+template <bool LIN, int MC>
 __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restrict__ tw, DevOps ops,
                                                 XPassArgs a) {
   extern __shared__ double2 lds[];
-  const int M = g.nx2 / 2, SS = M + 1, RZ = a.rz;
+  const int M = MC > 0 ? MC : g.nx2 / 2, SS = fft_seq_cells(M), RZ = a.rz;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int r = tid / M, n = tid - r * M;
   const bool act = r < RZ;
@@ -68,6 +68,9 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
   const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
   const double2* rootx = tw + g.twx;
   const double2 wconj = cconj(rootx[n < M ? n : 0]);
+  double2* rl = lds + 8 * RZ * SS;
+  if (MC > 0) load_roots(rl, tw + g.px.tw, M, tid, nthr);
+  const int pin = fft_in_pos<MC>(n < M ? n : 0), pout = fft_out_pos<MC>(n < M ? n : 0);
   const long rowoff = (long)zz * kxp;
 
   double2 buf = make_double2(0.0, 0.0);
@@ -83,14 +86,14 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
         const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
         v = c2r_pack_nl(row, nxh, M, n, wconj);
       }
-      if (act) lds[(c * RZ + r) * SS + n] = v;
+      if (act) lds[(c * RZ + r) * SS + pin] = v;
     }
     __syncthreads();
-    fft_batch<1, 12>(lds, SS, 3 * RZ, g.px, tw, tid, nthr);
+    fft_run<MC, 1, 12>(lds, SS, 3 * RZ, g.px, tw, rl, tid, nthr);
     if (act) {
-      U = lds[(0 * RZ + r) * SS + n];
-      V = lds[(1 * RZ + r) * SS + n];
-      W = lds[(2 * RZ + r) * SS + n];
+      U = lds[(0 * RZ + r) * SS + pout];
+      V = lds[(1 * RZ + r) * SS + pout];
+      W = lds[(2 * RZ + r) * SS + pout];
     }
     __syncthreads();
   };
@@ -213,16 +216,16 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
     if (act) {
 #pragma unroll
       for (int f = 0; f < 8; ++f)
-        if (f < nf) lds[(f * RZ + r) * SS + n] = fld[f];
+        if (f < nf) lds[(f * RZ + r) * SS + pin] = fld[f];
     }
     __syncthreads();
-    fft_batch<-1, 12>(lds, SS, nf * RZ, g.px, tw, tid, nthr);
+    fft_run<MC, -1, 12>(lds, SS, nf * RZ, g.px, tw, rl, tid, nthr);
     if (act) {
       // outputs kx = n (and kx = M for an undealiased x grid)
       for (int kq = n; kq < nxh; kq += M) {
         const double2 wk = rootx[kq];
         const double kxv = ops.kx[kq];
-        auto X = [&](int f) { return r2c_post_nl(lds + (f * RZ + r) * SS, M, kq, wk); };
+        auto X = [&](int f) { return r2c_post_nl<MC>(lds + (f * RZ + r) * SS, M, kq, wk); };
         auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
         double2 X0 = X(0), X1 = X(1), X2 = X(2), X3 = X(3), X4 = X(4);
         double2 Au = cadd(X1, ikxm(X0));
@@ -254,26 +257,35 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
   XPassArgs a = a0;
   const int M = g.nx2 / 2;
   if (a.kb < a.ka) return;
-  // rows per workgroup: largest divisor of nz2 with rz*M <= 640
+  // rows per workgroup: largest divisor of nz2 with rz*M <= maxthr (default
+  // 640; CHANFLOW_XPASS_THREADS overrides it for tuning)
+  static const int maxthr = [] {
+    const char* e = getenv("CHANFLOW_XPASS_THREADS");
+    int v = e ? atoi(e) : 640;
+    return (v >= 64 && v <= 640) ? v : 640;
+  }();
   int rz = 1;
   for (int d = 1; d <= g.nz2; ++d)
-    if (g.nz2 % d == 0 && d * M <= 640) rz = d;
+    if (g.nz2 % d == 0 && d * M <= std::max(maxthr, M)) rz = d;
   a.rz = rz;
   int nthr = ((rz * M + 63) / 64) * 64;
-  if (nthr > 1024) throw chan_error("x-pass workgroup too large");
-  // staging bound of the in-place FFT: 8*rz sequences of length M
-  for (int i = 0; i < g.px.nst; ++i) {
-    int R = g.px.rad[i];
-    long nbf = (long)8 * rz * (M / R);
-    if (((nbf + nthr - 1) / nthr) * R > 12) throw chan_error("x-pass FFT staging bound");
-  }
+  if (nthr > 640) throw chan_error("x-pass: nx2/2 above 640 is not supported");
+  const bool ct = fft2_split(M).n1 > 0;
+  // staging bound of the runtime-planned in-place FFT: 8*rz sequences of length M
+  if (!ct)
+    for (int i = 0; i < g.px.nst; ++i) {
+      int R = g.px.rad[i];
+      long nbf = (long)8 * rz * (M / R);
+      if (((nbf + nthr - 1) / nthr) * R > 12) throw chan_error("x-pass FFT staging bound");
+    }
   int nplanes = a.kb - a.ka + 1;
   int nch = (nplanes + a.chunk - 1) / a.chunk;
   dim3 grid(g.nz2 / rz, nch);
-  size_t lds = sizeof(double2) * 8 * rz * (M + 1);
-  if (a.base)
-    hipLaunchKernelGGL(k_xpass<true>, grid, dim3(nthr), lds, s, g, tw, ops, a);
-  else
-    hipLaunchKernelGGL(k_xpass<false>, grid, dim3(nthr), lds, s, g, tw, ops, a);
-  LAUNCH_CHECK("k_xpass<false>");
+  size_t lds = sizeof(double2) * (8 * rz * fft_seq_cells(M) + (ct ? M : 0));
+#define XPASS_CASE(N)                                                                 \
+  if (a.base) hipLaunchKernelGGL((k_xpass<true, N>), grid, dim3(nthr), lds, s, g, tw, ops, a); \
+  else hipLaunchKernelGGL((k_xpass<false, N>), grid, dim3(nthr), lds, s, g, tw, ops, a)
+  FFT_DISPATCH(M, XPASS_CASE)
+#undef XPASS_CASE
+  LAUNCH_CHECK("k_xpass");
 }

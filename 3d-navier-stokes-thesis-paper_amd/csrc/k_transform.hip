@@ -15,6 +15,7 @@
 
 #define ZT_KX 8
 
+template <int NC>
 __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
                                               const double2* __restrict__ in, long in_ps,
                                               double2* __restrict__ out, long out_ps) {
@@ -22,7 +23,9 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int kx0 = blockIdx.x * ZT_KX;
   const long P = blockIdx.y;
-  const int nz = g.nz, nz2 = g.nz2, ss = nz2 + 1;
+  const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
+  double2* rl = lds + ZT_KX * ss;
+  if (NC > 0) load_roots(rl, tw + g.pz.tw, nz2, tid, nthr);
   const int ncol = min(ZT_KX, g.nxh - kx0);
   const double2* src = in + P * in_ps;
   const bool pad = (nz2 > nz) && !g.small;
@@ -42,14 +45,14 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
         v = col[t];
       }
     }
-    lds[s * ss + t] = v;
+    lds[s * ss + fft_in_pos<NC>(t)] = v;
   }
   __syncthreads();
-  fft_batch<1>(lds, ss, ZT_KX, g.pz, tw, tid, nthr);
+  fft_run<NC, 1, FFT_MAXV>(lds, ss, ZT_KX, g.pz, tw, rl, tid, nthr);
   double2* dst = out + P * out_ps;
   for (int e = tid; e < nz2 * ZT_KX; e += nthr) {
     int t = e / ZT_KX, s = e - t * ZT_KX;
-    if (s < ncol) dst[(long)t * g.kxpad + kx0 + s] = lds[s * ss + t];
+    if (s < ncol) dst[(long)t * g.kxpad + kx0 + s] = lds[s * ss + fft_out_pos<NC>(t)];
   }
 }
 
@@ -147,6 +150,7 @@ __global__ __launch_bounds__(512) void k_xr2c(XZGeom g, const double2* __restric
   }
 }
 
+template <int NC>
 __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restrict__ tw,
                                               const double2* __restrict__ A,
                                               const double2* __restrict__ B, long ab_ps,
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int kx0 = blockIdx.x * ZT_KX;
   const long P = blockIdx.y;
-  const int nz = g.nz, nz2 = g.nz2, ss = nz2 + 1;
+  const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   const int ncol = min(ZT_KX, g.nxh - kx0);
   double2* dst = out + P * out_ps;
   if (nl_zero_ny > 0) {
@@ -174,38 +178,41 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
     }
   }
   const int nf = B ? 2 : 1;
+  double2* rl = lds + nf * ZT_KX * ss;
+  if (NC > 0) load_roots(rl, tw + g.pz.tw, nz2, tid, nthr);
   for (int e = tid; e < nf * nz2 * ZT_KX; e += nthr) {
     int f = e / (nz2 * ZT_KX);
     int e2 = e - f * nz2 * ZT_KX;
     int t = e2 / ZT_KX, s = e2 - t * ZT_KX;
     double2 v = make_double2(0.0, 0.0);
     if (s < ncol) v = (f == 0 ? A : B)[P * ab_ps + (long)t * g.kxpad + kx0 + s];
-    lds[(f * ZT_KX + s) * ss + t] = v;
+    lds[(f * ZT_KX + s) * ss + fft_in_pos<NC>(t)] = v;
   }
   __syncthreads();
-  fft_batch<-1>(lds, ss, nf * ZT_KX, g.pz, tw, tid, nthr);
+  fft_run<NC, -1, FFT_MAXV>(lds, ss, nf * ZT_KX, g.pz, tw, rl, tid, nthr);
   const int h = nz / 2, off = nz2 - nz;
+  auto at = [&](const double2* a, int q) { return a[fft_out_pos<NC>(q)]; };
   for (int e = tid; e < ncol * nz; e += nthr) {
     int s = e / nz, q = e - s * nz;
     const double2* a = lds + s * ss;
     double2 val;
     if (g.small) {
       double sc = 1.0 / (double)(g.nx * g.nz);
-      val = cscale(sc, a[q]);
+      val = cscale(sc, at(a, q));
     } else {
       const double scale = 1.0 / (double)(g.nx2 * g.nz2);
       const double f = (kx0 + s == g.nxh - 1) ? 2.0 * scale : scale;
       double2 fa;
-      if (q < h) fa = a[q];
-      else if (q == h) fa = cadd(a[h], cconj(a[off + h]));
-      else fa = a[q + off];
+      if (q < h) fa = at(a, q);
+      else if (q == h) fa = cadd(at(a, h), cconj(at(a, off + h)));
+      else fa = at(a, q + off);
       val = cscale(f, fa);
       if (B) {
         const double2* b = lds + (ZT_KX + s) * ss;
         double2 fb;
-        if (q < h) fb = b[q];
-        else if (q == h) fb = cadd(b[h], cconj(b[off + h]));
-        else fb = b[q + off];
+        if (q < h) fb = at(b, q);
+        else if (q == h) fb = cadd(at(b, h), cconj(at(b, off + h)));
+        else fb = at(b, q + off);
         fb = cscale(f, fb);
         double kz = kzv[q];
         val = make_double2(val.x - kz * fb.y, val.y + kz * fb.x);
@@ -231,9 +238,13 @@ void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_
                  long out_ps, int nplanes, hipStream_t s) {
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nthr = 256;
-  check_stage_fit(g.pz, ZT_KX, nthr);
-  size_t lds = sizeof(double2) * ZT_KX * (g.nz2 + 1);
-  hipLaunchKernelGGL(k_zinv, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps);
+  const bool ct = fft2_split(g.nz2).n1 > 0;
+  if (!ct) check_stage_fit(g.pz, ZT_KX, nthr);
+  size_t lds = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
+#define ZINV_CASE(N) \
+  hipLaunchKernelGGL(k_zinv<N>, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps)
+  FFT_DISPATCH(g.nz2, ZINV_CASE)
+#undef ZINV_CASE
   LAUNCH_CHECK("k_zinv");
 }
 
@@ -275,9 +286,13 @@ void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const dou
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nf = B ? 2 : 1;
   const int nthr = 256 * nf;
-  check_stage_fit(g.pz, nf * ZT_KX, nthr);
-  size_t lds = sizeof(double2) * nf * ZT_KX * (g.nz2 + 1);
-  hipLaunchKernelGGL(k_zfwd, grid, dim3(nthr), lds, s, g, tw, A, B, ab_ps, kz, out, out_ps,
-                     nl_zero_ny);
+  const bool ct = fft2_split(g.nz2).n1 > 0;
+  if (!ct) check_stage_fit(g.pz, nf * ZT_KX, nthr);
+  size_t lds = sizeof(double2) * (nf * ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
+#define ZFWD_CASE(N)                                                                              \
+  hipLaunchKernelGGL(k_zfwd<N>, grid, dim3(nthr), lds, s, g, tw, A, B, ab_ps, kz, out, out_ps, \
+                     nl_zero_ny)
+  FFT_DISPATCH(g.nz2, ZFWD_CASE)
+#undef ZFWD_CASE
   LAUNCH_CHECK("k_zfwd");
 }
