@@ -32,6 +32,49 @@ __device__ __forceinline__ double2 rmul(double s, double2 a) { return make_doubl
 __device__ __forceinline__ double2 imul(double s, double2 a) { return make_double2(-s * a.y, s * a.x); }
 
 
+// Row-blocked sweeps.  Every pass walks the y rows of its mode with a short
+// recurrence; the loads of a block of BL rows do not depend on it, so they
+// are all issued before the block's recurrence runs (one memory round trip
+// per block instead of one per row).
+template <int BL, class LoadF, class CompF>
+__device__ __forceinline__ void rows_up(int j0, int j1, LoadF load, CompF comp) {
+  using RD = decltype(load(0));
+  int j = j0;
+  for (; j + BL - 1 <= j1; j += BL) {
+    RD d[BL];
+#pragma unroll
+    for (int t = 0; t < BL; ++t) d[t] = load(j + t);
+#pragma unroll
+    for (int t = 0; t < BL; ++t) comp(j + t, d[t]);
+  }
+  for (; j <= j1; ++j) {
+    RD d = load(j);
+    comp(j, d);
+  }
+}
+
+template <int BL, class LoadF, class CompF>
+__device__ __forceinline__ void rows_down(int j1, int j0, LoadF load, CompF comp) {
+  using RD = decltype(load(0));
+  int j = j1;
+  for (; j - BL + 1 >= j0; j -= BL) {
+    RD d[BL];
+#pragma unroll
+    for (int t = 0; t < BL; ++t) d[t] = load(j - t);
+#pragma unroll
+    for (int t = 0; t < BL; ++t) comp(j - t, d[t]);
+  }
+  for (; j >= j0; --j) {
+    RD d = load(j);
+    comp(j, d);
+  }
+}
+
+struct RowA { double2 n0, n1, n2, r0, r1, r2, p; };
+struct RowB { double2 s0, s1, s2; double c0, c1; };
+struct RowC { double2 s3, s0, s1, s2, p, n0, n1, n2; double c0; };
+
+template <int SOLVE_BL>
 __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   const long nm = (long)a.nxh * a.nz;
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,6 +86,7 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   const double k2 = kxv * kxv + kzv * kzv;
   const bool mean = (k2 == 0.0);
   const int r = a.rk - 1;
+  const bool first = (r == 0);
   const double dt = a.sc->dt;
   const double re = a.re;
 
@@ -56,6 +100,7 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   auto rr = [&](int c, int j) { return R[((long)c * nyp + j) * MS + m]; };
   auto pp = [&](int j) { return P[(long)j * MS + m]; };
   auto sidx = [&](int c, int j) { return ((long)c * nyp + j) * MS + m; };
+  auto neg = [](double2 v) { return make_double2(-v.x, -v.y); };
   const double* d2h = ops.d2h;
   const double* d2n = ops.d2n;
   const double* dm = ops.dm;
@@ -66,92 +111,113 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   const double vis_exp1 = c_rkc[r] * (dt / re);
   const double alpha = 1.0 + vis_imp * k2;
 
-  const double2 top[3] = {a.cbc[(0L * 3 + 0) * MS + m], a.cbc[(0L * 3 + 1) * MS + m], a.cbc[(0L * 3 + 2) * MS + m]};
-  const double2 bot[3] = {a.cbc[(1L * 3 + 0) * MS + m], a.cbc[(1L * 3 + 1) * MS + m], a.cbc[(1L * 3 + 2) * MS + m]};
+  const double2 top0 = a.cbc[(0L * 3 + 0) * MS + m], top1 = a.cbc[(0L * 3 + 1) * MS + m],
+                top2 = a.cbc[(0L * 3 + 2) * MS + m];
+  const double2 bot0 = a.cbc[(1L * 3 + 0) * MS + m], bot1 = a.cbc[(1L * 3 + 1) * MS + m],
+                bot2 = a.cbc[(1L * 3 + 2) * MS + m];
 
   // mean-mode pressure gradient (compute_p_grad, :369-373)
   double p_grad = a.sc->p_grad;
   if (mean && a.fix_mass == 1) {
     double tmp = 0.0;
+#pragma unroll 8
     for (int j = 1; j < ny; ++j) tmp = tmp + ops.dyh[j] * (-nl(0, j).x);
     p_grad = -tmp + 0.5 * (a.dudy[ny - 1] - a.dudy[0]) / re;
     a.sc->p_grad = p_grad;
   }
 
   // ---------------- pass A: RHS and forward elimination ----------------
+  // Rows 0 and ny of u, w and rows 0 and ny-1 of v are Dirichlet rows whose
+  // right-hand side is the wall value (SOLVE_HELMHOLTZ, :174-213).
   double cH = 0.0, cN = 0.0;
-  double2 fu = cz(), fw = cz(), fv = cz();
-  for (int j = 0; j <= ny; ++j) {
-    double2 cu0 = nl(0, j), cu1 = nl(1, j), cu2 = nl(2, j);
-    cu0 = make_double2(-cu0.x, -cu0.y); cu1 = make_double2(-cu1.x, -cu1.y); cu2 = make_double2(-cu2.x, -cu2.y);
-    double2 R0 = rr(0, j), R1 = rr(1, j), R2 = rr(2, j);
-    double2 v0 = cadd(R0, rmul(coef, cu0));
-    double2 v1 = cadd(R1, rmul(coef, cu1));
-    double2 v2 = cadd(R2, rmul(coef, cu2));
-    if (r == 0) {
+  double2 fu = bot0, fw = bot2, fv = bot1;
+  S[sidx(0, 0)] = fu;
+  S[sidx(1, 0)] = fw;
+  SR[sidx(0, 0)] = 0.0;
+  if (!mean) {
+    S[sidx(2, 0)] = fv;
+    SR[sidx(1, 0)] = 0.0;
+  }
+  // window of the explicit-diffusion stencil: R at rows j-1, j and P at row j
+  double2 Rm0 = rr(0, 0), Rm1 = rr(1, 0), Rm2 = rr(2, 0);
+  double2 Rc0 = rr(0, 1), Rc1 = rr(1, 1), Rc2 = rr(2, 1);
+  double2 Pc = first ? pp(1) : cz();
+  auto loadA = [&](int j) {
+    RowA d;
+    d.n0 = nl(0, j); d.n1 = nl(1, j); d.n2 = nl(2, j);
+    d.r0 = rr(0, j + 1); d.r1 = rr(1, j + 1); d.r2 = rr(2, j + 1);
+    d.p = first ? pp(j + 1) : cz();
+    return d;
+  };
+  auto compA = [&](int j, const RowA& d) {
+    double2 v0 = cadd(Rc0, rmul(coef, neg(d.n0)));
+    double2 v1 = cadd(Rc1, rmul(coef, neg(d.n1)));
+    double2 v2 = cadd(Rc2, rmul(coef, neg(d.n2)));
+    if (first) {
       // add_explicit_diffusive_terms(cv, cru) (:339-351)
-      int j0 = min(max(j - 1, 0), nyp - 3);
-      double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
-      double2 du = cadd(cadd(rmul(a0, rr(0, j0)), rmul(a1, rr(0, j0 + 1))), rmul(a2, rr(0, j0 + 2)));
-      v0 = cadd(v0, rmul(vis_exp1, csub(du, rmul(k2, R0))));
-      double2 dw = cadd(cadd(rmul(a0, rr(2, j0)), rmul(a1, rr(2, j0 + 1))), rmul(a2, rr(2, j0 + 2)));
-      v2 = cadd(v2, rmul(vis_exp1, csub(dw, rmul(k2, R2))));
-      if (j < ny) {
-        int jv = min(max(j - 1, 0), ny - 3);
-        double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
-        double2 dv = cadd(cadd(rmul(b0, rr(1, jv)), rmul(b1, rr(1, jv + 1))), rmul(b2, rr(1, jv + 2)));
-        v1 = cadd(v1, rmul(vis_exp1, csub(dv, rmul(k2, R1))));
+      const double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
+      double2 du = cadd(cadd(rmul(a0, Rm0), rmul(a1, Rc0)), rmul(a2, d.r0));
+      v0 = cadd(v0, rmul(vis_exp1, csub(du, rmul(k2, Rc0))));
+      double2 dw = cadd(cadd(rmul(a0, Rm2), rmul(a1, Rc2)), rmul(a2, d.r2));
+      v2 = cadd(v2, rmul(vis_exp1, csub(dw, rmul(k2, Rc2))));
+      if (j <= ny - 2) {
+        const double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+        double2 dv = cadd(cadd(rmul(b0, Rm1), rmul(b1, Rc1)), rmul(b2, d.r1));
+        v1 = cadd(v1, rmul(vis_exp1, csub(dv, rmul(k2, Rc1))));
       }
       if (!mean) {  // subtract_pressure_gradient(cv, cp, rk_press) (:54-77)
-        if (j >= 1 && j <= ny - 1) {
-          double2 pj = pp(j);
-          v0 = csub(v0, imul(rk_press * kxv, pj));
-          v2 = csub(v2, imul(rk_press * kzv, pj));
-        }
-        if (j >= 1 && j <= ny - 2) v1 = csub(v1, rmul(rk_press * ops.dn[j], csub(pp(j + 1), pp(j))));
+        v0 = csub(v0, imul(rk_press * kxv, Pc));
+        v2 = csub(v2, imul(rk_press * kzv, Pc));
+        if (j <= ny - 2) v1 = csub(v1, rmul(rk_press * ops.dn[j], csub(d.p, Pc)));
       }
     }
     if (mean) {  // mean pressure gradient (:220-221)
       v0 = csub(v0, make_double2(a.cosb * rk_press * p_grad, 0.0));
       v2 = csub(v2, make_double2(a.sinb * rk_press * p_grad, 0.0));
     }
-    // Helmholtz rows (SOLVE_HELMHOLTZ, Dirichlet walls, :174-213)
-    double ah, bh, ch;
-    if (j == 0) { ah = 0.0; bh = 1.0; ch = 0.0; v0 = bot[0]; v2 = bot[2]; }
-    else if (j == ny) { ah = 0.0; bh = 1.0; ch = 0.0; v0 = top[0]; v2 = top[2]; }
-    else {
-      ah = -vis_imp * d2h[3 * j]; bh = -vis_imp * d2h[3 * j + 1] + alpha; ch = -vis_imp * d2h[3 * j + 2];
+    {
+      const double ah = -vis_imp * d2h[3 * j], bh = -vis_imp * d2h[3 * j + 1] + alpha,
+                   ch = -vis_imp * d2h[3 * j + 2];
+      const double bp = bh - ah * cH;
+      const double2 gu = csub(v0, rmul(ah, fu));
+      const double2 gw = csub(v2, rmul(ah, fw));
+      const double rb = 1.0 / bp;
+      cH = ch * rb;
+      fu = rmul(rb, gu);
+      fw = rmul(rb, gw);
+      S[sidx(0, j)] = fu;
+      S[sidx(1, j)] = fw;
+      SR[sidx(0, j)] = cH;
     }
-    double bp = (j == 0) ? bh : bh - ah * cH;
-    double2 gu = (j == 0) ? v0 : csub(v0, rmul(ah, fu));
-    double2 gw = (j == 0) ? v2 : csub(v2, rmul(ah, fw));
-    double rb = 1.0 / bp;
-    cH = ch * rb;
-    fu = rmul(rb, gu);
-    fw = rmul(rb, gw);
-    S[sidx(0, j)] = fu;
-    S[sidx(1, j)] = fw;
-    SR[sidx(0, j)] = cH;
-    if (j < ny && !mean) {
-      double an, bn, cn;
-      if (j == 0) { an = 0.0; bn = 1.0; cn = 0.0; v1 = bot[1]; }
-      else if (j == ny - 1) { an = 0.0; bn = 1.0; cn = 0.0; v1 = top[1]; }
-      else { an = -vis_imp * d2n[3 * j]; bn = -vis_imp * d2n[3 * j + 1] + alpha; cn = -vis_imp * d2n[3 * j + 2]; }
-      double bq = (j == 0) ? bn : bn - an * cN;
-      double2 gv = (j == 0) ? v1 : csub(v1, rmul(an, fv));
-      double rq = 1.0 / bq;
-      cN = cn * rq;
-      fv = rmul(rq, gv);
+    if (!mean) {
+      if (j <= ny - 2) {
+        const double an = -vis_imp * d2n[3 * j], bn = -vis_imp * d2n[3 * j + 1] + alpha,
+                     cn = -vis_imp * d2n[3 * j + 2];
+        const double bq = bn - an * cN;
+        const double2 gv = csub(v1, rmul(an, fv));
+        const double rq = 1.0 / bq;
+        cN = cn * rq;
+        fv = rmul(rq, gv);
+      } else {
+        cN = 0.0;
+        fv = top1;
+      }
       S[sidx(2, j)] = fv;
       SR[sidx(1, j)] = cN;
     }
-  }
+    Rm0 = Rc0; Rm1 = Rc1; Rm2 = Rc2;
+    Rc0 = d.r0; Rc1 = d.r1; Rc2 = d.r2;
+    Pc = d.p;
+  };
+  rows_up<SOLVE_BL>(1, ny - 1, loadA, compA);
+  S[sidx(0, ny)] = top0;
+  S[sidx(1, ny)] = top2;
+  SR[sidx(0, ny)] = 0.0;
 
   // ---------------- pass B: back substitution + Poisson elimination ----------------
   const double tmpc = 1.0 / rk_press;
   const double alp = -k2;
-  double2 xu1 = cz(), xw1 = cz(), xv1 = cz();  // values at j+1
-  double ap = 0.0;                              // a'_{i+1} of the reversed Poisson elimination
+  double ap = 0.0;   // a'_{i+1} of the reversed Poisson elimination
   double2 fp = cz();
   auto poisson_row = [&](int i, double& a_, double& b_, double& c_) {
     const double* dh = ops.dh; const double* dn = ops.dn;
@@ -166,87 +232,78 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   auto poisson_elim = [&](int i, double2 rhs) {  // reversed Thomas, row i (descending)
     double a_, b_, c_;
     poisson_row(i, a_, b_, c_);
-    double bp2 = (i == ny) ? b_ : b_ - c_ * ap;
-    double2 g = (i == ny) ? rhs : csub(rhs, rmul(c_, fp));
-    double rb2 = 1.0 / bp2;
+    const double bp2 = (i == ny) ? b_ : b_ - c_ * ap;
+    const double2 g = (i == ny) ? rhs : csub(rhs, rmul(c_, fp));
+    const double rb2 = 1.0 / bp2;
     ap = a_ * rb2;
     fp = rmul(rb2, g);
     S[sidx(3, i)] = fp;
     SR[sidx(0, i)] = ap;
   };
-  for (int j = ny; j >= 0; --j) {
-    double2 xu, xw, xv;
-    if (j == ny) { xu = S[sidx(0, j)]; xw = S[sidx(1, j)]; }
-    else {
-      double c = SR[sidx(0, j)];
-      xu = csub(S[sidx(0, j)], rmul(c, xu1));
-      xw = csub(S[sidx(1, j)], rmul(c, xw1));
-    }
-    if (mean) xv = (j < ny) ? bot[1] : cz();  // (:261-263)
-    else if (j == ny) xv = cz();
-    else if (j == ny - 1) xv = S[sidx(2, j)];
-    else xv = csub(S[sidx(2, j)], rmul(SR[sidx(1, j)], xv1));
+  // row ny: Dirichlet rows of u, w; v has no row ny
+  double2 xu1 = top0, xw1 = top2, xv1 = cz();
+  S[sidx(2, ny)] = cz();
+  auto loadB = [&](int j) {
+    RowB d;
+    d.s0 = S[sidx(0, j)]; d.s1 = S[sidx(1, j)];
+    d.s2 = mean ? cz() : S[sidx(2, j)];
+    d.c0 = SR[sidx(0, j)]; d.c1 = mean ? 0.0 : SR[sidx(1, j)];
+    return d;
+  };
+  auto compB = [&](int j, const RowB& d) {
+    const double2 xu = csub(d.s0, rmul(d.c0, xu1));
+    const double2 xw = csub(d.s1, rmul(d.c0, xw1));
+    double2 xv;
+    if (mean) xv = bot1;  // (:261-263)
+    else if (j == ny - 1) xv = d.s2;
+    else xv = csub(d.s2, rmul(d.c1, xv1));
     S[sidx(0, j)] = xu;
     S[sidx(1, j)] = xw;
     S[sidx(2, j)] = xv;
-    if (!mean && j + 1 <= ny) {
-      int i = j + 1;
+    if (!mean) {
+      const int i = j + 1;
       double2 rhs = cz();
       if (i <= ny - 1) {  // strip_divergence rhs (:28-31)
-        double2 d = cadd(cadd(imul(kxv, xu1), rmul(ops.dh[i], csub(xv1, xv))), imul(kzv, xw1));
-        rhs = rmul(tmpc, d);
+        double2 dv = cadd(cadd(imul(kxv, xu1), rmul(ops.dh[i], csub(xv1, xv))), imul(kzv, xw1));
+        rhs = rmul(tmpc, dv);
       }
       poisson_elim(i, rhs);
     }
     xu1 = xu; xw1 = xw; xv1 = xv;
-  }
+  };
+  rows_down<SOLVE_BL>(ny - 1, 0, loadB, compB);
   if (!mean) poisson_elim(0, cz());
 
-  // ---------------- pass C1: pressure-correction back substitution ----------------
-  if (!mean) {
-    double2 ph = S[sidx(3, 0)];
-    S[sidx(3, 0)] = ph;
-    double2 p1 = cz(), p2 = cz(), pm2 = cz(), pm1 = cz();
-    for (int i = 1; i <= ny; ++i) {
-      ph = csub(S[sidx(3, i)], rmul(SR[sidx(0, i)], ph));
-      S[sidx(3, i)] = ph;
-      if (i == 1) p1 = ph;
-      if (i == 2) p2 = ph;
-      if (i == ny - 2) pm2 = ph;
-      if (i == ny - 1) pm1 = ph;
-    }
-    // wall re-extrapolation (SOLVE_P :335-336), zero wall data
-    double2 pb = make_double2((0.0 - dm[1] * p1.x - dm[2] * p2.x) / dm[0], (0.0 - dm[1] * p1.y - dm[2] * p2.y) / dm[0]);
-    double2 pt = make_double2((0.0 - dm[3] * pm2.x - dm[4] * pm1.x) / dm[5],
-                              (0.0 - dm[3] * pm2.y - dm[4] * pm1.y) / dm[5]);
-    S[sidx(3, 0)] = pb;
-    S[sidx(3, ny)] = pt;
-  }
-
-  // ---------------- pass C2: projection, pressure update, next RK register ----------------
+  // ---------------- pass C: pressure correction, projection, next RK register ----------------
+  // The Poisson back substitution (ascending) runs two rows ahead of the
+  // projection, which needs phi at rows i-1..i+1; phi at the walls is
+  // re-extrapolated from the interior (SOLVE_P :335-336, zero wall data).
   const bool more = (a.rk < 3);
   const double coef2 = more ? dt * c_rka[r + 1] : 0.0;
   const double vis_exp2 = more ? (dt / re) * c_rkc[r + 1] : 0.0;
   const double rk_press2 = more ? (c_rkc[r + 1] + c_rkd[r + 1]) * dt : 0.0;
+  const double alpha2 = 1.0 + vis_imp * k2;
   double2 w0[4][3];   // cv at rows i-3..i
-  double2 Pw[2];      // P_new at rows i-1, i
-  for (int t = 0; t < 4; ++t) for (int c = 0; c < 3; ++c) w0[t][c] = cz();
-  Pw[0] = Pw[1] = cz();
-  auto phi = [&](int i) { return S[sidx(3, i)]; };
+  double2 Pw0 = cz(), Pw1 = cz();  // P_new at rows i-1, i
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w0[t][c] = cz();
+  // buildR: next RK register at row j from cv at row j, the D2H stencil rows
+  // st (three rows), the D2N stencil rows stv, P_new at rows j, j+1 and NL(j).
   auto buildR = [&](int j, const double2* cvj, const double2 (*st)[3], const double2 (*stv)[3], double2 Pj,
-                    double2 Pj1) {
-    // cvj: cv at row j; st: three rows for the D2H stencil; stv: three rows for D2N
-    double2 cu[3];
-    for (int c = 0; c < 3; ++c) { double2 t = nl(c, j); cu[c] = make_double2(-t.x, -t.y); }
+                    double2 Pj1, double2 n0, double2 n1, double2 n2) {
     double2 out[3];
-    for (int c = 0; c < 3; ++c) out[c] = cadd(cvj[c], rmul(coef2, cu[c]));
-    double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
+    out[0] = cadd(cvj[0], rmul(coef2, neg(n0)));
+    out[1] = cadd(cvj[1], rmul(coef2, neg(n1)));
+    out[2] = cadd(cvj[2], rmul(coef2, neg(n2)));
+    const double a0 = d2h[3 * j], a1 = d2h[3 * j + 1], a2 = d2h[3 * j + 2];
     double2 du = cadd(cadd(rmul(a0, st[0][0]), rmul(a1, st[1][0])), rmul(a2, st[2][0]));
     double2 dw = cadd(cadd(rmul(a0, st[0][2]), rmul(a1, st[1][2])), rmul(a2, st[2][2]));
     out[0] = cadd(out[0], rmul(vis_exp2, csub(du, rmul(k2, cvj[0]))));
     out[2] = cadd(out[2], rmul(vis_exp2, csub(dw, rmul(k2, cvj[2]))));
     if (j < ny) {
-      double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+      const double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
       double2 dv = cadd(cadd(rmul(b0, stv[0][1]), rmul(b1, stv[1][1])), rmul(b2, stv[2][1]));
       out[1] = cadd(out[1], rmul(vis_exp2, csub(dv, rmul(k2, cvj[1]))));
     }
@@ -257,64 +314,106 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
       }
       if (j >= 1 && j <= ny - 2) out[1] = csub(out[1], rmul(rk_press2 * ops.dn[j], csub(Pj1, Pj)));
     }
+#pragma unroll
     for (int c = 0; c < 3; ++c) R[((long)c * nyp + j) * MS + m] = out[c];
   };
-  const double alpha2 = 1.0 + vis_imp * k2;
-  for (int i = 0; i <= ny; ++i) {
-    double2 cv[3];
-    cv[0] = S[sidx(0, i)];
-    cv[1] = S[sidx(1, i)];  // w stored in slot 1
-    cv[2] = S[sidx(2, i)];  // v stored in slot 2
-    // reorder to (u, v, w)
-    double2 cu_ = cv[0], cw_ = cv[1], cvv = cv[2];
-    cv[0] = cu_; cv[1] = cvv; cv[2] = cw_;
-    double2 pn;
+  // projection of row i: s0,s1,s2 = provisional u, w, v; Pold = P(i); f0 = phi(i),
+  // fn = phi(i+1), st0..st2 = phi on the D2H stencil rows of i; nb* = NL(i-1).
+  auto c2row = [&](int i, double2 s0, double2 s1, double2 s2, double2 Pold, double2 f0, double2 fn,
+                   double2 st0, double2 st1, double2 st2, double2 nb0, double2 nb1, double2 nb2) {
+    double2 cv[3] = {s0, s2, s1};  // (u, v, w)
+    double2 pn = cz();
     if (!mean) {
-      double2 f0 = phi(i);
       if (i >= 1 && i <= ny - 1) {
         cv[0] = csub(cv[0], imul(rk_press * kxv, f0));
         cv[2] = csub(cv[2], imul(rk_press * kzv, f0));
       }
-      if (i >= 1 && i <= ny - 2) cv[1] = csub(cv[1], rmul(rk_press * ops.dn[i], csub(phi(i + 1), f0)));
-      int j0 = min(max(i - 1, 0), nyp - 3);
-      double2 d2 = cadd(cadd(rmul(d2h[3 * i], phi(j0)), rmul(d2h[3 * i + 1], phi(j0 + 1))),
-                        rmul(d2h[3 * i + 2], phi(j0 + 2)));
-      pn = csub(cadd(pp(i), rmul(alpha2, f0)), rmul(vis_imp, d2));
-    } else {
-      pn = cz();
+      if (i >= 1 && i <= ny - 2) cv[1] = csub(cv[1], rmul(rk_press * ops.dn[i], csub(fn, f0)));
+      double2 d2 = cadd(cadd(rmul(d2h[3 * i], st0), rmul(d2h[3 * i + 1], st1)), rmul(d2h[3 * i + 2], st2));
+      pn = csub(cadd(Pold, rmul(alpha2, f0)), rmul(vis_imp, d2));
     }
+#pragma unroll
     for (int c = 0; c < 3; ++c) U[((long)c * nyp + i) * MS + m] = cv[c];
     P[(long)i * MS + m] = pn;
-    // slide windows
-    for (int t = 0; t < 3; ++t) for (int c = 0; c < 3; ++c) w0[t][c] = w0[t + 1][c];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) w0[t][c] = w0[t + 1][c];
+#pragma unroll
     for (int c = 0; c < 3; ++c) w0[3][c] = cv[c];
-    Pw[0] = Pw[1]; Pw[1] = pn;
+    Pw0 = Pw1; Pw1 = pn;
     if (mean) {  // d_um_dy update from the new mean profile (:84-89)
       if (i >= 2 && i <= ny - 1) a.dudy[i - 1] = (ops.dn[i - 1] * (w0[3][0].x - w0[2][0].x));
       if (i == 2) a.dudy[0] = dm[0] * w0[1][0].x + dm[1] * w0[2][0].x + dm[2] * w0[3][0].x;
       if (i == ny) a.dudy[ny - 1] = dm[3] * w0[1][0].x + dm[4] * w0[2][0].x + dm[5] * w0[3][0].x;
     }
     if (!more) {
+#pragma unroll
       for (int c = 0; c < 3; ++c) R[((long)c * nyp + i) * MS + m] = cv[c];
-      continue;
+      return;
     }
     if (i >= 2) {
-      if (i == 2) {  // row 0 (clamped stencils on rows 0..2, no gradient)
-        buildR(0, w0[1], &w0[1], &w0[1], cz(), cz());
-      }
+      if (i == 2)  // row 0 (clamped stencils on rows 0..2, no gradient)
+        buildR(0, w0[1], &w0[1], &w0[1], cz(), cz(), nl(0, 0), nl(1, 0), nl(2, 0));
       // row j = i-1: centred D2H on rows i-2..i; D2N clamped at the last v row
       const int j = i - 1;
       const double2 (*stv)[3] = (j == ny - 1) ? &w0[0] : &w0[1];
-      buildR(j, w0[2], &w0[1], stv, Pw[0], Pw[1]);
+      buildR(j, w0[2], &w0[1], stv, Pw0, Pw1, nb0, nb1, nb2);
     }
+  };
+  auto s3 = [&](int i) { return mean ? cz() : S[sidx(3, i)]; };
+  auto sr0 = [&](int i) { return mean ? 0.0 : SR[sidx(0, i)]; };
+  double2 ph0 = s3(0);
+  const double2 ph1 = csub(s3(1), rmul(sr0(1), ph0));
+  const double2 ph2 = csub(s3(2), rmul(sr0(2), ph1));
+  const double2 pb = make_double2((0.0 - dm[1] * ph1.x - dm[2] * ph2.x) / dm[0],
+                                  (0.0 - dm[1] * ph1.y - dm[2] * ph2.y) / dm[0]);
+  c2row(0, S[sidx(0, 0)], S[sidx(1, 0)], S[sidx(2, 0)], pp(0), pb, cz(), pb, ph1, ph2, cz(), cz(), cz());
+  c2row(1, S[sidx(0, 1)], S[sidx(1, 1)], S[sidx(2, 1)], pp(1), ph1, ph2, pb, ph1, ph2, cz(), cz(), cz());
+  double2 q0 = pb, q1 = ph1, q2 = ph2, ph = ph2;  // phi at rows i-3..i-1 and the chain value
+  auto loadC = [&](int i) {
+    RowC d;
+    d.s3 = s3(i); d.c0 = sr0(i);
+    d.s0 = S[sidx(0, i - 1)]; d.s1 = S[sidx(1, i - 1)]; d.s2 = S[sidx(2, i - 1)];
+    d.p = pp(i - 1);
+    if (more) { d.n0 = nl(0, i - 2); d.n1 = nl(1, i - 2); d.n2 = nl(2, i - 2); }
+    else { d.n0 = d.n1 = d.n2 = cz(); }
+    return d;
+  };
+  auto compC = [&](int i, const RowC& d) {
+    ph = csub(d.s3, rmul(d.c0, ph));
+    q0 = q1; q1 = q2; q2 = ph;
+    c2row(i - 1, d.s0, d.s1, d.s2, d.p, q1, q2, q0, q1, q2, d.n0, d.n1, d.n2);
+  };
+  rows_up<SOLVE_BL>(3, ny - 1, loadC, compC);
+  // top wall: phi(ny) from phi(ny-2), phi(ny-1)
+  const double2 pt = make_double2((0.0 - dm[3] * q1.x - dm[4] * q2.x) / dm[5],
+                                  (0.0 - dm[3] * q1.y - dm[4] * q2.y) / dm[5]);
+  if (more) {
+    c2row(ny - 1, S[sidx(0, ny - 1)], S[sidx(1, ny - 1)], S[sidx(2, ny - 1)], pp(ny - 1), q2, pt, q1, q2, pt,
+          nl(0, ny - 2), nl(1, ny - 2), nl(2, ny - 2));
+    c2row(ny, S[sidx(0, ny)], S[sidx(1, ny)], S[sidx(2, ny)], pp(ny), pt, cz(), q1, q2, pt,
+          nl(0, ny - 1), nl(1, ny - 1), nl(2, ny - 1));
+    buildR(ny, w0[3], &w0[1], &w0[1], Pw1, cz(), nl(0, ny), nl(1, ny), nl(2, ny));
+  } else {
+    c2row(ny - 1, S[sidx(0, ny - 1)], S[sidx(1, ny - 1)], S[sidx(2, ny - 1)], pp(ny - 1), q2, pt, q1, q2, pt,
+          cz(), cz(), cz());
+    c2row(ny, S[sidx(0, ny)], S[sidx(1, ny)], S[sidx(2, ny)], pp(ny), pt, cz(), q1, q2, pt, cz(), cz(), cz());
   }
-  if (more) buildR(ny, w0[3], &w0[1], &w0[1], Pw[1], cz());
 }
 
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));
-  hipLaunchKernelGGL(k_solve, grid, dim3(64), 0, s, ops, a);
+  // rows per prefetch block (CHANFLOW_SOLVE_BL = 4, 8 or 16 overrides it for tuning)
+  static const int bl = [] {
+    const char* e = getenv("CHANFLOW_SOLVE_BL");
+    int v = e ? atoi(e) : 8;
+    return (v == 4 || v == 16) ? v : 8;
+  }();
+  if (bl == 4) hipLaunchKernelGGL(k_solve<4>, grid, dim3(64), 0, s, ops, a);
+  else if (bl == 16) hipLaunchKernelGGL(k_solve<16>, grid, dim3(64), 0, s, ops, a);
+  else hipLaunchKernelGGL(k_solve<8>, grid, dim3(64), 0, s, ops, a);
   LAUNCH_CHECK("k_solve");
 }
 
