@@ -256,6 +256,37 @@ __device__ __forceinline__ int fft2_out_pos(int k) {
   return (k % N1) * (N2 + 1) + k / N1;
 }
 
+// The two passes of the two-pass FFT for one task each.
+template <int N, int SGN>
+__device__ __forceinline__ void fft2_pass1(double2* seq, int b, const double2* rl, const double2* __restrict__ rg) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2, RS = N2 + 1;
+  double2* p = seq + b;
+  double2 v[N1];
+#pragma unroll
+  for (int a = 0; a < N1; ++a) v[a] = p[a * RS];
+  rdft<N1, SGN, N2>(v, rg);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) {
+    double2 w = rl[b * k1];
+    if (SGN > 0) w = cconj(w);
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) p[k1 * RS] = v[k1];
+}
+
+template <int N, int SGN>
+__device__ __forceinline__ void fft2_pass2(double2* seq, int k1, const double2* __restrict__ rg) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2, RS = N2 + 1;
+  double2* p = seq + k1 * RS;
+  double2 v[N2];
+#pragma unroll
+  for (int b = 0; b < N2; ++b) v[b] = p[b];
+  rdft<N2, SGN, N1>(v, rg);
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) p[k2] = v[k2];
+}
+
 // In-place two-pass FFT of B sequences (sequence s at lds + s*sstride).
 // rl: root table exp(-2 pi i t/N), t < N, in LDS; rg: the same table in
 // global memory (scalar-loaded constants of the register DFTs).  Call with
@@ -263,36 +294,45 @@ __device__ __forceinline__ int fft2_out_pos(int k) {
 template <int N, int SGN>
 __device__ __forceinline__ void fft2p(double2* lds, int sstride, int B, const double2* rl,
                                       const double2* __restrict__ rg, int tid, int nthr) {
-  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2, RS = N2 + 1;
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2;
   static_assert(N1 * N2 == N && N1 > 0, "unsupported two-pass length");
   for (int g = tid; g < B * N2; g += nthr) {
     const int s = g / N2, b = g - s * N2;
-    double2* p = lds + s * sstride + b;
-    double2 v[N1];
-#pragma unroll
-    for (int a = 0; a < N1; ++a) v[a] = p[a * RS];
-    rdft<N1, SGN, N2>(v, rg);
-#pragma unroll
-    for (int k1 = 1; k1 < N1; ++k1) {
-      double2 w = rl[b * k1];
-      if (SGN > 0) w = cconj(w);
-      v[k1] = cmul(v[k1], w);
-    }
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) p[k1 * RS] = v[k1];
+    fft2_pass1<N, SGN>(lds + s * sstride, b, rl, rg);
   }
   __syncthreads();
   for (int g = tid; g < B * N1; g += nthr) {
     const int s = g / N1, k1 = g - s * N1;
-    double2* p = lds + s * sstride + k1 * RS;
-    double2 v[N2];
-#pragma unroll
-    for (int b = 0; b < N2; ++b) v[b] = p[b];
-    rdft<N2, SGN, N1>(v, rg);
-#pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) p[k2] = v[k2];
+    fft2_pass2<N, SGN>(lds + s * sstride, k1, rg);
   }
   __syncthreads();
+}
+
+// Orders the LDS accesses of one wavefront: everything a lane wrote before
+// is visible to every lane of the same wavefront after it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Two-pass FFT of B sequences owned by one wavefront (lane = 0..63); no
+// workgroup barrier, so wavefronts of a workgroup run independently.
+template <int N, int SGN>
+__device__ __forceinline__ void fft2p_wave(double2* lds, int sstride, int B, const double2* rl,
+                                           const double2* __restrict__ rg, int lane) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2;
+  static_assert(N1 * N2 == N && N1 > 0, "unsupported two-pass length");
+  for (int g = lane; g < B * N2; g += 64) {
+    const int s = g / N2, b = g - s * N2;
+    fft2_pass1<N, SGN>(lds + s * sstride, b, rl, rg);
+  }
+  wave_sync();
+  for (int g = lane; g < B * N1; g += 64) {
+    const int s = g / N1, k1 = g - s * N1;
+    fft2_pass2<N, SGN>(lds + s * sstride, k1, rg);
+  }
+  wave_sync();
 }
 
 // Position of input element n / output element k of a sequence: the two-pass

@@ -252,6 +252,282 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-centric x-pass for the compile-time lengths M <= 192.  A wavefront
+// owns one z-row and walks a chunk of planes on its own: lane l holds the
+// physical point pairs n = l + 64q (q < PPL) of the row, and the x transforms
+// of the row run in the wavefront's private LDS slice (four sequences),
+// ordered with wavefront-scope fences only.  The eight forward fields go
+// through the slice in two batches of four: {u^2, d/dy flux_u, w u,
+// d/dy flux_w} give Au, Bu, Aw, and {w^2, v-interp u, d/dy flux_v,
+// v-interp w} give Bw, Av, Bv.  The four wavefronts of a workgroup share
+// only the root table.
+// ---------------------------------------------------------------------------
+template <bool LIN, int M>
+__global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
+                                                  XPassArgs a) {
+  extern __shared__ double2 lds[];
+  constexpr int SEQ = fft_seq_cells(M), PPL = (M + 63) / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double2* rl = lds;
+  load_roots(rl, tw + g.px.tw, M, tid, 256);
+  __syncthreads();
+  const int zz = blockIdx.x * 4 + wv;
+  if (zz >= g.nz2) return;  // after the only workgroup barrier
+  double2* W = lds + M + wv * 4 * SEQ;
+  const double2* rg = tw + g.px.tw;
+  const double2* rootx = tw + g.twx;
+  const int ny = a.ny, nyp = ny + 1;
+  const int ka = a.ka + blockIdx.y * a.chunk;
+  const int kb = min(a.kb, ka + a.chunk - 1);
+  const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
+  const long rowoff = (long)zz * kxp;
+  const bool hasbuf = ops.buffer != nullptr;
+
+  int nq[PPL], pin[PPL], pout[PPL];
+  bool vq[PPL];
+  double2 wc[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    nq[q] = lane + 64 * q;
+    vq[q] = nq[q] < M;
+    const int n = vq[q] ? nq[q] : 0;
+    wc[q] = cconj(rootx[n]);
+    pin[q] = fft2_in_pos<M>(n);
+    pout[q] = fft2_out_pos<M>(n);
+  }
+
+  // c2r of plane p (1-based) into U, V, Wv at the owned points
+  auto c2r_plane = [&](int p, double2* U, double2* V, double2* Wv, bool needv) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        if (!vq[q]) continue;
+        double2 v = make_double2(0.0, 0.0);
+        if (c != 1 || needv) v = c2r_pack_nl(row, nxh, M, nq[q], wc[q]);
+        W[c * SEQ + pin[q]] = v;
+      }
+    }
+    wave_sync();
+    fft2p_wave<M, 1>(W, SEQ, 3, rl, rg, lane);
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      if (!vq[q]) continue;
+      U[q] = W[0 * SEQ + pout[q]];
+      V[q] = W[1 * SEQ + pout[q]];
+      Wv[q] = W[2 * SEQ + pout[q]];
+    }
+    wave_sync();
+  };
+  auto load_base = [&](int p, double2* U, double2* V, double2* Wv) {
+    if (!LIN) return;
+    const double* b = a.base;
+    const long cs = (long)nyp * nz2 * g.nx2;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      if (!vq[q]) continue;
+      const long o = ((long)(p - 1) * nz2 + zz) * g.nx2 + 2 * nq[q];
+      U[q] = *reinterpret_cast<const double2*>(b + o);
+      V[q] = *reinterpret_cast<const double2*>(b + cs + o);
+      Wv[q] = *reinterpret_cast<const double2*>(b + 2 * cs + o);
+    }
+  };
+  auto bufv = [&](int q) {
+    return *reinterpret_cast<const double2*>(ops.buffer + (long)zz * g.nx2 + 2 * nq[q]);
+  };
+
+  const double2 z2 = make_double2(0.0, 0.0);
+  double2 U0[PPL], V0[PPL], W0[PPL], U1[PPL], V1[PPL], W1[PPL];
+  double2 BU0[PPL], BV0[PPL], BW0[PPL], BU1[PPL], BV1[PPL], BW1[PPL];
+  double2 FU[PPL], FW[PPL], GV[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    U0[q] = V0[q] = W0[q] = U1[q] = V1[q] = W1[q] = z2;
+    BU0[q] = BV0[q] = BW0[q] = BU1[q] = BV1[q] = BW1[q] = z2;
+  }
+  const double half = 0.5, two = 2.0;
+
+  // prologue: planes ka-1 and ka, fluxes at y(ka-1) and yh(ka)
+  c2r_plane(ka - 1, U0, V0, W0, true);
+  load_base(ka - 1, BU0, BV0, BW0);
+  c2r_plane(ka, U1, V1, W1, true);
+  load_base(ka, BU1, BV1, BW1);
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    if (!LIN) {
+      if (ka - 1 == 1) { FU[q] = pmul(V0[q], U0[q]); FW[q] = pmul(V0[q], W0[q]); }   // :53,56
+      else {
+        FU[q] = pmul(V0[q], psc(half, padd(U0[q], U1[q])));
+        FW[q] = pmul(V0[q], psc(half, padd(W0[q], W1[q])));
+      }
+      double2 t = psc(half, padd(V0[q], V1[q]));
+      GV[q] = pmul(t, t);                                                                 // :54-55
+    } else {
+      if (ka - 1 == 1) {                                                                  // lin :160-170
+        FU[q] = padd(pmul(BV0[q], U0[q]), pmul(V0[q], BU0[q]));
+        FW[q] = padd(pmul(BV0[q], W0[q]), pmul(V0[q], BW0[q]));
+      } else {                                                                            // lin :265-273
+        FU[q] = padd(pmul(psc(half, BV0[q]), padd(U0[q], U1[q])), pmul(psc(half, V0[q]), padd(BU0[q], BU1[q])));
+        FW[q] = padd(pmul(psc(half, BV0[q]), padd(W0[q], W1[q])), pmul(psc(half, V0[q]), padd(BW0[q], BW1[q])));
+      }
+      double2 t = psc(half, padd(V0[q], V1[q])), tb = psc(half, padd(BV0[q], BV1[q]));
+      GV[q] = pmul(psc(two, t), tb);
+    }
+    U0[q] = U1[q]; V0[q] = V1[q]; W0[q] = W1[q];
+    BU0[q] = BU1[q]; BV0[q] = BV1[q]; BW0[q] = BW1[q];
+  }
+
+  for (int k = ka; k <= kb; ++k) {
+    const bool top = (k == ny);
+    c2r_plane(k + 1, U1, V1, W1, !top);
+    load_base(k + 1, BU1, BV1, BW1);
+    const int kk = k - 1;  // 0-based plane index
+    const double dh = ops.dh[kk];
+    const double dnk = top ? 0.0 : ops.dn[kk];
+    const double i1 = top ? 0.0 : ops.iu[2 * kk], i2 = top ? 0.0 : ops.iu[2 * kk + 1];
+    double2 Fu[PPL], Fw[PPL], G1[PPL];
+    // batch 1: u^2, d/dy flux_u, w u, d/dy flux_w (:138-155)
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      if (!LIN) {
+        if (!top) {
+          Fu[q] = pmul(V0[q], psc(half, padd(U0[q], U1[q])));
+          Fw[q] = pmul(V0[q], psc(half, padd(W0[q], W1[q])));
+        } else {
+          Fu[q] = pmul(V0[q], U1[q]);
+          Fw[q] = pmul(V0[q], W1[q]);
+        }
+      } else {
+        if (!top) {
+          Fu[q] = padd(pmul(psc(half, BV0[q]), padd(U0[q], U1[q])), pmul(psc(half, V0[q]), padd(BU0[q], BU1[q])));
+          Fw[q] = padd(pmul(psc(half, BV0[q]), padd(W0[q], W1[q])), pmul(psc(half, V0[q]), padd(BW0[q], BW1[q])));
+        } else {
+          Fu[q] = padd(pmul(BV0[q], U1[q]), pmul(V0[q], BU1[q]));
+          Fw[q] = padd(pmul(BV0[q], W1[q]), pmul(V0[q], BW1[q]));
+        }
+      }
+      if (!vq[q]) continue;
+      double2 yfu = psc(dh, psub(Fu[q], FU[q]));
+      double2 yfw = psc(dh, psub(Fw[q], FW[q]));
+      if (hasbuf) {
+        const double2 bf = bufv(q);
+        const double tu = ops.target[0 * nyp + kk], tw_ = ops.target[2 * nyp + kk];
+        yfu = padd(yfu, pmul(bf, make_double2(U0[q].x - tu, U0[q].y - tu)));
+        yfw = padd(yfw, pmul(bf, make_double2(W0[q].x - tw_, W0[q].y - tw_)));
+      }
+      double2 f0, f2;
+      if (!LIN) { f0 = pmul(U0[q], U0[q]); f2 = pmul(W0[q], U0[q]); }
+      else { f0 = pmul(psc(two, BU0[q]), U0[q]); f2 = padd(pmul(BW0[q], U0[q]), pmul(W0[q], BU0[q])); }
+      W[0 * SEQ + pin[q]] = f0;
+      W[1 * SEQ + pin[q]] = yfu;
+      W[2 * SEQ + pin[q]] = f2;
+      W[3 * SEQ + pin[q]] = yfw;
+    }
+    // CFL maxima for this plane (check_cfl :211-213)
+    if (a.cflbits) {
+      double au = 0.0, av = 0.0, aw = 0.0;
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        if (!vq[q]) continue;
+        au = fmax(au, pabsmax(padd(U0[q], U1[q])));
+        av = fmax(av, pabsmax(V0[q]));
+        aw = fmax(aw, pabsmax(padd(W0[q], W1[q])));
+        if (LIN) {
+          au = fmax(au, pabsmax(padd(BU0[q], BU1[q])));
+          av = fmax(av, pabsmax(BV0[q]));
+          aw = fmax(aw, pabsmax(padd(BW0[q], BW1[q])));
+        }
+      }
+      au = wave_max(au); av = wave_max(av); aw = wave_max(aw);
+      if (lane == 0) {
+        unsigned long long* cb = a.cflbits + 3 * kk;
+        atomicMax(cb + 0, (unsigned long long)__double_as_longlong(au));
+        atomicMax(cb + 1, (unsigned long long)__double_as_longlong(av));
+        atomicMax(cb + 2, (unsigned long long)__double_as_longlong(aw));
+      }
+    }
+    wave_sync();
+    fft2p_wave<M, -1>(W, SEQ, 4, rl, rg, lane);
+    const long bst = (long)nz2 * kxp;
+    const long ab0 = ((long)((0 * nyp + kk) * 2) * nz2 + zz) * kxp;
+    const long ab1 = ((long)((1 * nyp + kk) * 2) * nz2 + zz) * kxp;
+    const long ab2 = ((long)((2 * nyp + kk) * 2) * nz2 + zz) * kxp;
+    for (int kq = lane; kq < nxh; kq += 64) {
+      const double2 wk = rootx[kq];
+      const double kxv = ops.kx[kq];
+      auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
+      const double2 X0 = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk), X1 = r2c_post_nl<M>(W + 1 * SEQ, M, kq, wk);
+      const double2 X2 = r2c_post_nl<M>(W + 2 * SEQ, M, kq, wk), X3 = r2c_post_nl<M>(W + 3 * SEQ, M, kq, wk);
+      a.AB[ab0 + kq] = cadd(X1, ikxm(X0));
+      a.AB[ab0 + bst + kq] = X2;
+      a.AB[ab2 + kq] = cadd(X3, ikxm(X2));
+    }
+    wave_sync();
+    // batch 2: w^2 and, below the top plane, the v fluxes (:183-195)
+    const int nb2 = top ? 1 : 4;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      if (!LIN) {
+        if (!top) { double2 t = psc(half, padd(V0[q], V1[q])); G1[q] = pmul(t, t); }
+        else G1[q] = z2;
+      } else {
+        if (!top) {
+          double2 t = psc(half, padd(V0[q], V1[q])), tb = psc(half, padd(BV0[q], BV1[q]));
+          G1[q] = pmul(psc(two, t), tb);
+        } else {
+          G1[q] = z2;
+        }
+      }
+      if (!vq[q]) continue;
+      W[0 * SEQ + pin[q]] = LIN ? pmul(psc(two, BW0[q]), W0[q]) : pmul(W0[q], W0[q]);
+      if (!top) {
+        double2 f5, f7;
+        if (!LIN) {
+          f5 = pmul(V0[q], padd(psc(i1, U0[q]), psc(i2, U1[q])));
+          f7 = pmul(V0[q], padd(psc(i1, W0[q]), psc(i2, W1[q])));
+        } else {
+          f5 = padd(pmul(BV0[q], padd(psc(i1, U0[q]), psc(i2, U1[q]))),
+                    pmul(V0[q], padd(psc(i1, BU0[q]), psc(i2, BU1[q]))));
+          f7 = padd(pmul(BV0[q], padd(psc(i1, W0[q]), psc(i2, W1[q]))),
+                    pmul(V0[q], padd(psc(i1, BW0[q]), psc(i2, BW1[q]))));
+        }
+        double2 yfv = psc(dnk, psub(G1[q], GV[q]));                                    // :188-189
+        if (hasbuf) {
+          const double tv = ops.target[1 * nyp + kk];
+          yfv = padd(yfv, pmul(bufv(q), make_double2(V0[q].x - tv, V0[q].y - tv)));
+        }
+        W[1 * SEQ + pin[q]] = f5;
+        W[2 * SEQ + pin[q]] = yfv;
+        W[3 * SEQ + pin[q]] = f7;
+      }
+    }
+    wave_sync();
+    fft2p_wave<M, -1>(W, SEQ, nb2, rl, rg, lane);
+    for (int kq = lane; kq < nxh; kq += 64) {
+      const double2 wk = rootx[kq];
+      const double kxv = ops.kx[kq];
+      auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
+      a.AB[ab2 + bst + kq] = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk);
+      if (!top) {
+        const double2 X5 = r2c_post_nl<M>(W + 1 * SEQ, M, kq, wk), X6 = r2c_post_nl<M>(W + 2 * SEQ, M, kq, wk);
+        const double2 X7 = r2c_post_nl<M>(W + 3 * SEQ, M, kq, wk);
+        a.AB[ab1 + kq] = cadd(X6, ikxm(X5));
+        a.AB[ab1 + bst + kq] = X7;
+      }
+    }
+    wave_sync();
+    // slide the window (:81, :170-171, :198)
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      FU[q] = Fu[q]; FW[q] = Fw[q]; GV[q] = G1[q];
+      U0[q] = U1[q]; V0[q] = V1[q]; W0[q] = W1[q];
+      BU0[q] = BU1[q]; BV0[q] = BV1[q]; BW0[q] = BW1[q];
+    }
+  }
+}
+
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a0,
                   hipStream_t s) {
   XPassArgs a = a0;
@@ -280,6 +556,23 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     }
   int nplanes = a.kb - a.ka + 1;
   int nch = (nplanes + a.chunk - 1) / a.chunk;
+  static const bool use_wave = [] {
+    const char* e = getenv("CHANFLOW_XPASS_WAVE");
+    return !(e && atoi(e) == 0);
+  }();
+  if (use_wave && ct && M <= 192) {
+    dim3 gw((g.nz2 + 3) / 4, nch);
+    size_t ldw = sizeof(double2) * (M + 4 * 4 * fft_seq_cells(M));
+#define XPASSW_CASE(N)                                                                            \
+  if constexpr (N > 0 && N <= 192) {                                                              \
+    if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N>), gw, dim3(256), ldw, s, g, tw, ops, a);   \
+    else hipLaunchKernelGGL((k_xpass_w<false, N>), gw, dim3(256), ldw, s, g, tw, ops, a);         \
+  }
+    FFT_DISPATCH(M, XPASSW_CASE)
+#undef XPASSW_CASE
+    LAUNCH_CHECK("k_xpass_w");
+    return;
+  }
   dim3 grid(g.nz2 / rz, nch);
   size_t lds = sizeof(double2) * (8 * rz * fft_seq_cells(M) + (ct ? M : 0));
 #define XPASS_CASE(N)                                                                 \

@@ -238,3 +238,68 @@ def test_cfl_abort(g0):
     with pytest.raises(cf.ChanError, match="CFL IS TOO BIG"):
         ch.nonlinear(1, 1, dt=1.0)
     ch.close()
+
+
+@pytest.mark.parametrize("dims", [(16, 17, 16, 24, 24), (32, 17, 16, 48, 24), (64, 17, 32, 96, 48),
+                                  (96, 17, 16, 144, 24), (128, 9, 16, 192, 24), (192, 9, 16, 288, 24),
+                                  (384, 9, 8, 576, 12)])
+def test_nonlinear_x_lengths(dims):
+    """One nonlinear pass at x lengths that take each x-pass variant: the
+    workgroup kernel with the runtime-planned transform (nx2/2 = 12), the
+    wave-per-row kernel with one, two and three points per lane
+    (nx2/2 = 24 ... 144), and the workgroup kernel with a compile-time
+    length (nx2/2 = 288)."""
+    nx, ny, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, ny, nz, nx2, nz2, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1)
+    ch = make_channel(g)
+    s = __import__("helpers").synth()
+    U = O.remove_initial_divergence(g, s.laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=11, amp=5e-2, kmax=6.0))
+    ch.set_state(data1=U)
+    dt, cfl = ch.nonlinear(1, 1, dt=0.002)
+    ref, dtr, cflr = O.nonlinear(g, U, step=1, rk_step=1, dt=0.002, return_cfl=True)
+    assert rel_l2(ch.get_data1(), ref) < TOL1
+    assert abs(cfl - cflr) <= 1e-14 * abs(cflr)
+    ch.close()
+
+
+@pytest.fixture(scope="module")
+def g1():
+    """Headline shape: Re_c 3300, box 4pi x 2 x 4pi/3, 192x129x192, 288x288, stretch 2.75."""
+    return O.make_grid(192, 129, 192, 288, 288, 4.0, 3.0, 3300.0, stretch=2.75, fix_mass=1)
+
+
+def _headline_field(g):
+    s = __import__("helpers").synth()
+    return s.turbulent_like(g.nx, g.ny, g.nz, g.yh, g.y, seed=3, rms=0.1, re_tau=180.0)
+
+
+def test_headline_shape_nonlinear(g1):
+    ch = make_channel(g1)
+    U = _headline_field(g1)
+    ch.set_state(data1=U)
+    dt, cfl = ch.nonlinear(1, 1, dt=0.005)
+    ref, dtr, cflr = O.nonlinear(g1, U, step=1, rk_step=1, dt=0.005, return_cfl=True)
+    assert rel_l2(ch.get_data1(), ref) < TOL1
+    assert abs(cfl - cflr) <= 1e-14 * abs(cflr)
+    ch.close()
+
+
+def test_headline_shape_one_step(g1):
+    """Initialisation plus one RK3 step at the headline shape: rel L2 <= 1e-11
+    for u, v, w, the pressure and the next RK register; divergence <= 1e-12."""
+    ch = make_channel(g1)
+    U0 = _headline_field(g1)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.005, p_grad=-2.0 / g1.re)
+    ch.initialize_flow()
+    st = O.initial_state(g1, U0, dt=0.005)
+    ch.step(1, step0=1)
+    O.rk3_step(g1, st, 1)
+    d1, d2, d4 = ch.get_state()
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-11, c
+    assert rel_l2(d2, O.to_fortran_data2(st.R)) <= 1e-11
+    assert rel_l2(d4, O.to_fortran_data4(st.P)) <= 1e-10
+    assert ch.divergence_max() <= 1e-12
+    assert abs(ch.get_scalars()["p_grad"] - st.p_grad) <= 1e-10 * abs(st.p_grad)
+    ch.close()
