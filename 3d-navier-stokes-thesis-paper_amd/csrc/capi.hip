@@ -382,7 +382,7 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     const long zsz = 3L * nyp * p.nz2 * c->kxpad;
     c->d_Z = (double2*)dalloc(sizeof(double2) * zsz);
     c->d_AB = (double2*)dalloc(sizeof(double2) * 2 * zsz);
-    c->d_scr = (double2*)dalloc((sizeof(double2) * 4 + sizeof(double) * 2) * nyp * nm);
+    c->d_scr = (double2*)dalloc((sizeof(double2) * 4 + sizeof(double) * 4) * nyp * nm);
     CHAN_CHECK(hipMemset(c->d_vel, 0, vel_bytes(c)));
     CHAN_CHECK(hipMemset(c->d_nl, 0, vel_bytes(c)));
     CHAN_CHECK(hipMemset(c->d_R, 0, vel_bytes(c)));

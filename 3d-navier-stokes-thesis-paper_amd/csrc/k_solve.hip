@@ -402,6 +402,304 @@ __global__ __launch_bounds__(64) void k_solve(DevOps ops, SolveArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Component-parallel variant: three lanes per (kx,kz) mode, lane c owning
+// velocity component c (0 = u, 1 = v, 2 = w), five modes per 16-lane row
+// (lane 15 of each row idles), 20 modes per wavefront.  Each lane runs the
+// RHS assembly, Helmholtz elimination, back substitution, projection and
+// next-register assembly of its own component; the Poisson elimination, its
+// back substitution and the pressure update are per-mode and run
+// redundantly on the three lanes, which exchange their divergence terms
+// with lane shuffles.  Arithmetic and its order per value are those of
+// k_solve above.
+//
+// Addressing: every array is indexed as base + (32-bit byte offset), with
+// the offset advanced once per row, so a load costs one vector add.
+// Scratch (the a.scr buffer): SX[3][ny+1][modes] (f, then x, per
+// component), SP[ny+1][modes] (Poisson f), SC[3][ny+1][modes] (Helmholtz
+// c'), SA[ny+1][modes] (Poisson a').
+// ---------------------------------------------------------------------------
+struct RowA3 { double2 n, r, p; };
+struct RowB3 { double2 s; double cc; };
+struct RowC3 { double2 sp, sx, p, n; double sa; };
+
+__device__ __forceinline__ double2 ldo(const double2* b, uint32_t off) {
+  return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(b) + off);
+}
+__device__ __forceinline__ double ldo1(const double* b, uint32_t off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(b) + off);
+}
+__device__ __forceinline__ void sto(double2* b, uint32_t off, double2 v) {
+  *reinterpret_cast<double2*>(reinterpret_cast<char*>(b) + off) = v;
+}
+__device__ __forceinline__ void sto1(double* b, uint32_t off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(b) + off) = v;
+}
+__device__ __forceinline__ double2 shfl2(double2 v, int src) {
+  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+template <int BL>
+__global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
+  const int lane = threadIdx.x;
+  const int sub = lane & 15, g5 = sub / 3, c = sub - 3 * g5;
+  const long nm = (long)a.nxh * a.nz;
+  const long mg = (long)blockIdx.x * 20 + (lane >> 4) * 5 + g5;
+  const bool valid = (sub < 15) && (mg < nm);
+  const long m = valid ? mg : 0;
+  const int ix = (int)(m / a.nz), iz = (int)(m - (long)ix * a.nz);
+  const int ny = a.ny, nyp = ny + 1;
+  const double kxv = ops.kx[ix], kzv = ops.kz[iz];
+  const double k2 = kxv * kxv + kzv * kzv;
+  const bool mean = valid && (k2 == 0.0);
+  const bool cv1 = (c == 1);
+  const bool own = valid && (c == 0);   // stores the per-mode results
+  const int gbase = lane - c;           // lane of component 0 of this mode
+  const int r = a.rk - 1;
+  const bool first = (r == 0);
+  const double dt = a.sc->dt;
+  const double re = a.re;
+
+  const uint32_t rowB = (uint32_t)(nm * 16), cmpB = rowB * (uint32_t)nyp;
+  const uint32_t om0 = (uint32_t)(m * 16);             // mode offset (per-mode arrays)
+  const uint32_t oc0 = (uint32_t)c * cmpB + om0;      // component offset
+  auto OC = [&](int j) { return oc0 + (uint32_t)j * rowB; };
+  auto OM = [&](int j) { return om0 + (uint32_t)j * rowB; };
+
+  const double2* __restrict__ NL = a.nl;
+  double2* __restrict__ R = a.R;
+  double2* __restrict__ P = a.P;
+  double2* __restrict__ U = a.vel;
+  double2* __restrict__ SX = a.scr;
+  double2* __restrict__ SP = a.scr + 3L * nyp * nm;
+  double* __restrict__ SC = reinterpret_cast<double*>(a.scr + 4L * nyp * nm);
+  double* __restrict__ SA = SC + 3L * nyp * nm;
+  const double* d2h = ops.d2h;
+  const double* d2n = ops.d2n;
+  const double* dm = ops.dm;
+  auto neg = [](double2 v) { return make_double2(-v.x, -v.y); };
+
+  const double vis_imp = c_rkd[r] * (dt / re);
+  const double rk_press = (c_rkc[r] + c_rkd[r]) * dt;
+  const double coef = c_rkb[r] * dt;
+  const double vis_exp1 = c_rkc[r] * (dt / re);
+  const double alpha = 1.0 + vis_imp * k2;
+  const double kc = (c == 0) ? kxv : kzv;        // wavenumber of the u or w pressure gradient
+  const int jt = cv1 ? ny - 1 : ny;              // Dirichlet top row of this component
+
+  const double2 topc = a.cbc[(0L * 3 + c) * nm + m];
+  const double2 botc = a.cbc[(1L * 3 + c) * nm + m];
+
+  // mean-mode pressure gradient (compute_p_grad, :369-373)
+  double p_grad = a.sc->p_grad;
+  if (mean && a.fix_mass == 1) {
+    double tmp = 0.0;
+#pragma unroll 8
+    for (int j = 1; j < ny; ++j) tmp = tmp + ops.dyh[j] * (-ldo(NL, OM(j)).x);
+    p_grad = -tmp + 0.5 * (a.dudy[ny - 1] - a.dudy[0]) / re;
+    if (own) a.sc->p_grad = p_grad;
+  }
+  const double mean_shift = (c == 0) ? a.cosb * rk_press * p_grad : (c == 2 ? a.sinb * rk_press * p_grad : 0.0);
+
+  // ---------------- pass A: RHS and forward elimination (own component) ----------------
+  double cp = 0.0;
+  double2 f = botc;
+  if (valid) { sto(SX, OC(0), f); sto1(SC, OC(0) >> 1, 0.0); }
+  double2 Rm = ldo(R, OC(0)), Rc = ldo(R, OC(1));
+  double2 Pc = first ? ldo(P, OM(1)) : cz();
+  auto loadA = [&](int j) {
+    RowA3 d;
+    d.n = ldo(NL, OC(j));
+    d.r = ldo(R, OC(j + 1));
+    d.p = first ? ldo(P, OM(j + 1)) : cz();
+    return d;
+  };
+  auto compA = [&](int j, const RowA3& d) {
+    const double h0 = d2h[3 * j], h1 = d2h[3 * j + 1], h2 = d2h[3 * j + 2];
+    const double n0 = d2n[3 * j], n1 = d2n[3 * j + 1], n2 = d2n[3 * j + 2];
+    const double a0 = cv1 ? n0 : h0, a1 = cv1 ? n1 : h1, a2 = cv1 ? n2 : h2;
+    double2 v = cadd(Rc, rmul(coef, neg(d.n)));
+    if (first) {
+      // add_explicit_diffusive_terms(cv, cru) (:339-351)
+      double2 dd = cadd(cadd(rmul(a0, Rm), rmul(a1, Rc)), rmul(a2, d.r));
+      double2 ve = cadd(v, rmul(vis_exp1, csub(dd, rmul(k2, Rc))));
+      if (j < jt) v = ve;
+      if (!mean) {  // subtract_pressure_gradient(cv, cp, rk_press) (:54-77)
+        if (cv1) {
+          if (j <= ny - 2) v = csub(v, rmul(rk_press * ops.dn[j], csub(d.p, Pc)));
+        } else {
+          v = csub(v, imul(rk_press * kc, Pc));
+        }
+      }
+    }
+    if (mean && !cv1) v = csub(v, make_double2(mean_shift, 0.0));  // (:220-221)
+    if (j < jt) {
+      const double ah = -vis_imp * a0, bh = -vis_imp * a1 + alpha, ch = -vis_imp * a2;
+      const double bp = bh - ah * cp;
+      const double2 gg = csub(v, rmul(ah, f));
+      const double rb = 1.0 / bp;
+      cp = ch * rb;
+      f = rmul(rb, gg);
+    } else {
+      cp = 0.0;
+      f = topc;
+    }
+    if (valid) { sto(SX, OC(j), f); sto1(SC, OC(j) >> 1, cp); }
+    Rm = Rc; Rc = d.r; Pc = d.p;
+  };
+  rows_up<BL>(1, ny - 1, loadA, compA);
+  // row ny: Dirichlet row of u and w; v has no row ny (its value there is zero)
+  if (valid) { sto(SX, OC(ny), cv1 ? cz() : topc); sto1(SC, OC(ny) >> 1, 0.0); }
+
+  // ---------------- pass B: back substitution + Poisson elimination ----------------
+  const double tmpc = 1.0 / rk_press;
+  const double alp = -k2;
+  double ap = 0.0;
+  double2 fp = cz();
+  auto poisson_elim = [&](int i, double2 rhs) {  // reversed Thomas, row i (descending)
+    const double* dh = ops.dh; const double* dn = ops.dn;
+    double a_, b_, c_;
+    if (i == 0) { a_ = 0.0; b_ = -dn[0]; c_ = dn[0]; }
+    else if (i == 1) { a_ = dh[1] * dn[0]; b_ = -dh[1] * dn[0] - dh[1] * dn[1] + alp; c_ = dh[1] * dn[1]; }
+    else if (i == ny - 1) {
+      a_ = dh[ny - 1] * dn[ny - 2]; b_ = -dh[ny - 1] * dn[ny - 2] - dh[ny - 1] * dn[ny - 1] + alp;
+      c_ = dh[ny - 1] * dn[ny - 1];
+    } else if (i == ny) { a_ = -dn[ny - 1]; b_ = dn[ny - 1]; c_ = 0.0; }
+    else { a_ = d2h[3 * i]; b_ = d2h[3 * i + 1] + alp; c_ = d2h[3 * i + 2]; }
+    const double bp2 = (i == ny) ? b_ : b_ - c_ * ap;
+    const double2 gg = (i == ny) ? rhs : csub(rhs, rmul(c_, fp));
+    const double rb2 = 1.0 / bp2;
+    ap = a_ * rb2;
+    fp = rmul(rb2, gg);
+    if (own && !mean) { sto(SP, OM(i), fp); sto1(SA, OM(i) >> 1, ap); }
+  };
+  double2 x1 = cv1 ? cz() : topc;  // x at row j+1
+  if (mean && cv1) x1 = cz();
+  auto loadB = [&](int j) {
+    RowB3 d;
+    d.s = ldo(SX, OC(j));
+    d.cc = ldo1(SC, OC(j) >> 1);
+    return d;
+  };
+  auto compB = [&](int j, const RowB3& d) {
+    double2 x = csub(d.s, rmul(d.cc, x1));
+    if (mean && cv1) x = botc;  // (:261-263)
+    if (valid) sto(SX, OC(j), x);
+    // divergence of the provisional velocity at row i = j+1 (strip_divergence rhs, :28-31)
+    const int i = j + 1;
+    double2 t;
+    if (cv1) t = rmul(ops.dh[i <= ny - 1 ? i : ny - 1], csub(x1, x));
+    else t = imul(kc, x1);
+    const double2 t0 = shfl2(t, gbase), t1 = shfl2(t, gbase + 1), t2 = shfl2(t, gbase + 2);
+    if (!mean) {
+      double2 rhs = cz();
+      if (i <= ny - 1) rhs = rmul(tmpc, cadd(cadd(t0, t1), t2));
+      poisson_elim(i, rhs);
+    }
+    x1 = x;
+  };
+  rows_down<BL>(ny - 1, 0, loadB, compB);
+  if (!mean) poisson_elim(0, cz());
+
+  // ---------------- pass C: pressure correction, projection, next RK register ----------------
+  const bool more = (a.rk < 3);
+  const double coef2 = more ? dt * c_rka[r + 1] : 0.0;
+  const double vis_exp2 = more ? (dt / re) * c_rkc[r + 1] : 0.0;
+  const double rk_press2 = more ? (c_rkc[r + 1] + c_rkd[r + 1]) * dt : 0.0;
+  const double alpha2 = 1.0 + vis_imp * k2;
+  double2 w0 = cz(), w1 = cz(), w2 = cz(), w3 = cz();   // own component at rows i-3..i
+  double2 Pw0 = cz(), Pw1 = cz();                        // P_new at rows i-1, i
+  // next RK register of the own component at row j (from cv at row j, its
+  // D2H/D2N stencil rows s0..s2, P_new at j, j+1 and NL at j)
+  auto buildR = [&](int j, double2 cvj, double2 s0, double2 s1, double2 s2, double2 Pj, double2 Pj1, double2 nj) {
+    double2 out = cadd(cvj, rmul(coef2, neg(nj)));
+    const double h0 = d2h[3 * j], h1 = d2h[3 * j + 1], h2 = d2h[3 * j + 2];
+    if (!cv1) {
+      double2 dd = cadd(cadd(rmul(h0, s0), rmul(h1, s1)), rmul(h2, s2));
+      out = cadd(out, rmul(vis_exp2, csub(dd, rmul(k2, cvj))));
+    } else if (j < ny) {
+      const double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+      double2 dd = cadd(cadd(rmul(b0, s0), rmul(b1, s1)), rmul(b2, s2));
+      out = cadd(out, rmul(vis_exp2, csub(dd, rmul(k2, cvj))));
+    }
+    if (!mean) {
+      if (cv1) {
+        if (j >= 1 && j <= ny - 2) out = csub(out, rmul(rk_press2 * ops.dn[j], csub(Pj1, Pj)));
+      } else if (j >= 1 && j <= ny - 1) {
+        out = csub(out, imul(rk_press2 * kc, Pj));
+      }
+    }
+    if (valid) sto(R, OC(j), out);
+  };
+  // projection of row i: s = provisional own component, Pold = P(i), f0 = phi(i),
+  // fn = phi(i+1), st0..st2 = phi on the D2H stencil rows of i, nb = NL(i-1)
+  auto c2row = [&](int i, double2 s, double2 Pold, double2 f0, double2 fn, double2 st0, double2 st1,
+                   double2 st2, double2 nb) {
+    double2 cvv = s;
+    double2 pn = cz();
+    if (!mean) {
+      if (cv1) {
+        if (i >= 1 && i <= ny - 2) cvv = csub(cvv, rmul(rk_press * ops.dn[i], csub(fn, f0)));
+      } else if (i >= 1 && i <= ny - 1) {
+        cvv = csub(cvv, imul(rk_press * kc, f0));
+      }
+      double2 d2 = cadd(cadd(rmul(d2h[3 * i], st0), rmul(d2h[3 * i + 1], st1)), rmul(d2h[3 * i + 2], st2));
+      pn = csub(cadd(Pold, rmul(alpha2, f0)), rmul(vis_imp, d2));
+    }
+    if (valid) sto(U, OC(i), cvv);
+    if (own) sto(P, OM(i), pn);
+    w0 = w1; w1 = w2; w2 = w3; w3 = cvv;
+    Pw0 = Pw1; Pw1 = pn;
+    if (mean && c == 0) {  // d_um_dy update from the new mean profile (:84-89)
+      if (i >= 2 && i <= ny - 1) a.dudy[i - 1] = (ops.dn[i - 1] * (w3.x - w2.x));
+      if (i == 2) a.dudy[0] = dm[0] * w1.x + dm[1] * w2.x + dm[2] * w3.x;
+      if (i == ny) a.dudy[ny - 1] = dm[3] * w1.x + dm[4] * w2.x + dm[5] * w3.x;
+    }
+    if (!more) {
+      if (valid) sto(R, OC(i), cvv);
+      return;
+    }
+    if (i >= 2) {
+      if (i == 2)  // row 0 (clamped stencils on rows 0..2, no gradient)
+        buildR(0, w1, w1, w2, w3, cz(), cz(), ldo(NL, OC(0)));
+      // row j = i-1: centred stencil on rows i-2..i; D2N clamped at the last v row
+      const int j = i - 1;
+      if (j == ny - 1 && cv1) buildR(j, w2, w0, w1, w2, Pw0, Pw1, nb);
+      else buildR(j, w2, w1, w2, w3, Pw0, Pw1, nb);
+    }
+  };
+  auto lsp = [&](int i) { return mean ? cz() : ldo(SP, OM(i)); };
+  auto lsa = [&](int i) { return mean ? 0.0 : ldo1(SA, OM(i) >> 1); };
+  const double2 ph0 = lsp(0);
+  const double2 ph1 = csub(lsp(1), rmul(lsa(1), ph0));
+  const double2 ph2 = csub(lsp(2), rmul(lsa(2), ph1));
+  const double2 pb = make_double2((0.0 - dm[1] * ph1.x - dm[2] * ph2.x) / dm[0],
+                                  (0.0 - dm[1] * ph1.y - dm[2] * ph2.y) / dm[0]);
+  c2row(0, ldo(SX, OC(0)), ldo(P, OM(0)), pb, cz(), pb, ph1, ph2, cz());
+  c2row(1, ldo(SX, OC(1)), ldo(P, OM(1)), ph1, ph2, pb, ph1, ph2, cz());
+  double2 q0 = pb, q1 = ph1, q2 = ph2, ph = ph2;
+  auto loadC = [&](int i) {
+    RowC3 d;
+    d.sp = lsp(i); d.sa = lsa(i);
+    d.sx = ldo(SX, OC(i - 1));
+    d.p = ldo(P, OM(i - 1));
+    d.n = more ? ldo(NL, OC(i - 2)) : cz();
+    return d;
+  };
+  auto compC = [&](int i, const RowC3& d) {
+    ph = csub(d.sp, rmul(d.sa, ph));
+    q0 = q1; q1 = q2; q2 = ph;
+    c2row(i - 1, d.sx, d.p, q1, q2, q0, q1, q2, d.n);
+  };
+  rows_up<BL>(3, ny - 1, loadC, compC);
+  // top wall: phi(ny) from phi(ny-2), phi(ny-1)
+  const double2 pt = make_double2((0.0 - dm[3] * q1.x - dm[4] * q2.x) / dm[5],
+                                  (0.0 - dm[3] * q1.y - dm[4] * q2.y) / dm[5]);
+  c2row(ny - 1, ldo(SX, OC(ny - 1)), ldo(P, OM(ny - 1)), q2, pt, q1, q2, pt, more ? ldo(NL, OC(ny - 2)) : cz());
+  c2row(ny, ldo(SX, OC(ny)), ldo(P, OM(ny)), pt, cz(), q1, q2, pt, more ? ldo(NL, OC(ny - 1)) : cz());
+  if (more) buildR(ny, w3, w1, w2, w3, Pw1, cz(), ldo(NL, OC(ny)));
+}
+
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));
@@ -411,6 +709,20 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
     int v = e ? atoi(e) : 8;
     return (v == 4 || v == 16) ? v : 8;
   }();
+  // component-parallel kernel unless CHANFLOW_SOLVE_LANES=1 or the byte
+  // offsets of one array would not fit 32 bits
+  static const int lanes = [] {
+    const char* e = getenv("CHANFLOW_SOLVE_LANES");
+    return (e && atoi(e) == 1) ? 1 : 3;
+  }();
+  if (lanes == 3 && 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9) {
+    dim3 g3((unsigned)((nm + 19) / 20));
+    if (bl == 4) hipLaunchKernelGGL(k_solve3<4>, g3, dim3(64), 0, s, ops, a);
+    else if (bl == 16) hipLaunchKernelGGL(k_solve3<16>, g3, dim3(64), 0, s, ops, a);
+    else hipLaunchKernelGGL(k_solve3<8>, g3, dim3(64), 0, s, ops, a);
+    LAUNCH_CHECK("k_solve3");
+    return;
+  }
   if (bl == 4) hipLaunchKernelGGL(k_solve<4>, grid, dim3(64), 0, s, ops, a);
   else if (bl == 16) hipLaunchKernelGGL(k_solve<16>, grid, dim3(64), 0, s, ops, a);
   else hipLaunchKernelGGL(k_solve<8>, grid, dim3(64), 0, s, ops, a);
