@@ -76,7 +76,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprof), print no JSON")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many MiB (a plain copy kernel "
+                         "of known size, used to calibrate rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_mib:
+        import torch
+        n = args.copy_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {args.copy_mib} MiB x 5", flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(the multi-GPU sharding is a design note, not implemented)"}))
