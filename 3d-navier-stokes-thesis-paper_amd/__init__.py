@@ -47,7 +47,8 @@ ENTRY_POINTS = (
     "chan_xz_fft_real_to_complex", "chan_compute_divergence_of_fluxes", "chan_small_fft_complex_to_real",
     "chan_small_fft_real_to_complex", "chan_solve_helmholtz", "chan_solve_p", "chan_strip_divergence",
     "chan_linearized_matvec", "chan_divergence_max", "chan_mass_flux", "chan_set_timing",
-    "chan_get_kernel_times", "chan_sync",
+    "chan_get_kernel_times", "chan_sync", "chan_io_last_error", "chan_savevel_write",
+    "chan_savevel_read_header", "chan_savevel_read", "chan_save_velocity", "chan_read_velocity",
 )
 
 _lib = None
@@ -99,6 +100,14 @@ def load_library(path: str = LIBPATH):
         "chan_set_timing": (C.c_int, [vp, C.c_int]),
         "chan_get_kernel_times": (C.c_int, [vp, _dp, C.POINTER(C.c_long), C.c_int]),
         "chan_sync": (C.c_int, [vp]),
+        "chan_io_last_error": (C.c_char_p, []),
+        "chan_savevel_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, _dp]),
+        "chan_savevel_read_header": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]),
+        "chan_savevel_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
+        "chan_save_velocity": (C.c_int, [vp, C.c_char_p]),
+        "chan_read_velocity": (C.c_int, [vp, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -197,6 +206,13 @@ class Channel:
         d4 = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
         self._chk(self.lib.chan_get_state(self.h, _ptr(d1), _ptr(d2), _ptr(d4)))
         return d1, d2, d4
+
+    # -- checkpoints (savevel.out / newvel.out)
+    def save_velocity(self, path):
+        self._chk(self.lib.chan_save_velocity(self.h, os.fsencode(path)))
+
+    def read_velocity(self, path):
+        self._chk(self.lib.chan_read_velocity(self.h, os.fsencode(path)))
 
     def get_data1(self):
         d1 = np.empty((3, self.nyp, self.nxh, self.nz), np.complex128)
@@ -340,6 +356,36 @@ class Channel:
 
     def sync(self):
         self._chk(self.lib.chan_sync(self.h))
+
+
+def savevel_write(path, data1, lx, lz, re, dt, t_total):
+    """Write data1 complex [3][ny+1][nx/2+1][nz] (the Fortran data1(nz, nx/2+1, ny+1, 3))
+    in the savevel.out format; nx is taken as 2*(nx/2+1 - 1)."""
+    lib = load_library()
+    d1 = _c128(data1)
+    nc, nyp, nxh, nz = d1.shape
+    if nc != 3:
+        raise ValueError("data1 must have shape (3, ny+1, nx/2+1, nz)")
+    rc = lib.chan_savevel_write(os.fsencode(path), 2 * (nxh - 1), nyp - 1, nz, lx, lz, re, dt, t_total, _ptr(d1))
+    if rc != 0:
+        raise ChanError(lib.chan_io_last_error().decode())
+
+
+def savevel_read(path):
+    """Read a savevel.out file: returns (header dict, data1 [3][ny+1][nx/2+1][nz])."""
+    lib = load_library()
+    n = [C.c_int() for _ in range(3)]
+    r = [C.c_double() for _ in range(5)]
+    p = os.fsencode(path)
+    if lib.chan_savevel_read_header(p, *(C.byref(v) for v in n), *(C.byref(v) for v in r)) != 0:
+        raise ChanError(lib.chan_io_last_error().decode())
+    nx, ny, nz = (v.value for v in n)
+    hdr = dict(nx=nx, ny=ny, nz=nz, lx=r[0].value, lz=r[1].value, re=r[2].value, dt=r[3].value,
+               t_total=r[4].value)
+    d1 = np.empty((3, ny + 1, nx // 2 + 1, nz), np.complex128)
+    if lib.chan_savevel_read(p, nx, ny, nz, _ptr(d1)) != 0:
+        raise ChanError(lib.chan_io_last_error().decode())
+    return hdr, d1
 
 
 def version() -> str:

@@ -178,6 +178,26 @@ int chan_get_kernel_times(chan_ctx* ctx, double* ms_total, long* counts, int nki
 /* Synchronise the context stream and report a deferred device error. */
 int chan_sync(chan_ctx* ctx);
 
+/* Velocity checkpoints in the reference's savevel.out / newvel.out format:
+ * read_velocity and SAVE_VELOCITY (PFC/read_write.f90:21-176, 180-292),
+ * Fortran sequential unformatted records with 4-byte length markers: one
+ * header record NX, NY, NZ (int32), LX, LZ, RE, DT, T_TOTAL (float64), then
+ * one record per x-mode x = 1..nx/2+1 holding data1(1:nz, x, 1:ny+1, 1:3).
+ * The chan_savevel_* functions need no context and no GPU; their errors are
+ * reported by chan_io_last_error. */
+const char* chan_io_last_error(void);
+int chan_savevel_write(const char* path, int nx, int ny, int nz, double lx, double lz, double re, double dt,
+                       double t_total, const double* data1);
+int chan_savevel_read_header(const char* path, int* nx, int* ny, int* nz, double* lx, double* lz, double* re,
+                             double* dt, double* t_total);
+int chan_savevel_read(const char* path, int nx, int ny, int nz, double* data1);
+/* SAVE_VELOCITY of the device velocity, dt and t_total; read_velocity into
+ * the device: NX, NY, NZ, LX, LZ must match the context (else
+ * "INCONSISTENT PARAMETERS"), t_total is taken from the file and dt only
+ * when dt_flag == 1 (PFC/read_write.f90:36-63). */
+int chan_save_velocity(chan_ctx* ctx, const char* path);
+int chan_read_velocity(chan_ctx* ctx, const char* path);
+
 #ifdef __cplusplus
 }
 #endif
