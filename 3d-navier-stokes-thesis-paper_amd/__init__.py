@@ -49,6 +49,7 @@ ENTRY_POINTS = (
     "chan_linearized_matvec", "chan_divergence_max", "chan_mass_flux", "chan_set_timing",
     "chan_get_kernel_times", "chan_sync", "chan_io_last_error", "chan_savevel_write",
     "chan_savevel_read_header", "chan_savevel_read", "chan_save_velocity", "chan_read_velocity",
+    "chan_set_coordinates", "chan_l2_norm", "chan_mode_energy", "chan_plane_moments",
 )
 
 _lib = None
@@ -108,6 +109,10 @@ def load_library(path: str = LIBPATH):
         "chan_savevel_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
         "chan_save_velocity": (C.c_int, [vp, C.c_char_p]),
         "chan_read_velocity": (C.c_int, [vp, C.c_char_p]),
+        "chan_set_coordinates": (C.c_int, [vp, _dp]),
+        "chan_l2_norm": (C.c_int, [vp, _dp, _dp]),
+        "chan_mode_energy": (C.c_int, [vp, _dp]),
+        "chan_plane_moments": (C.c_int, [vp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -186,6 +191,8 @@ class Channel:
         """Upload the operators of an oracle-style grid object (same attribute names)."""
         self.upload_operators(g.d2h, g.d2n, g.dm, g.dh, g.dn, g.hn, g.iu, g.dyh, g.dy, g.kx, g.kz,
                               getattr(g, "buffer", None), getattr(g, "target_inflow", None))
+        if getattr(g, "y", None) is not None:
+            self.set_coordinates(g.y)
 
     def build_operators(self, stretch=2.75):
         y = np.empty(self.ny)
@@ -206,6 +213,27 @@ class Channel:
         d4 = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
         self._chk(self.lib.chan_get_state(self.h, _ptr(d1), _ptr(d2), _ptr(d4)))
         return d1, d2, d4
+
+    # -- diagnostics
+    def set_coordinates(self, y):
+        self._chk(self.lib.chan_set_coordinates(self.h, _ptr(_f64(y))))
+
+    def l2_norm(self):
+        l2, drag = C.c_double(), C.c_double()
+        self._chk(self.lib.chan_l2_norm(self.h, C.byref(l2), C.byref(drag)))
+        return l2.value, drag.value
+
+    def mode_energy(self):
+        """Per-mode kinetic energy as [nx/2+1, nz]."""
+        ke = np.empty((self.nz, self.nxh), np.float64)
+        self._chk(self.lib.chan_mode_energy(self.h, _ptr(ke)))
+        return ke.T.copy()
+
+    def plane_moments(self):
+        """[15, ny+1]: means, 2nd, 3rd, 4th central moments of u, v, w, fluxes <u_i' v'>."""
+        out = np.empty((15, self.nyp), np.float64)
+        self._chk(self.lib.chan_plane_moments(self.h, _ptr(out)))
+        return out
 
     # -- checkpoints (savevel.out / newvel.out)
     def save_velocity(self, path):

@@ -521,6 +521,7 @@ int chan_build_operators(chan_ctx* c, double stretch, double* out_y, double* out
     o.hn = hn.data(); o.iu = iu.data(); o.dyh = dyh.data(); o.dy = dy.data(); o.kx = kx.data(); o.kz = kz.data();
     o.buffer = nullptr; o.target_inflow = nullptr;
     if (chan_upload_operators(c, &o)) throw chan_error(c->last_error);
+    c->h_y = y;
     if (out_y) std::memcpy(out_y, y.data(), sizeof(double) * ny);
     if (out_yh) std::memcpy(out_yh, yh.data(), sizeof(double) * nyp);
   });
@@ -896,6 +897,77 @@ int chan_divergence_max(chan_ctx* c, double* out) {
     unsigned long long h;
     d2h(c, &h, d, 8);
     std::memcpy(out, &h, 8);
+  });
+}
+
+int chan_set_coordinates(chan_ctx* c, const double* y) {
+  return guarded(c, [&] {
+    if (!y) throw chan_error("null argument");
+    c->h_y.assign(y, y + c->prm.ny);
+  });
+}
+
+int chan_l2_norm(chan_ctx* c, double* l2, double* drag) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if ((int)c->h_y.size() != c->prm.ny) throw chan_error("coordinates not set (chan_set_coordinates)");
+    ensure_vel_in_vel(c);
+    const XZGeom& g = c->gd;
+    const int ny = c->prm.ny, nyp = c->nyp;
+    const long ps = (long)g.nx2 * g.nz2;
+    if ((size_t)nyp * ps * sizeof(double) > (size_t)2 * 3 * nyp * g.nz2 * g.kxpad * sizeof(double2))
+      throw chan_error("l2_norm: physical planes do not fit the transform buffer");
+    // laminar profile at the v-point coordinate y(plane); plane ny+1 uses y(ny)
+    // (the reference reads y(ny+1), one past the end of y)
+    std::vector<double> lam(nyp);
+    for (int j = 0; j < nyp; ++j) {
+      const double yy = c->h_y[std::min(j, ny - 1)];
+      lam[j] = 1.0 - yy * yy;
+    }
+    char* s = (char*)stage(c, sizeof(double) * (nyp + 3 * nyp + 2) + 64);
+    double* d_lam = (double*)s;
+    double* part = d_lam + nyp;
+    double* res = part + 3 * nyp;
+    h2d(c, d_lam, lam.data(), sizeof(double) * nyp);
+    double* phys = (double*)c->d_AB;
+    for (int comp = 0; comp < 3; ++comp) {
+      dev_c2r(c, g, c->d_vel + (long)comp * nyp * nmodes(c), nyp, phys);
+      launch_l2_planes(phys, ps, nyp, comp == 0 ? d_lam : nullptr, c->ops.dy, part + (long)comp * nyp, c->stream);
+    }
+    launch_l2_final(part, 3 * nyp, (double)ps, c->d_dudy, ny, res, c->stream);
+    double h[2];
+    d2h(c, h, res, sizeof(h));
+    if (l2) *l2 = h[0];
+    if (drag) *drag = h[1];
+  });
+}
+
+int chan_mode_energy(chan_ctx* c, double* ke) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    const size_t b = sizeof(double) * nmodes(c);
+    double* d = (double*)stage(c, b);
+    launch_mode_energy(c->ops, c->prm.ny, c->nxh, c->prm.nz, c->d_vel, d, c->stream);
+    d2h(c, ke, d, b);
+  });
+}
+
+int chan_plane_moments(chan_ctx* c, double* out) {
+  return guarded(c, [&] {
+    check_ops(c);
+    ensure_vel_in_vel(c);
+    const XZGeom& g = c->gs;
+    const int nyp = c->nyp;
+    const long ps = (long)g.nx2 * g.nz2, cs = (long)nyp * ps;
+    if ((size_t)3 * cs * sizeof(double) > (size_t)2 * 3 * nyp * c->gd.nz2 * c->gd.kxpad * sizeof(double2))
+      throw chan_error("plane_moments: physical planes do not fit the transform buffer");
+    double* phys = (double*)c->d_AB;
+    dev_c2r(c, g, c->d_vel, 3 * nyp, phys);
+    const size_t b = sizeof(double) * 15 * nyp;
+    double* d = (double*)stage(c, b);
+    launch_plane_moments(phys, ps, cs, nyp, d, c->stream);
+    d2h(c, out, d, b);
   });
 }
 

@@ -90,6 +90,7 @@ struct chan_ctx {
   double* d_ops = nullptr;  // packed
   DevOps ops{};
   double h_dm[12] = {0};         // host copy of dm(3,4) for the feedback law
+  std::vector<double> h_y;       // y(ny) (v-point coordinates), for l2_norm
   bool has_buffer = false;
   // state (plane-major, kz fastest): vel and nl are [3][ny+1][nxh][nz]
   double2* d_vel = nullptr;
@@ -156,6 +157,11 @@ struct SolveArgs {
   int fix_mass;
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
+void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,
+                      hipStream_t s);
+void launch_l2_final(const double* part, int n, double npts, const double* dudy, int ny, double* out, hipStream_t s);
+void launch_mode_energy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel, double* ke, hipStream_t s);
+void launch_plane_moments(const double* phys, long ps, long cs, int nyp, double* out, hipStream_t s);
 void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
                     double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
                     double re, hipStream_t s);

@@ -178,6 +178,23 @@ int chan_get_kernel_times(chan_ctx* ctx, double* ms_total, long* counts, int nki
 /* Synchronise the context stream and report a deferred device error. */
 int chan_sync(chan_ctx* ctx);
 
+/* Diagnostics of the device velocity.
+ * chan_set_coordinates: y(ny), the v-point coordinates (PFC/setup.f90:601-606),
+ *   needed by chan_l2_norm when the operators were uploaded rather than built.
+ * chan_l2_norm: l2_norm (PFC/read_write.f90:1432-1462) on the dealiased grid,
+ *   l2 = sqrt(sum_planes,comps dy(j) sum_xz (u - lam)^2 / (nx2 nz2) / 2) with
+ *   lam = 1 - y(j)^2 for u (the reference reads y(ny+1) one past the end of y;
+ *   y(ny) is used for that plane), and drag = d_um_dy(1) - d_um_dy(ny).
+ * chan_mode_energy: ke(nx/2+1, nz) = int_y(|u|^2 + |w|^2 + |v at yh|^2, dyh)
+ *   per mode (PFC/pass2.f90:113-118).
+ * chan_plane_moments: out(ny+1, 15) on the nx x nz grid, per plane: means of
+ *   u, v, w, their 2nd, 3rd and 4th central moments and <u_i' v'>
+ *   (compute_velocity_moments, PFC/stats.f90:423-440, for one field). */
+int chan_set_coordinates(chan_ctx* ctx, const double* y);
+int chan_l2_norm(chan_ctx* ctx, double* l2, double* drag);
+int chan_mode_energy(chan_ctx* ctx, double* ke);
+int chan_plane_moments(chan_ctx* ctx, double* out);
+
 /* Velocity checkpoints in the reference's savevel.out / newvel.out format:
  * read_velocity and SAVE_VELOCITY (PFC/read_write.f90:21-176, 180-292),
  * Fortran sequential unformatted records with 4-byte length markers: one

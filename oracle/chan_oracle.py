@@ -800,6 +800,61 @@ def max_divergence(g: Grid, U):
 
 
 # ---------------------------------------------------------------------------
+# Diagnostics: l2_norm (PFC/read_write.f90:1432-1462), per-mode kinetic
+# energy (PFC/pass2.f90:113-118), per-plane velocity moments
+# (PFC/stats.f90:423-440, one instantaneous field: wgt_old = 0, wgt_new = 1)
+# ---------------------------------------------------------------------------
+def l2_norm(g: Grid, U):
+    """The l2 value l2_norm writes to l2.out.
+    u is taken on the dealiased grid, the streamwise component minus the
+    laminar profile 1 - y(plane)^2 at the v-point coordinate y(plane), and
+    each plane is weighted with dy(plane); l2 = sqrt(sum/(nx2 nz2)/2).
+    y has ny entries while the loop reaches plane ny+1 (y(ny+1) is read one
+    past the end of the array, PFC/setup.f90:551): the top wall value
+    y(ny) is used for that plane here (parity unpinned for that term)."""
+    yext = np.concatenate([g.y, g.y[-1:]])
+    tot = 0.0
+    for j in range(g.ny + 1):
+        for c in range(3):
+            u = xz_c2r(g, U[c, j][None])[0]
+            if c == 0:
+                u = u - (1.0 - yext[j] ** 2)
+            tot += float(np.sum(u ** 2)) * g.dy[j]
+    return math.sqrt(tot / g.nx2 / g.nz2 / 2.0)
+
+
+def mode_energy(g: Grid, U):
+    """KE(x, z) = int_y(|u|^2 + |w|^2 + |v interpolated to yh|^2, dyh)
+    (PFC/pass2.f90:113-118) for every stored mode; returns [nx/2+1, nz]."""
+    ny = g.ny
+    t = np.abs(U[0]) ** 2 + np.abs(U[2]) ** 2
+    t[0] += np.abs(U[1, 0]) ** 2
+    t[1:ny] += np.abs(0.5 * (U[1, 1:ny] + U[1, 0:ny - 1])) ** 2
+    t[ny] += np.abs(U[1, ny - 1]) ** 2
+    return np.tensordot(g.dyh, t, axes=(0, 0))
+
+
+def plane_moments(g: Grid, U):
+    """Per plane j on the nx x nz grid (small transform): mean, 2nd, 3rd
+    and 4th central moments of u, v, w and the fluxes <u_i' v'>
+    (compute_velocity_moments, PFC/stats.f90:423-440).  Returns [15, ny+1]:
+    rows 0-2 means, 3-5 second, 6-8 third, 9-11 fourth moments, 12-14 fluxes."""
+    out = np.zeros((15, g.ny + 1))
+    scale = 1.0 / float(g.nx * g.nz)
+    for j in range(g.ny + 1):
+        u = small_c2r(g, U[:, j])                       # [3, nz, nx]
+        m = np.array([scale * np.sum(u[c]) for c in range(3)])
+        d = u - m[:, None, None]
+        out[0:3, j] = m
+        for c in range(3):
+            out[3 + c, j] = scale * np.sum(d[c] ** 2)
+            out[6 + c, j] = scale * np.sum(d[c] ** 3)
+            out[9 + c, j] = scale * np.sum(d[c] ** 4)
+            out[12 + c, j] = scale * np.sum(d[c] * d[1])
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Arnoldi matvec (PFC/arnoldi.f90:485-567) and eigenvalue map (:805-806)
 # ---------------------------------------------------------------------------
 def arnoldi_weights(g: Grid):

@@ -501,3 +501,71 @@ def test_orr_sommerfeld_known_answer():
     c_phase = -ci[i] / g.kx[1]
     assert abs(abs(c_phase) - 0.24989154) < 2e-3, (cr, ci)
     assert cr[i] > 0.0  # the mode is (weakly) unstable at Re = 7500
+
+
+# ---------------------------------------------------------------------------
+# diagnostics (l2_norm, per-mode energy, plane moments)
+# ---------------------------------------------------------------------------
+def _diag_grid():
+    return O.make_grid(16, 17, 12, 24, 18, 2.0, 2.0, 1000.0, stretch=1.5, fix_mass=1)
+
+
+def test_mode_energy_parseval():
+    """Sum of the per-mode energies (kx > 0 counted twice) equals the energy
+    integrated in physical space on the nx x nz grid (no Nyquist content)."""
+    g = _diag_grid()
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal((3, g.ny + 1, g.nxh, g.nz)) + 1j * rng.standard_normal((3, g.ny + 1, g.nxh, g.nz))
+    U[:, :, -1, :] = 0.0                       # x-Nyquist
+    U[:, :, :, g.nz // 2] = 0.0                # z-Nyquist
+    U[:, :, 0, :] = 0.5 * (U[:, :, 0, :] + np.conj(U[:, :, 0, (-np.arange(g.nz)) % g.nz]))  # Hermitian kx = 0
+    ke = O.mode_energy(g, U)
+    w = np.full(g.nxh, 2.0)
+    w[0] = 1.0
+    spectral = float(np.sum(w[:, None] * ke))
+    phys = O.small_c2r(g, U.reshape(-1, g.nxh, g.nz)).reshape(3, g.ny + 1, g.nz, g.nx)
+    t = phys[0] ** 2 + phys[2] ** 2
+    vi = np.empty_like(phys[1])
+    vi[0] = phys[1, 0]
+    vi[1:g.ny] = 0.5 * (phys[1, 1:g.ny] + phys[1, 0:g.ny - 1])
+    vi[g.ny] = phys[1, g.ny - 1]
+    t = t + vi ** 2
+    physical = float(np.sum(g.dyh[:, None, None] * t) / (g.nx * g.nz))
+    assert abs(spectral - physical) <= 1e-12 * physical
+
+
+def test_plane_moments_closed_form():
+    """u = a cos x + m, v = b cos x, w = c sin 2x on every plane: variance
+    a^2/2, third moment 0, fourth 3a^4/8, <u'v'> = ab/2."""
+    g = _diag_grid()
+    a, b, c, mval = 0.7, -1.3, 0.4, 0.25
+    U = np.zeros((3, g.ny + 1, g.nxh, g.nz), np.complex128)
+    U[0, :, 0, 0] = mval
+    U[0, :, 1, 0] = a / 2
+    U[1, :, 1, 0] = b / 2
+    U[2, :, 2, 0] = -0.5j * c
+    M = O.plane_moments(g, U)
+    np.testing.assert_allclose(M[0], mval, rtol=1e-14)
+    np.testing.assert_allclose(M[1:3], 0.0, atol=1e-15)
+    np.testing.assert_allclose(M[3], a * a / 2, rtol=1e-13)
+    np.testing.assert_allclose(M[4], b * b / 2, rtol=1e-13)
+    np.testing.assert_allclose(M[5], c * c / 2, rtol=1e-13)
+    np.testing.assert_allclose(M[6:9], 0.0, atol=1e-14)
+    np.testing.assert_allclose(M[9], 3 * a ** 4 / 8, rtol=1e-13)
+    np.testing.assert_allclose(M[10], 3 * b ** 4 / 8, rtol=1e-13)
+    np.testing.assert_allclose(M[12], a * b / 2, rtol=1e-13)
+    np.testing.assert_allclose(M[13], b * b / 2, rtol=1e-13)
+    np.testing.assert_allclose(M[14], 0.0, atol=1e-15)
+
+
+def test_l2_norm_closed_form():
+    """u = 1 - y(j)^2 + a cos x on every plane, v = w = 0:
+    l2 = sqrt(a^2/2 * sum_j dy_j / 2)."""
+    g = _diag_grid()
+    a = 0.3
+    yext = np.concatenate([g.y, g.y[-1:]])
+    U = np.zeros((3, g.ny + 1, g.nxh, g.nz), np.complex128)
+    U[0, :, 0, 0] = 1.0 - yext ** 2
+    U[0, :, 1, 0] = a / 2
+    l2 = O.l2_norm(g, U)
+    assert abs(l2 - math.sqrt(a * a / 2 * float(np.sum(g.dy)) / 2)) <= 1e-14
