@@ -1,6 +1,7 @@
 """Device diagnostics through the C-ABI against the oracle: l2_norm and the
 wall drag, per-mode kinetic energy and per-plane velocity moments.
-Tolerance: rel 1e-12 (reductions in a different order)."""
+Tolerance: rel 1e-12 for l2 and the energies, rel 1e-10 with an absolute
+floor of 1e-12 of the row scale for the moments (reductions in another order)."""
 import numpy as np
 import pytest
 
@@ -28,6 +29,9 @@ def test_diagnostics_match_oracle():
     assert rel_l2(ch.mode_energy(), O.mode_energy(g, U)) <= 1e-12
     M = ch.plane_moments()
     R = O.plane_moments(g, U)
-    for k in range(15):
-        assert rel_l2(M[k], R[k]) <= 1e-11, k
+    # rows of one kind (means, 2nd, 3rd, 4th moments, fluxes) share a scale;
+    # the mean of v is zero up to round-off, so an absolute floor is used
+    for k0 in range(0, 15, 3):
+        scale = np.abs(R[k0:k0 + 3]).max()
+        np.testing.assert_allclose(M[k0:k0 + 3], R[k0:k0 + 3], rtol=1e-10, atol=1e-12 * scale)
     ch.close()
