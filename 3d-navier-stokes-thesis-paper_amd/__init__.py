@@ -50,6 +50,8 @@ ENTRY_POINTS = (
     "chan_get_kernel_times", "chan_sync", "chan_io_last_error", "chan_savevel_write",
     "chan_savevel_read_header", "chan_savevel_read", "chan_save_velocity", "chan_read_velocity",
     "chan_set_coordinates", "chan_l2_norm", "chan_mode_energy", "chan_plane_moments",
+    "chan_set_tuning", "chan_get_tuning", "chan_autotune", "chan_ensemble_create", "chan_ensemble_step",
+    "chan_ensemble_invalidate", "chan_ensemble_destroy", "chan_ensemble_last_error",
 )
 
 _lib = None
@@ -113,6 +115,14 @@ def load_library(path: str = LIBPATH):
         "chan_l2_norm": (C.c_int, [vp, _dp, _dp]),
         "chan_mode_energy": (C.c_int, [vp, _dp]),
         "chan_plane_moments": (C.c_int, [vp, _dp]),
+        "chan_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+        "chan_get_tuning": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+        "chan_autotune": (C.c_int, [vp, C.c_int, _dp]),
+        "chan_ensemble_create": (C.c_int, [C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]),
+        "chan_ensemble_step": (C.c_int, [vp, C.c_int]),
+        "chan_ensemble_invalidate": (C.c_int, [vp]),
+        "chan_ensemble_destroy": (C.c_int, [vp]),
+        "chan_ensemble_last_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -213,6 +223,28 @@ class Channel:
         d4 = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
         self._chk(self.lib.chan_get_state(self.h, _ptr(d1), _ptr(d2), _ptr(d4)))
         return d1, d2, d4
+
+    # -- launch-shape tuning
+    TUNING_KEYS = ("xpass_wave", "xpass_threads", "zfwd_v2", "solve_lanes", "solve_bl")
+
+    def set_tuning(self, key, value):
+        k = self.TUNING_KEYS.index(key) if isinstance(key, str) else int(key)
+        self._chk(self.lib.chan_set_tuning(self.h, k, int(value)))
+
+    def get_tuning(self):
+        out = {}
+        for k, name in enumerate(self.TUNING_KEYS):
+            v = C.c_int()
+            self._chk(self.lib.chan_get_tuning(self.h, k, C.byref(v)))
+            out[name] = v.value
+        return out
+
+    def autotune(self, steps=2):
+        """Times every candidate launch shape over `steps` timesteps, keeps the
+        fastest (state restored); returns ms per step of the chosen shape."""
+        ms = C.c_double()
+        self._chk(self.lib.chan_autotune(self.h, steps, C.byref(ms)))
+        return ms.value
 
     # -- diagnostics
     def set_coordinates(self, y):
@@ -384,6 +416,38 @@ class Channel:
 
     def sync(self):
         self._chk(self.lib.chan_sync(self.h))
+
+
+class Ensemble:
+    """Independent Channel instances advanced together by one captured graph
+    per timestep (members run concurrently on `nstreams` streams)."""
+
+    def __init__(self, members, nstreams=8, linearized=False):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        if self.lib.chan_ensemble_create(len(self.members), arr, nstreams, int(linearized), C.byref(h)) != 0:
+            raise ChanError(self.lib.chan_ensemble_last_error(None).decode())
+        self.h = h
+
+    def step(self, nsteps=1):
+        if self.lib.chan_ensemble_step(self.h, nsteps) != 0:
+            raise ChanError(self.lib.chan_ensemble_last_error(self.h).decode())
+
+    def invalidate(self):
+        self.lib.chan_ensemble_invalidate(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.chan_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def savevel_write(path, data1, lx, lz, re, dt, t_total):

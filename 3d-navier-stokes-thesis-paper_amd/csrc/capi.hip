@@ -174,12 +174,13 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
     a.cflbits = cfl ? c->d_cfl : nullptr;
     a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->xpass_chunk; a.rz = 0;
+    a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
   }
   {
     KTimer t(c, KT_ZFWD);
     launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * nyp, ny,
-                c->stream);
+                c->stream, c->tune.zfwd_v2);
   }
   if (cfl) launch_cfl_finalize(c->ops, c->prm, c->d_cfl, c->d_sc, step, c->stream);
 }
@@ -191,6 +192,7 @@ void solve_pass(chan_ctx* c, int rk) {
   a.sc = c->d_sc; a.scr = c->d_scr; a.rk = rk; a.ny = c->prm.ny; a.nxh = c->nxh; a.nz = c->prm.nz;
   a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
   a.fix_mass = c->prm.fix_mass;
+  a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl;
   launch_solve(c->ops, a, c->stream);
 }
 
@@ -300,9 +302,68 @@ void roots_for(std::vector<double2>& roots, std::map<int, int>& off, int n) {
   }
 }
 
+// One iteration of the timestep loop (PFC/main_mpi.f90:73-90): boundary,
+// three (nonlinear, rhs_solve) substeps, t_total += dt.  Kernels only, no
+// host synchronisation, so it can be captured into a graph.
+void step_body(chan_ctx* c, int step, bool lin) {
+  boundary_dev(c);
+  for (int rk = 1; rk <= 3; ++rk) {
+    nl_pass(c, step, rk, lin, rk == 1);
+    solve_pass(c, rk);
+  }
+  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, c->stream, c->d_sc);
+  LAUNCH_CHECK("k_advance_time");
+}
+
+// Everything step_body may allocate or synchronise for, done ahead of a capture.
+void prepare_capture(chan_ctx* c) {
+  check_ops(c);
+  ensure_vel_in_vel(c);
+  if (c->prm.ctrlu != 0.0 || c->prm.ctrlw != 0.0) (void)stage(c, sizeof(double) * 6 * (size_t)c->gd.nz2 * c->gd.nx2);
+  if (c->timing) throw chan_error("disable timing before batching or tuning");
+  CHAN_CHECK(hipStreamSynchronize(c->stream));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl.
+Tuning tuning_defaults() {
+  Tuning t;
+  auto env = [](const char* n, int& v) {
+    if (const char* e = getenv(n)) v = atoi(e);
+  };
+  int v;
+  v = t.xpass_wave; env("CHANFLOW_XPASS_WAVE", v); tuning_set(t, 0, v);
+  v = t.xpass_threads; env("CHANFLOW_XPASS_THREADS", v); tuning_set(t, 1, v);
+  v = t.zfwd_v2; env("CHANFLOW_ZFWD_V2", v); tuning_set(t, 2, v);
+  v = t.solve_lanes; env("CHANFLOW_SOLVE_LANES", v); tuning_set(t, 3, v);
+  v = t.solve_bl; env("CHANFLOW_SOLVE_BL", v); tuning_set(t, 4, v);
+  return t;
+}
+
+bool tuning_set(Tuning& t, int key, int v) {
+  switch (key) {
+    case 0: if (v != 0 && v != 1) return false; t.xpass_wave = v; return true;
+    case 1: if (v < 64 || v > 640) return false; t.xpass_threads = v; return true;
+    case 2: if (v != 0 && v != 1) return false; t.zfwd_v2 = v; return true;
+    case 3: if (v != 1 && v != 3) return false; t.solve_lanes = v; return true;
+    case 4: if (v != 4 && v != 8 && v != 16) return false; t.solve_bl = v; return true;
+    default: return false;
+  }
+}
+
+int tuning_get(const Tuning& t, int key) {
+  switch (key) {
+    case 0: return t.xpass_wave;
+    case 1: return t.xpass_threads;
+    case 2: return t.zfwd_v2;
+    case 3: return t.solve_lanes;
+    case 4: return t.solve_bl;
+    default: return -1;
+  }
+}
+
 FFTPlan make_plan(int n, int tw_offset) {
   FFTPlan p{};
   p.n = n;
@@ -326,6 +387,62 @@ FFTPlan make_plan(int n, int tw_offset) {
   return p;
 }
 
+struct chan_ensemble {
+  std::vector<chan_ctx*> m;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  hipStream_t master = nullptr;
+  hipEvent_t start = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int lin = 0;
+  std::string last_error;
+};
+
+namespace {
+
+void ensemble_release(chan_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+// Capture one timestep of every member: the members are spread round-robin
+// over the ensemble's streams, which fork from and join into the master
+// stream, so independent members run concurrently inside one graph.
+void ensemble_capture(chan_ensemble* e) {
+  for (chan_ctx* c : e->m) prepare_capture(c);
+  CHAN_CHECK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+  std::vector<hipStream_t> saved(e->m.size());
+  try {
+    CHAN_CHECK(hipEventRecord(e->start, e->master));
+    for (hipStream_t st : e->streams) CHAN_CHECK(hipStreamWaitEvent(st, e->start, 0));
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      chan_ctx* c = e->m[i];
+      saved[i] = c->stream;
+      c->stream = e->streams[i % e->streams.size()];
+      step_body(c, 1, e->lin != 0);
+      c->stream = saved[i];
+    }
+    for (size_t k = 0; k < e->streams.size(); ++k) {
+      CHAN_CHECK(hipEventRecord(e->done[k], e->streams[k]));
+      CHAN_CHECK(hipStreamWaitEvent(e->master, e->done[k], 0));
+    }
+  } catch (...) {
+    for (size_t i = 0; i < e->m.size(); ++i)
+      if (saved[i]) e->m[i]->stream = saved[i];
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(e->master, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  CHAN_CHECK(hipStreamEndCapture(e->master, &e->graph));
+  CHAN_CHECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* chan_version(void) { return "chanflow 1 gfx950 fp64"; }
@@ -348,6 +465,7 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
   if (p.re <= 0.0 || p.lx <= 0.0 || p.lz <= 0.0) return fail(nullptr, "invalid re/lx/lz");
   chan_ctx* c = new chan_ctx();
   c->prm = p;
+  c->tune = tuning_defaults();
   try {
     c->nxh = p.nx / 2 + 1;
     c->nyp = p.ny + 1;
@@ -662,15 +780,7 @@ int chan_step(chan_ctx* c, int step0, int nsteps, int lin) {
   return guarded(c, [&] {
     check_ops(c);
     ensure_vel_in_vel(c);
-    for (int s = 0; s < nsteps; ++s) {
-      boundary_dev(c);
-      for (int rk = 1; rk <= 3; ++rk) {
-        nl_pass(c, step0 + s, rk, lin != 0, rk == 1);
-        solve_pass(c, rk);
-      }
-      hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, c->stream, c->d_sc);
-      LAUNCH_CHECK("k_advance_time");
-    }
+    for (int s = 0; s < nsteps; ++s) step_body(c, step0 + s, lin != 0);
     c->data1_is_nl = false;
     CHAN_CHECK(hipStreamSynchronize(c->stream));
     check_err(c);
@@ -1013,6 +1123,171 @@ int chan_sync(chan_ctx* c) {
     CHAN_CHECK(hipStreamSynchronize(c->stream));
     check_err(c);
   });
+}
+
+
+// ---------------------------------------------------------------------------
+// Launch-shape tuning and batched ensembles
+// ---------------------------------------------------------------------------
+int chan_set_tuning(chan_ctx* c, int key, int value) {
+  return guarded(c, [&] {
+    if (!tuning_set(c->tune, key, value)) throw chan_error("invalid tuning key or value");
+  });
+}
+
+int chan_get_tuning(chan_ctx* c, int key, int* value) {
+  return guarded(c, [&] {
+    const int v = tuning_get(c->tune, key);
+    if (v < 0 || !value) throw chan_error("invalid tuning key");
+    *value = v;
+  });
+}
+
+// Times `steps` timesteps for each candidate launch shape and keeps the
+// fastest; the state (velocity, RK register, pressure, scalars, d_um_dy) is
+// restored after every trial, so the call leaves the solution unchanged.
+int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
+  return guarded(c, [&] {
+    if (steps < 1) throw chan_error("steps must be >= 1");
+    prepare_capture(c);
+    const size_t vb = vel_bytes(c), pb = p_bytes(c), db = sizeof(double) * c->prm.ny;
+    char* bk = (char*)dalloc(2 * vb + pb + sizeof(DevScalars) + db + 256);
+    auto save = [&](bool back) {
+      auto cp = [&](void* a, void* b, size_t n) { d2d(c, back ? b : a, back ? a : b, n); };
+      cp(bk, c->d_vel, vb);
+      cp(bk + vb, c->d_R, vb);
+      cp(bk + 2 * vb, c->d_P, pb);
+      cp(bk + 2 * vb + pb, c->d_sc, sizeof(DevScalars));
+      cp(bk + 2 * vb + pb + sizeof(DevScalars), c->d_dudy, db);
+      CHAN_CHECK(hipStreamSynchronize(c->stream));
+    };
+    std::vector<Tuning> cand;
+    for (int w : {1, 0})
+      for (int z : {1, 0})
+        for (int sl : {3, 1})
+          for (int bl : {8, 16}) {
+            Tuning t = c->tune;
+            t.xpass_wave = w; t.zfwd_v2 = z; t.solve_lanes = sl; t.solve_bl = bl;
+            cand.push_back(t);
+          }
+    hipEvent_t a, b;
+    CHAN_CHECK(hipEventCreate(&a));
+    CHAN_CHECK(hipEventCreate(&b));
+    double best = 1e300;
+    Tuning bestt = c->tune;
+    try {
+      save(false);
+      for (const Tuning& t : cand) {
+        c->tune = t;
+        step_body(c, 1, false);  // first launch of each variant, untimed
+        CHAN_CHECK(hipEventRecord(a, c->stream));
+        for (int s = 0; s < steps; ++s) step_body(c, 1, false);
+        CHAN_CHECK(hipEventRecord(b, c->stream));
+        CHAN_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        CHAN_CHECK(hipEventElapsedTime(&ms, a, b));
+        save(true);
+        check_err(c);
+        if (ms / steps < best) { best = ms / steps; bestt = t; }
+      }
+    } catch (...) {
+      c->tune = bestt;
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      dfree(bk);
+      throw;
+    }
+    c->tune = bestt;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    dfree(bk);
+    if (best_ms) *best_ms = best;
+  });
+}
+
+int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int linearized, chan_ensemble** out) {
+  if (!out || !members || n < 1) return fail(nullptr, "invalid ensemble arguments");
+  *out = nullptr;
+  chan_ensemble* e = new chan_ensemble();
+  try {
+    for (int i = 0; i < n; ++i) {
+      if (!members[i]) throw chan_error("null ensemble member");
+      for (int k = 0; k < i; ++k)
+        if (members[k] == members[i]) throw chan_error("ensemble member given twice");
+      e->m.push_back(members[i]);
+    }
+    const int ns = std::max(1, std::min(nstreams, n));
+    CHAN_CHECK(hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
+    CHAN_CHECK(hipEventCreateWithFlags(&e->start, hipEventDisableTiming));
+    e->streams.resize(ns, nullptr);
+    e->done.resize(ns, nullptr);
+    for (int k = 0; k < ns; ++k) {
+      CHAN_CHECK(hipStreamCreateWithFlags(&e->streams[k], hipStreamNonBlocking));
+      CHAN_CHECK(hipEventCreateWithFlags(&e->done[k], hipEventDisableTiming));
+    }
+    e->lin = linearized;
+  } catch (const chan_error& err) {
+    g_create_error = err.msg;
+    chan_ensemble_destroy(e);
+    return 1;
+  }
+  *out = e;
+  return 0;
+}
+
+const char* chan_ensemble_last_error(const chan_ensemble* e) {
+  return e ? e->last_error.c_str() : g_create_error.c_str();
+}
+
+int chan_ensemble_step(chan_ensemble* e, int nsteps) {
+  if (!e) return fail(nullptr, "null ensemble");
+  try {
+    if (nsteps < 0) throw chan_error("nsteps must be >= 0");
+    for (chan_ctx* c : e->m) {
+      check_ops(c);
+      ensure_vel_in_vel(c);
+      CHAN_CHECK(hipStreamSynchronize(c->stream));
+    }
+    if (!e->exec) ensemble_capture(e);
+    for (int s = 0; s < nsteps; ++s) CHAN_CHECK(hipGraphLaunch(e->exec, e->master));
+    CHAN_CHECK(hipStreamSynchronize(e->master));
+    std::string errs;
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      try {
+        check_err(e->m[i]);
+      } catch (const chan_error& err) {
+        errs += "member " + std::to_string(i) + ": " + err.msg + "; ";
+      }
+    }
+    if (!errs.empty()) throw chan_error(errs);
+    return 0;
+  } catch (const chan_error& err) {
+    e->last_error = err.msg;
+  } catch (const std::exception& err) {
+    e->last_error = err.what();
+  }
+  return 1;
+}
+
+// Drops the captured graph, e.g. after a member's tuning or gains changed.
+int chan_ensemble_invalidate(chan_ensemble* e) {
+  if (!e) return 1;
+  ensemble_release(e);
+  return 0;
+}
+
+int chan_ensemble_destroy(chan_ensemble* e) {
+  if (!e) return 0;
+  if (e->master) (void)hipStreamSynchronize(e->master);
+  ensemble_release(e);
+  for (hipEvent_t ev : e->done)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipStream_t st : e->streams)
+    if (st) (void)hipStreamDestroy(st);
+  if (e->start) (void)hipEventDestroy(e->start);
+  if (e->master) (void)hipStreamDestroy(e->master);
+  delete e;
+  return 0;
 }
 
 }  // extern "C"

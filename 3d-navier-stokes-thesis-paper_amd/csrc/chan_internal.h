@@ -73,7 +73,23 @@ struct XPassArgs {
   int ka, kb;             // reference planes computed: NL(k) for ka <= k <= kb
   int chunk;              // planes per workgroup
   int rz;                 // rows per workgroup
+  int wave;               // 1: wave-per-row kernel where the length allows it
+  int maxthr;             // workgroup-kernel threads bound (64..640)
 };
+
+// Launch-shape choices of one context (chan_set_tuning, chan_autotune);
+// the defaults can be overridden with CHANFLOW_XPASS_WAVE, CHANFLOW_XPASS_THREADS,
+// CHANFLOW_ZFWD_V2, CHANFLOW_SOLVE_LANES and CHANFLOW_SOLVE_BL.
+struct Tuning {
+  int xpass_wave = 1;     // wave-per-row x-pass (lengths <= 192)
+  int xpass_threads = 640;
+  int zfwd_v2 = 1;        // sequential-A/B z-forward kernel
+  int solve_lanes = 3;    // 3: component-parallel solve, 1: one lane per mode
+  int solve_bl = 8;       // rows per prefetch block (4, 8, 16)
+};
+Tuning tuning_defaults();
+bool tuning_set(Tuning& t, int key, int value);
+int tuning_get(const Tuning& t, int key);
 
 struct chan_ctx {
   chan_params prm;
@@ -114,6 +130,7 @@ struct chan_ctx {
   bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
   // options
   int xpass_chunk = 8;
+  Tuning tune;
   int timing = 0;
   std::vector<hipEvent_t> ev;    // timing events (pairs)
   std::vector<int> ev_kind;
@@ -137,7 +154,7 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
                  const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s);
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s);
+                 hipStream_t s, int v2 = 1);
 // (k_nonlinear.hip)
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
                   hipStream_t s);
@@ -155,6 +172,7 @@ struct SolveArgs {
   int ny, nxh, nz;
   double re, cosb, sinb;
   int fix_mass;
+  int lanes, bl;      // kernel variant (Tuning::solve_lanes, solve_bl)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,

@@ -533,13 +533,8 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
   XPassArgs a = a0;
   const int M = g.nx2 / 2;
   if (a.kb < a.ka) return;
-  // rows per workgroup: largest divisor of nz2 with rz*M <= maxthr (default
-  // 640; CHANFLOW_XPASS_THREADS overrides it for tuning)
-  static const int maxthr = [] {
-    const char* e = getenv("CHANFLOW_XPASS_THREADS");
-    int v = e ? atoi(e) : 640;
-    return (v >= 64 && v <= 640) ? v : 640;
-  }();
+  // rows per workgroup: largest divisor of nz2 with rz*M <= maxthr
+  const int maxthr = (a.maxthr >= 64 && a.maxthr <= 640) ? a.maxthr : 640;
   int rz = 1;
   for (int d = 1; d <= g.nz2; ++d)
     if (g.nz2 % d == 0 && d * M <= std::max(maxthr, M)) rz = d;
@@ -556,11 +551,7 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     }
   int nplanes = a.kb - a.ka + 1;
   int nch = (nplanes + a.chunk - 1) / a.chunk;
-  static const bool use_wave = [] {
-    const char* e = getenv("CHANFLOW_XPASS_WAVE");
-    return !(e && atoi(e) == 0);
-  }();
-  if (use_wave && ct && M <= 192) {
+  if (a.wave && ct && M <= 192) {
     dim3 gw((g.nz2 + 3) / 4, nch);
     size_t ldw = sizeof(double2) * (M + 4 * 4 * fft_seq_cells(M));
 #define XPASSW_CASE(N)                                                                            \

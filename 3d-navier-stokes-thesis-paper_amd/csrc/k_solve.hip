@@ -703,18 +703,8 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));
-  // rows per prefetch block (CHANFLOW_SOLVE_BL = 4, 8 or 16 overrides it for tuning)
-  static const int bl = [] {
-    const char* e = getenv("CHANFLOW_SOLVE_BL");
-    int v = e ? atoi(e) : 8;
-    return (v == 4 || v == 16) ? v : 8;
-  }();
-  // component-parallel kernel unless CHANFLOW_SOLVE_LANES=1 or the byte
-  // offsets of one array would not fit 32 bits
-  static const int lanes = [] {
-    const char* e = getenv("CHANFLOW_SOLVE_LANES");
-    return (e && atoi(e) == 1) ? 1 : 3;
-  }();
+  const int bl = a.bl, lanes = a.lanes;
+  // the component-parallel kernel needs the byte offsets of one array to fit 32 bits
   if (lanes == 3 && 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9) {
     dim3 g3((unsigned)((nm + 19) / 20));
     if (bl == 4) hipLaunchKernelGGL(k_solve3<4>, g3, dim3(64), 0, s, ops, a);

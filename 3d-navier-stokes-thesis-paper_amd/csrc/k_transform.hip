@@ -222,6 +222,100 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
   }
 }
 
+// Same result as k_zfwd with half the LDS: A and B are transformed one after
+// the other in one set of 8 sequences.  B's loads are issued into registers
+// before A's transform so that their latency overlaps it; A's folded and
+// scaled values wait in registers for B's.  Used for the compile-time
+// lengths with a second field (the nonlinear pass).
+template <int NC>
+__global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restrict__ tw,
+                                               const double2* __restrict__ A,
+                                               const double2* __restrict__ B, long ab_ps,
+                                               const double* __restrict__ kzv,
+                                               double2* __restrict__ out, long out_ps,
+                                               int nl_zero_ny) {
+  extern __shared__ double2 lds[];
+  constexpr int NZ2 = NC, SS = fft_seq_cells(NC);
+  constexpr int NLD = (NZ2 * ZT_KX + 255) / 256;      // loads per thread per field
+  const int tid = threadIdx.x;
+  const int kx0 = blockIdx.x * ZT_KX;
+  const long P = blockIdx.y;
+  const int nz = g.nz;
+  const int ncol = min(ZT_KX, g.nxh - kx0);
+  double2* dst = out + P * out_ps;
+  if (nl_zero_ny > 0) {
+    // NL planes that the reference leaves zero: plane 1, plane ny+1, and v at
+    // plane ny (PFC/pass1.f90:62,96-99).
+    const int ny = nl_zero_ny;
+    const int c = (int)(P / (ny + 1)), j = (int)(P % (ny + 1));
+    if (j == 0 || j == ny || (c == 1 && j == ny - 1)) {
+      for (int e = tid; e < ncol * nz; e += 256) {
+        const int s = e / nz, q = e - s * nz;
+        dst[(long)(kx0 + s) * nz + q] = make_double2(0.0, 0.0);
+      }
+      return;
+    }
+  }
+  double2* rl = lds + ZT_KX * SS;
+  load_roots(rl, tw + g.pz.tw, NZ2, tid, 256);
+  const double2* Ap = A + P * ab_ps;
+  const double2* Bp = B + P * ab_ps;
+  double2 rb[NLD];
+#pragma unroll
+  for (int k = 0; k < NLD; ++k) {
+    const int e = tid + 256 * k;
+    const int t = e / ZT_KX, s = e - t * ZT_KX;
+    double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+    if (e < NZ2 * ZT_KX && s < ncol) {
+      va = Ap[(long)t * g.kxpad + kx0 + s];
+      vb = Bp[(long)t * g.kxpad + kx0 + s];
+    }
+    if (e < NZ2 * ZT_KX) lds[s * SS + fft2_in_pos<NC>(t)] = va;
+    rb[k] = vb;
+  }
+  __syncthreads();
+  fft2p<NC, -1>(lds, SS, ZT_KX, rl, tw + g.pz.tw, tid, 256);
+  const int h = nz / 2, off = NZ2 - nz;
+  const double scale = 1.0 / (double)(g.nx2 * NZ2);
+  auto fold = [&](const double2* a, int q) {
+    if (q < h) return a[fft2_out_pos<NC>(q)];
+    if (q == h) return cadd(a[fft2_out_pos<NC>(h)], cconj(a[fft2_out_pos<NC>(off + h)]));
+    return a[fft2_out_pos<NC>(q + off)];
+  };
+  constexpr int NOUT = (ZT_KX * 192 + 255) / 256;      // outputs per thread for nz <= 192
+  double2 acc[NOUT];
+#pragma unroll
+  for (int k = 0; k < NOUT; ++k) {
+    const int e = tid + 256 * k;
+    acc[k] = make_double2(0.0, 0.0);
+    if (e < ncol * nz) {
+      const int s = e / nz, q = e - s * nz;
+      const double f = (kx0 + s == g.nxh - 1) ? 2.0 * scale : scale;
+      acc[k] = cscale(f, fold(lds + s * SS, q));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NLD; ++k) {
+    const int e = tid + 256 * k;
+    const int t = e / ZT_KX, s = e - t * ZT_KX;
+    if (e < NZ2 * ZT_KX) lds[s * SS + fft2_in_pos<NC>(t)] = rb[k];
+  }
+  __syncthreads();
+  fft2p<NC, -1>(lds, SS, ZT_KX, rl, tw + g.pz.tw, tid, 256);
+#pragma unroll
+  for (int k = 0; k < NOUT; ++k) {
+    const int e = tid + 256 * k;
+    if (e < ncol * nz) {
+      const int s = e / nz, q = e - s * nz;
+      const double f = (kx0 + s == g.nxh - 1) ? 2.0 * scale : scale;
+      const double2 fb = cscale(f, fold(lds + s * SS, q));
+      const double kz = kzv[q];
+      dst[(long)(kx0 + s) * nz + q] = make_double2(acc[k].x - kz * fb.y, acc[k].y + kz * fb.x);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -282,11 +376,21 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
 
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s) {
+                 hipStream_t s, int v2) {
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nf = B ? 2 : 1;
   const int nthr = 256 * nf;
   const bool ct = fft2_split(g.nz2).n1 > 0;
+  if (v2 && ct && B && !g.small && g.nz <= 192) {
+    size_t l2 = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + g.nz2);
+#define ZFWD2_CASE(N)                                                                                \
+  if constexpr (N > 0) hipLaunchKernelGGL(k_zfwd2<N>, grid, dim3(256), l2, s, g, tw, A, B, ab_ps, kz, out, \
+                                          out_ps, nl_zero_ny)
+    FFT_DISPATCH(g.nz2, ZFWD2_CASE)
+#undef ZFWD2_CASE
+    LAUNCH_CHECK("k_zfwd2");
+    return;
+  }
   if (!ct) check_stage_fit(g.pz, nf * ZT_KX, nthr);
   size_t lds = sizeof(double2) * (nf * ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
 #define ZFWD_CASE(N)                                                                              \

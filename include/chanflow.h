@@ -195,6 +195,32 @@ int chan_l2_norm(chan_ctx* ctx, double* l2, double* drag);
 int chan_mode_energy(chan_ctx* ctx, double* ke);
 int chan_plane_moments(chan_ctx* ctx, double* out);
 
+/* Launch-shape tuning of one context (no counterpart in the reference: the
+ * reference's only tuning is FFTW_PATIENT planning, PFC/transform.f90:60-160).
+ * Keys: 0 x-pass kernel (1: one wavefront per z-row, 0: workgroup per rows),
+ * 1 x-pass workgroup thread bound (64..640), 2 z-forward kernel (1: A and B
+ * transformed in turn, 0: together), 3 solve lanes per mode (3 or 1),
+ * 4 solve rows per prefetch block (4, 8 or 16).  chan_autotune times `steps`
+ * timesteps of every candidate, keeps the fastest and restores the state. */
+int chan_set_tuning(chan_ctx* ctx, int key, int value);
+int chan_get_tuning(chan_ctx* ctx, int key, int* value);
+int chan_autotune(chan_ctx* ctx, int steps, double* best_ms_per_step);
+
+/* Batched ensemble (no counterpart in the reference): independent channel
+ * contexts of any sizes and parameters, each set up as usual (operators,
+ * state, chan_initialize_flow), advanced together.  One timestep of every
+ * member is captured once into a HIP graph whose members run concurrently on
+ * nstreams streams; chan_ensemble_step replays it nsteps times and then
+ * reports any member's CFL abort.  Changing a member's tuning or gains
+ * needs chan_ensemble_invalidate (the next step captures again).  Members
+ * must not be used on their own while an ensemble step runs. */
+typedef struct chan_ensemble chan_ensemble;
+int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int linearized, chan_ensemble** out);
+int chan_ensemble_step(chan_ensemble* ens, int nsteps);
+int chan_ensemble_invalidate(chan_ensemble* ens);
+int chan_ensemble_destroy(chan_ensemble* ens);
+const char* chan_ensemble_last_error(const chan_ensemble* ens);
+
 /* Velocity checkpoints in the reference's savevel.out / newvel.out format:
  * read_velocity and SAVE_VELOCITY (PFC/read_write.f90:21-176, 180-292),
  * Fortran sequential unformatted records with 4-byte length markers: one

@@ -1,0 +1,84 @@
+"""Batched ensemble and launch-shape tuning through the C-ABI.
+
+* Every replay of the captured timestep graph is checked against twins of
+  the members stepped one by one with chan_step, with host synchronisation
+  between the replays (the graph holds only kernels; bitwise equality is
+  expected, 1e-14 is allowed).
+* One member is also checked against the oracle after the replays.
+* chan_autotune leaves the state unchanged and its choice gives the same
+  step to round-off.
+"""
+import numpy as np
+import pytest
+
+from helpers import make_channel, rel_l2, synth
+from oracle import chan_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # nx, ny, nz, nx2, nz2, Lx/pi, Lx/Lz, Re
+    (32, 49, 32, 48, 48, 2.0, 2.0, 2100.0),
+    (48, 65, 32, 72, 48, 2.0, 2.0, 2600.0),
+    (64, 49, 48, 96, 72, 2.0, 2.0, 3100.0),
+    (32, 97, 64, 48, 96, 2.0, 2.0, 3400.0),
+    (96, 49, 32, 144, 48, 2.0, 2.0, 2900.0),
+]
+
+
+def _member(shape, seed):
+    nx, ny, nz, nx2, nz2, lxp, asp, re = shape
+    g = O.make_grid(nx, ny, nz, nx2, nz2, lxp, asp, re, stretch=1.5, fix_mass=1)
+    U0 = synth().laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=seed, amp=1e-2, kmax=4.0)
+    ch = make_channel(g)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.005, t_total=0.0, p_grad=-2.0 / re)
+    ch.initialize_flow()
+    return g, U0, ch
+
+
+def test_ensemble_replays_match_eager():
+    cf = __import__("helpers").chanflow()
+    mem = [_member(s, 20 + i) for i, s in enumerate(SHAPES)]
+    twins = [_member(s, 20 + i) for i, s in enumerate(SHAPES)]
+    ens = cf.Ensemble([m[2] for m in mem], nstreams=3)
+    for step in range(1, 4):
+        ens.step(1)
+        for (g, _, a), (_, _, b) in zip(mem, twins):
+            b.step(1, step0=step)
+            da, db = a.get_state(), b.get_state()
+            for x, y in zip(da, db):
+                assert rel_l2(x, y) <= 1e-14
+            assert a.get_scalars()["t_total"] == b.get_scalars()["t_total"]
+    # member 1 against the oracle after the three replays
+    g, U0, ch = mem[1]
+    st = O.initial_state(g, U0, dt=0.005)
+    for s in (1, 2, 3):
+        O.rk3_step(g, st, s)
+    d1 = ch.get_data1()
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10
+    assert ch.divergence_max() <= 1e-12
+    ens.step(2)  # more replays after the checks above
+    ens.close()
+    for _, _, ch in mem + twins:
+        ch.close()
+
+
+def test_autotune_restores_state_and_keeps_results():
+    g, U0, a = _member(SHAPES[2], 31)
+    _, _, b = _member(SHAPES[2], 31)
+    before = a.get_state()
+    ms = a.autotune(steps=1)
+    assert ms > 0.0
+    after = a.get_state()
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    t = a.get_tuning()
+    assert t["solve_lanes"] in (1, 3) and t["solve_bl"] in (4, 8, 16)
+    a.step(2, step0=1)
+    b.step(2, step0=1)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert rel_l2(x, y) <= 1e-12
+    with pytest.raises(Exception):
+        a.set_tuning("solve_bl", 5)
+    a.close(); b.close()
