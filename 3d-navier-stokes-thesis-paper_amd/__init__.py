@@ -52,9 +52,12 @@ ENTRY_POINTS = (
     "chan_set_coordinates", "chan_l2_norm", "chan_mode_energy", "chan_plane_moments",
     "chan_set_tuning", "chan_get_tuning", "chan_autotune", "chan_ensemble_create", "chan_ensemble_step",
     "chan_ensemble_invalidate", "chan_ensemble_destroy", "chan_ensemble_last_error",
+    "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_num_params", "chan_mlp_set_params",
+    "chan_mlp_get_params", "chan_mlp_infer", "chan_mlp_forward_fp32", "chan_mlp_gradient", "chan_mlp_train",
 )
 
 _lib = None
+_fp = C.POINTER(C.c_float)
 
 
 def load_library(path: str = LIBPATH):
@@ -123,6 +126,16 @@ def load_library(path: str = LIBPATH):
         "chan_ensemble_invalidate": (C.c_int, [vp]),
         "chan_ensemble_destroy": (C.c_int, [vp]),
         "chan_ensemble_last_error": (C.c_char_p, [vp]),
+        "chan_mlp_create": (C.c_int, [C.POINTER(vp)]),
+        "chan_mlp_destroy": (C.c_int, [vp]),
+        "chan_mlp_last_error": (C.c_char_p, [vp]),
+        "chan_mlp_num_params": (C.c_int, []),
+        "chan_mlp_set_params": (C.c_int, [vp, _fp]),
+        "chan_mlp_get_params": (C.c_int, [vp, _fp]),
+        "chan_mlp_infer": (C.c_int, [vp, C.c_int, _fp, _fp]),
+        "chan_mlp_forward_fp32": (C.c_int, [vp, C.c_int, _fp, _fp]),
+        "chan_mlp_gradient": (C.c_int, [vp, C.c_int, _fp, _fp, _fp, _dp]),
+        "chan_mlp_train": (C.c_int, [vp, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_double, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -441,6 +454,85 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             self.lib.chan_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+class Mlp:
+    """The learned closure network 10 -> 128 -> 128 -> 128 -> 6 (GELU) on the GPU."""
+
+    def __init__(self, params=None):
+        self.lib = load_library()
+        h = C.c_void_p()
+        if self.lib.chan_mlp_create(C.byref(h)) != 0:
+            raise ChanError("chan_mlp_create failed")
+        self.h = h
+        self.nparams = self.lib.chan_mlp_num_params()
+        if params is not None:
+            self.set_params(params)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise ChanError(self.lib.chan_mlp_last_error(self.h).decode())
+
+    @staticmethod
+    def flatten(W, b):
+        return np.concatenate([np.concatenate([np.asarray(w).ravel(), np.asarray(bb).ravel()])
+                               for w, bb in zip(W, b)]).astype(np.float32)
+
+    def set_params(self, p):
+        p = _f32(p)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters")
+        self._chk(self.lib.chan_mlp_set_params(self.h, _fptr(p)))
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        self._chk(self.lib.chan_mlp_get_params(self.h, _fptr(p)))
+        return p
+
+    def infer(self, x):
+        x = _f32(x)
+        y = np.empty((x.shape[0], 6), np.float32)
+        self._chk(self.lib.chan_mlp_infer(self.h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def forward(self, x):
+        x = _f32(x)
+        y = np.empty((x.shape[0], 6), np.float32)
+        self._chk(self.lib.chan_mlp_forward_fp32(self.h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x, t = _f32(x), _f32(t)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_double()
+        self._chk(self.lib.chan_mlp_gradient(self.h, x.shape[0], _fptr(x), _fptr(t), _fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs=1, batch=256, lr=1e-3):
+        x, t = _f32(x), _f32(t)
+        loss = C.c_double()
+        self._chk(self.lib.chan_mlp_train(self.h, x.shape[0], _fptr(x), _fptr(t), epochs, batch, lr,
+                                          C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.chan_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -221,6 +221,32 @@ int chan_ensemble_invalidate(chan_ensemble* ens);
 int chan_ensemble_destroy(chan_ensemble* ens);
 const char* chan_ensemble_last_error(const chan_ensemble* ens);
 
+/* Learned subgrid closure (no counterpart in the reference, whose subgrid
+ * model is absent; only its flags remain, PFC/flow.dat:19-22,
+ * PFC/main_mpi.f90:10-13,74,77-78): an MLP 10 -> 128 -> 128 -> 128 -> 6 with
+ * GELU hidden layers mapping (du_i/dx_j, y+) to (tau_11, tau_12, tau_13,
+ * tau_22, tau_23, tau_33).  Parameters are float32, chan_mlp_num_params() of
+ * them in the order W1 (128x10), b1, W2 (128x128), b2, W3, b3, W4 (6x128), b4,
+ * weights row-major (out, in).  x is float32 (n, 10), y and t (n, 6), row-major.
+ * chan_mlp_infer: bfloat16 matrix-core inference with float32 accumulation.
+ * chan_mlp_forward_fp32 / chan_mlp_gradient: float32 forward pass and the
+ *   gradient of the mean squared error against t (and the loss).
+ * chan_mlp_train: mini-batch Adam (lr, beta 0.9/0.999, eps 1e-8), batches
+ *   taken in order, the incomplete last batch dropped; loss of the last batch.
+ * chan_mlp_set_params resets the Adam moments. */
+typedef struct chan_mlp chan_mlp;
+int chan_mlp_create(chan_mlp** out);
+int chan_mlp_destroy(chan_mlp* net);
+const char* chan_mlp_last_error(const chan_mlp* net);
+int chan_mlp_num_params(void);
+int chan_mlp_set_params(chan_mlp* net, const float* params);
+int chan_mlp_get_params(chan_mlp* net, float* params);
+int chan_mlp_infer(chan_mlp* net, int n, const float* x, float* y);
+int chan_mlp_forward_fp32(chan_mlp* net, int n, const float* x, float* y);
+int chan_mlp_gradient(chan_mlp* net, int n, const float* x, const float* t, float* grad, double* loss);
+int chan_mlp_train(chan_mlp* net, int n, const float* x, const float* t, int epochs, int batch, double lr,
+                   double* loss_out);
+
 /* Velocity checkpoints in the reference's savevel.out / newvel.out format:
  * read_velocity and SAVE_VELOCITY (PFC/read_write.f90:21-176, 180-292),
  * Fortran sequential unformatted records with 4-byte length markers: one
