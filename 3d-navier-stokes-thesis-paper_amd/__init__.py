@@ -54,6 +54,7 @@ ENTRY_POINTS = (
     "chan_ensemble_invalidate", "chan_ensemble_destroy", "chan_ensemble_last_error",
     "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_num_params", "chan_mlp_set_params",
     "chan_mlp_get_params", "chan_mlp_infer", "chan_mlp_forward_fp32", "chan_mlp_gradient", "chan_mlp_train",
+    "chan_vtk_write", "chan_write_vtk",
 )
 
 _lib = None
@@ -136,6 +137,8 @@ def load_library(path: str = LIBPATH):
         "chan_mlp_forward_fp32": (C.c_int, [vp, C.c_int, _fp, _fp]),
         "chan_mlp_gradient": (C.c_int, [vp, C.c_int, _fp, _fp, _fp, _dp]),
         "chan_mlp_train": (C.c_int, [vp, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_double, _dp]),
+        "chan_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
+        "chan_write_vtk": (C.c_int, [vp, C.c_char_p, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -279,6 +282,9 @@ class Channel:
         out = np.empty((15, self.nyp), np.float64)
         self._chk(self.lib.chan_plane_moments(self.h, _ptr(out)))
         return out
+
+    def write_vtk(self, path, yh):
+        self._chk(self.lib.chan_write_vtk(self.h, os.fsencode(path), _ptr(_f64(yh))))
 
     # -- checkpoints (savevel.out / newvel.out)
     def save_velocity(self, path):
@@ -570,6 +576,15 @@ def savevel_read(path):
     if lib.chan_savevel_read(p, nx, ny, nz, _ptr(d1)) != 0:
         raise ChanError(lib.chan_io_last_error().decode())
     return hdr, d1
+
+
+def vtk_write(path, phys, lx, lz, yh):
+    """Format phys [3][ny+1][nz][nx] (u, v, w on the yh planes) as output_hdf_dfsd's VTK file."""
+    lib = load_library()
+    ph = _f64(phys)
+    _, nyp, nz, nx = ph.shape
+    if lib.chan_vtk_write(os.fsencode(path), nx, nyp - 1, nz, lx, lz, _ptr(_f64(yh)), _ptr(ph)) != 0:
+        raise ChanError(lib.chan_io_last_error().decode())
 
 
 def version() -> str:

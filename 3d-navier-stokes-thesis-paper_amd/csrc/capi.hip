@@ -1081,6 +1081,27 @@ int chan_plane_moments(chan_ctx* c, double* out) {
   });
 }
 
+int chan_write_vtk(chan_ctx* c, const char* path, const double* yh) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (!path || !yh) throw chan_error("null argument");
+    ensure_vel_in_vel(c);
+    const XZGeom& g = c->gs;
+    const int ny = c->prm.ny, nyp = c->nyp;
+    const long nm = nmodes(c), ps = (long)g.nx2 * g.nz2;
+    const size_t pb = sizeof(double) * 3 * nyp * ps;
+    if (pb > (size_t)2 * 3 * nyp * c->gd.nz2 * c->gd.kxpad * sizeof(double2))
+      throw chan_error("write_vtk: physical planes do not fit the transform buffer");
+    launch_vtk_field(ny, nm, c->d_vel, c->d_scr, c->stream);
+    double* phys = (double*)c->d_AB;
+    dev_c2r(c, g, c->d_scr, 3 * nyp, phys);
+    std::vector<double> h(3 * nyp * ps);
+    d2h(c, h.data(), phys, pb);
+    if (chan_vtk_write(path, c->prm.nx, ny, c->prm.nz, c->prm.lx, c->prm.lz, yh, h.data()) != 0)
+      throw chan_error(chan_io_last_error());
+  });
+}
+
 int chan_mass_flux(chan_ctx* c, double* out) {
   return guarded(c, [&] {
     check_ops(c);

@@ -180,6 +180,7 @@ void launch_l2_planes(const double* phys, long ps, int nplanes, const double* la
 void launch_l2_final(const double* part, int n, double npts, const double* dudy, int ny, double* out, hipStream_t s);
 void launch_mode_energy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel, double* ke, hipStream_t s);
 void launch_plane_moments(const double* phys, long ps, long cs, int nyp, double* out, hipStream_t s);
+void launch_vtk_field(int ny, long nm, const double2* vel, double2* out, hipStream_t s);
 void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
                     double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
                     double re, hipStream_t s);

@@ -199,3 +199,80 @@ int chan_read_velocity(chan_ctx* c, const char* path) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Legacy VTK output as output_hdf_dfsd writes it with xskip = yskip = zskip = 1
+// (PFC/read_write.f90:296-500): a big-endian stream file with a
+// RECTILINEAR_GRID of nx x (ny+1) x nz points (x = lx (i-1)/nx, y = yh,
+// z = lz (k-1)/nz) and one float32 vector field "flowvec" ordered
+// ((((u_i, i), x), plane), z).  The 30-column header lines are Fortran
+// CHARACTER(30) variables with an I5/I3/I9 field written into them.
+// ---------------------------------------------------------------------------
+namespace {
+
+void put_be32(std::vector<unsigned char>& out, float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  out.push_back((unsigned char)(u >> 24));
+  out.push_back((unsigned char)(u >> 16));
+  out.push_back((unsigned char)(u >> 8));
+  out.push_back((unsigned char)u);
+}
+
+// CHARACTER(LEN=30) initialised to `text` with an integer written into
+// columns a..b (1-based) right-justified, then a line feed.
+std::string line30(const std::string& text, int a, int b, const std::vector<long>& vals, int width) {
+  std::string s(30, ' ');
+  for (size_t i = 0; i < text.size() && i < 30; ++i) s[i] = text[i];
+  std::string f;
+  for (long v : vals) {
+    std::string t = std::to_string(v);
+    if ((int)t.size() > width) t = std::string(width, '*');
+    f += std::string(width - t.size(), ' ') + t;
+  }
+  for (int i = a - 1, k = 0; i < b && k < (int)f.size(); ++i, ++k) s[i] = f[k];
+  return s + "\n";
+}
+
+void write_vtk(const char* path, int nx, int ny, int nz, double lx, double lz, const double* yh, const double* phys) {
+  if (!path || !yh || !phys) throw chan_error("null argument");
+  std::vector<unsigned char> b;
+  auto put = [&](const std::string& s) { b.insert(b.end(), s.begin(), s.end()); };
+  put("# vtk DataFile Version 2.0\n");
+  put("(This is header line) vtk rectilinear grid with binary data\n");
+  put("BINARY\n");
+  put("DATASET RECTILINEAR_GRID\n");
+  put(line30("DIMENSIONS", 12, 26, {nx, ny + 1, nz}, 5));
+  put(line30("X_COORDINATES     float", 15, 17, {nx}, 3));
+  for (int x = 0; x < nx; ++x) put_be32(b, (float)(lx * (double)x / (double)nx));
+  b.push_back('\n');
+  put(line30("Y_COORDINATES     float", 15, 17, {ny + 1}, 3));
+  for (int j = 0; j <= ny; ++j) put_be32(b, (float)yh[j]);
+  b.push_back('\n');
+  put(line30("Z_COORDINATES     float", 15, 17, {nz}, 3));
+  for (int z = 0; z < nz; ++z) put_be32(b, (float)(lz * (double)z / (double)nz));
+  b.push_back('\n');
+  put(line30("POINT_DATA ", 12, 20, {(long)nx * (ny + 1) * nz}, 9));
+  put("VECTORS flowvec float\n");
+  const long ps = (long)nx * nz, cs = (long)(ny + 1) * ps;
+  for (int z = 0; z < nz; ++z)
+    for (int j = 0; j <= ny; ++j)
+      for (int x = 0; x < nx; ++x)
+        for (int c = 0; c < 3; ++c) put_be32(b, (float)phys[c * cs + j * ps + (long)z * nx + x]);
+  b.push_back('\n');
+  File fh;
+  fh.f = std::fopen(path, "wb");
+  if (!fh.f) throw chan_error(std::string("cannot open ") + path);
+  if (std::fwrite(b.data(), 1, b.size(), fh.f) != b.size()) throw chan_error(std::string("write failed: ") + path);
+}
+
+}  // namespace
+
+extern "C" {
+
+int chan_vtk_write(const char* path, int nx, int ny, int nz, double lx, double lz, const double* yh,
+                   const double* phys) {
+  return io_guarded([&] { write_vtk(path, nx, ny, nz, lx, lz, yh, phys); });
+}
+
+}  // extern "C"

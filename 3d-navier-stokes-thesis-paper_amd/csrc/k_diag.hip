@@ -125,3 +125,32 @@ void launch_plane_moments(const double* phys, long ps, long cs, int nyp, double*
   hipLaunchKernelGGL(k_plane_moments, dim3(nyp), dim3(256), 0, s, phys, ps, cs, nyp, out);
   LAUNCH_CHECK("k_plane_moments");
 }
+
+// Spectral field of the VTK output: u and w at plane j, v at yh(j) as
+// half*(v(j-1) + v(j)), v(1) at the bottom and v(ny) at the top plane
+// (PFC/read_write.f90:418-432, yskip = 1).
+__global__ __launch_bounds__(256) void k_vtk_field(int ny, long nm, const double2* __restrict__ vel,
+                                                   double2* __restrict__ out) {
+  const int nyp = ny + 1;
+  const long tot = 3L * nyp * nm;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long m = e % nm, t = e / nm;
+    const int j = (int)(t % nyp), c = (int)(t / nyp);
+    double2 v;
+    if (c != 1) v = vel[e];
+    else if (j == 0) v = vel[(1L * nyp + 0) * nm + m];
+    else if (j == ny) v = vel[(1L * nyp + ny - 1) * nm + m];
+    else {
+      const double2 a = vel[(1L * nyp + j - 1) * nm + m], b = vel[(1L * nyp + j) * nm + m];
+      v = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    }
+    out[e] = v;
+  }
+}
+
+void launch_vtk_field(int ny, long nm, const double2* vel, double2* out, hipStream_t s) {
+  const long tot = 3L * (ny + 1) * nm;
+  const unsigned blocks = (unsigned)std::min<long>((tot + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_vtk_field, dim3(blocks), dim3(256), 0, s, ny, nm, vel, out);
+  LAUNCH_CHECK("k_vtk_field");
+}

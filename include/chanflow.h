@@ -247,6 +247,20 @@ int chan_mlp_gradient(chan_mlp* net, int n, const float* x, const float* t, floa
 int chan_mlp_train(chan_mlp* net, int n, const float* x, const float* t, int epochs, int batch, double lr,
                    double* loss_out);
 
+/* Legacy VTK output as output_hdf_dfsd writes it with xskip = yskip = zskip = 1
+ * (PFC/read_write.f90:296-500): big-endian RECTILINEAR_GRID, nx x (ny+1) x nz
+ * points (x = lx i/nx, y = yh, z = lz k/nz), float32 vectors "flowvec".
+ * chan_vtk_write (no context, no GPU) formats phys real(nx, nz, ny+1, 3),
+ *   i.e. [3][ny+1][nz][nx], with u, v, w on the yh planes; errors in
+ *   chan_io_last_error.
+ * chan_write_vtk takes the device velocity: u and w at plane j, v averaged
+ *   to yh (v(1) at the bottom plane, v(ny) at the top) in spectral space, then
+ *   the small transform to the nx x nz grid; yh(ny+1) is needed for the
+ *   coordinates. */
+int chan_vtk_write(const char* path, int nx, int ny, int nz, double lx, double lz, const double* yh,
+                   const double* phys);
+int chan_write_vtk(chan_ctx* ctx, const char* path, const double* yh);
+
 /* Velocity checkpoints in the reference's savevel.out / newvel.out format:
  * read_velocity and SAVE_VELOCITY (PFC/read_write.f90:21-176, 180-292),
  * Fortran sequential unformatted records with 4-byte length markers: one

@@ -34,3 +34,27 @@ def test_save_and_read_velocity():
         c.read_velocity(path)
     for ch in (a, b, c):
         ch.close()
+
+
+def test_write_vtk_from_device():
+    import struct
+    os.makedirs(WORK, exist_ok=True)
+    g = O.make_grid(16, 17, 12, 24, 18, 2.0, 2.0, 1000.0, stretch=1.5)
+    ch = make_channel(g)
+    U = O.remove_initial_divergence(g, init_cfg0(O, g, seed=4, amp=1e-1))
+    ch.set_state(data1=U)
+    path = os.path.join(WORK, "t_000.vtk")
+    ch.write_vtk(path, g.yh)
+    raw = open(path, "rb").read()
+    i = raw.index(b"VECTORS flowvec float\n") + len(b"VECTORS flowvec float\n")
+    n = g.nx * (g.ny + 1) * g.nz * 3
+    got = np.array(struct.unpack(f">{n}f", raw[i:i + 4 * n])).reshape(g.nz, g.ny + 1, g.nx, 3)
+    # expected: u, w at the planes, v averaged to yh, small transform
+    F = U.copy()
+    F[1, 0] = U[1, 0]
+    F[1, 1:g.ny] = 0.5 * (U[1, 0:g.ny - 1] + U[1, 1:g.ny])
+    F[1, g.ny] = U[1, g.ny - 1]
+    phys = O.small_c2r(g, F.reshape(-1, g.nxh, g.nz)).reshape(3, g.ny + 1, g.nz, g.nx)
+    exp = np.transpose(phys, (2, 1, 3, 0)).astype(np.float32)
+    assert np.abs(got - exp).max() <= 2.0 ** -22 * max(1.0, np.abs(exp).max())
+    ch.close()
