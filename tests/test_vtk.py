@@ -58,4 +58,4 @@ def test_vtk_bytes_match_reference_format():
     got = open(path, "rb").read()
     assert got == _expected(nx, ny, nz, 2.5, 1.25, yh, phys)
     assert got.startswith(b"# vtk DataFile Version 2.0\n")
-    assert b"DIMENSIONS      8    7    4    \n" in got
+    assert b"DIMENSIONS     8    7    4    \n" in got
