@@ -474,33 +474,41 @@ def test_wall_control_law_fixed_point():
 
 
 # --------------------------------------------------------------------------- Arnoldi / OS known answer
-@pytest.mark.slow
+OS_FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "os_block_eigs.npz")
+
+
 def test_orr_sommerfeld_known_answer():
-    """Leading eigenvalue of the linearized propagator (PFC/arnoldi.f90:485-567,
-    :805-806) about the laminar profile at Re = 7500, alpha = 1 has the phase
-    speed c_r = 0.24989154 hard-coded in PFC/setup.f90:1387 (least-stable
-    Orr-Sommerfeld mode)."""
-    from scipy.sparse.linalg import LinearOperator, eigs
-    g = O.make_grid(4, 97, 4, 6, 6, 2.0, 0.5, 7500.0, stretch=2.75, fix_mass=0)
+    """The leading Ritz value of the linearized propagator about the laminar
+    profile at Re = 7500 in the (kx = 1, kz = 0) block (written by
+    tests/golden/make_os_fixture.py from the oracle) is the least-stable
+    Orr-Sommerfeld mode: phase speed c_r = 0.24989154, the value the reference
+    hard-codes (PFC/setup.f90:1387), and the textbook growth rate
+    alpha c_i = 0.00223497, both to the second-order accuracy of the
+    97-point stretched grid."""
+    f = np.load(OS_FIXTURE)
+    cr, ci = f["cr"], f["ci"]
+    i = int(np.argmax(cr))
+    assert abs(abs(ci[i]) - 0.24989154) < 2e-3
+    assert abs(cr[i] - 0.00223497) < 0.1 * 0.00223497
+    # the eigenvalue map of the fixture is eig_map of its Ritz values
+    cr2, ci2 = O.eig_map(f["lam"], float(f["t_total"]))
+    np.testing.assert_allclose(cr2, cr, rtol=0, atol=1e-15)
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(os.environ.get("CHANFLOW_SLOW_TESTS") != "1", reason="about 80 s; set CHANFLOW_SLOW_TESTS=1")
+def test_orr_sommerfeld_fixture_reproducible():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk_os", os.path.join(os.path.dirname(__file__), "golden",
+                                                                        "make_os_fixture.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    g = mk.grid()
     base = O.laminar_baseflow(g)
     dt = O.linearized_dt(g)
     tend = O.linearized_tend(dt)
-    shp = (3, g.ny + 1, g.nz, g.nx)
-    n = int(np.prod(shp))
-    T = [0.0]
-
-    def mv(v):
-        w, t = O.compute_time_deriv(g, v.reshape(shp), base, dt, tend)
-        T[0] = t
-        return w.ravel()
-    op = LinearOperator((n, n), matvec=mv, dtype=np.float64)
-    lam = eigs(op, k=4, which="LM", tol=1e-10, v0=np.random.default_rng(0).standard_normal(n),
-               return_eigenvectors=False)
-    cr, ci = O.eig_map(lam, T[0])
-    i = int(np.argmax(cr))
-    c_phase = -ci[i] / g.kx[1]
-    assert abs(abs(c_phase) - 0.24989154) < 2e-3, (cr, ci)
-    assert cr[i] > 0.0  # the mode is (weakly) unstable at Re = 7500
+    lam = mk.leading_eigs(g, lambda v: O.compute_time_deriv(g, v, base, dt, tend)[0])
+    np.testing.assert_allclose(lam, np.load(OS_FIXTURE)["lam"], rtol=1e-9)
 
 
 # ---------------------------------------------------------------------------
