@@ -267,7 +267,7 @@ __device__ __forceinline__ void fft2_pass1(double2* seq, int b, const double2* r
   rdft<N1, SGN, N2>(v, rg);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) {
-    double2 w = rl[b * k1];
+    double2 w = rl[k1 * N2 + b];
     if (SGN > 0) w = cconj(w);
     v[k1] = cmul(v[k1], w);
   }
@@ -288,8 +288,10 @@ __device__ __forceinline__ void fft2_pass2(double2* seq, int k1, const double2* 
 }
 
 // In-place two-pass FFT of B sequences (sequence s at lds + s*sstride).
-// rl: root table exp(-2 pi i t/N), t < N, in LDS; rg: the same table in
-// global memory (scalar-loaded constants of the register DFTs).  Call with
+// rl: the inter-pass twiddles in LDS, rl[k1*N2 + b] = exp(-2 pi i b k1/N)
+// (load_tw2), so that the lanes of a pass-1 sweep (consecutive b) read
+// consecutive cells; rg: the length-N root table in global memory
+// (scalar-loaded constants of the register DFTs).  Call with
 // the input visible to all threads (a barrier before); ends with a barrier.
 template <int N, int SGN>
 __device__ __forceinline__ void fft2p(double2* lds, int sstride, int B, const double2* rl,
@@ -361,6 +363,21 @@ __device__ __forceinline__ void fft_run(double2* lds, int sstride, int B, const 
 __device__ __forceinline__ void load_roots(double2* dst, const double2* __restrict__ src, int n, int tid,
                                            int nthr) {
   for (int i = tid; i < n; i += nthr) dst[i] = src[i];
+}
+
+// The inter-pass twiddles of the two-pass length-NC transform in LDS:
+// dst[k1*N2 + b] = roots[b*k1] (b*k1 < N always), NC cells; NC == 0 (the
+// runtime-planned transform) copies nothing.  Whole workgroup; caller
+// synchronises.
+template <int NC>
+__device__ __forceinline__ void load_tw2(double2* dst, const double2* __restrict__ roots, int tid, int nthr) {
+  if constexpr (NC > 0) {
+    constexpr int N2 = fft2_split(NC).n2;
+    for (int e = tid; e < NC; e += nthr) {
+      const int k1 = e / N2, b = e - k1 * N2;
+      dst[e] = roots[b * k1];
+    }
+  }
 }
 
 // Dispatch a launcher body over the compile-time lengths: FFT_DISPATCH(n, F)

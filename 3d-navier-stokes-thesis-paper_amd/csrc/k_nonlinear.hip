@@ -69,7 +69,7 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
   const double2* rootx = tw + g.twx;
   const double2 wconj = cconj(rootx[n < M ? n : 0]);
   double2* rl = lds + 8 * RZ * SS;
-  if (MC > 0) load_roots(rl, tw + g.px.tw, M, tid, nthr);
+  load_tw2<MC>(rl, tw + g.px.tw, tid, nthr);
   const int pin = fft_in_pos<MC>(n < M ? n : 0), pout = fft_out_pos<MC>(n < M ? n : 0);
   const long rowoff = (long)zz * kxp;
 
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   constexpr int SEQ = fft_seq_cells(M), PPL = (M + 63) / 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double2* rl = lds;
-  load_roots(rl, tw + g.px.tw, M, tid, 256);
+  load_tw2<M>(rl, tw + g.px.tw, tid, 256);
   __syncthreads();
   const int zz = blockIdx.x * 4 + wv;
   if (zz >= g.nz2) return;  // after the only workgroup barrier

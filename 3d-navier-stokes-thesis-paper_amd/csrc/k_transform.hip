@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
   const long P = blockIdx.y;
   const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   double2* rl = lds + ZT_KX * ss;
-  if (NC > 0) load_roots(rl, tw + g.pz.tw, nz2, tid, nthr);
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
   const int ncol = min(ZT_KX, g.nxh - kx0);
   const double2* src = in + P * in_ps;
   const bool pad = (nz2 > nz) && !g.small;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
   }
   const int nf = B ? 2 : 1;
   double2* rl = lds + nf * ZT_KX * ss;
-  if (NC > 0) load_roots(rl, tw + g.pz.tw, nz2, tid, nthr);
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
   for (int e = tid; e < nf * nz2 * ZT_KX; e += nthr) {
     int f = e / (nz2 * ZT_KX);
     int e2 = e - f * nz2 * ZT_KX;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restri
     }
   }
   double2* rl = lds + ZT_KX * SS;
-  load_roots(rl, tw + g.pz.tw, NZ2, tid, 256);
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, 256);
   const double2* Ap = A + P * ab_ps;
   const double2* Bp = B + P * ab_ps;
   double2 rb[NLD];
