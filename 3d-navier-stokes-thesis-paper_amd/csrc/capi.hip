@@ -166,7 +166,7 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
   if (cfl) launch_fill_zero(c->d_cfl, sizeof(unsigned long long) * 3 * nyp, c->stream);
   {
     KTimer t(c, KT_ZINV);
-    launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream);
+    launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream, c->tune.zinv_kx);
   }
   {
     KTimer t(c, KT_XPASS);
@@ -174,13 +174,13 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
     a.cflbits = cfl ? c->d_cfl : nullptr;
     a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->xpass_chunk; a.rz = 0;
-    a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads;
+    a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads; a.prefetch = c->tune.xpass_prefetch;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
   }
   {
     KTimer t(c, KT_ZFWD);
     launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * nyp, ny,
-                c->stream, c->tune.zfwd_v2);
+                c->stream, c->tune.zfwd_v2, c->tune.zfwd_kx);
   }
   if (cfl) launch_cfl_finalize(c->ops, c->prm, c->d_cfl, c->d_sc, step, c->stream);
 }
@@ -339,6 +339,9 @@ Tuning tuning_defaults() {
   v = t.zfwd_v2; env("CHANFLOW_ZFWD_V2", v); tuning_set(t, 2, v);
   v = t.solve_lanes; env("CHANFLOW_SOLVE_LANES", v); tuning_set(t, 3, v);
   v = t.solve_bl; env("CHANFLOW_SOLVE_BL", v); tuning_set(t, 4, v);
+  v = t.zinv_kx; env("CHANFLOW_ZINV_KX", v); tuning_set(t, 5, v);
+  v = t.zfwd_kx; env("CHANFLOW_ZFWD_KX", v); tuning_set(t, 6, v);
+  v = t.xpass_prefetch; env("CHANFLOW_XPASS_PREFETCH", v); tuning_set(t, 7, v);
   return t;
 }
 
@@ -349,6 +352,9 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 2: if (v != 0 && v != 1) return false; t.zfwd_v2 = v; return true;
     case 3: if (v != 1 && v != 3) return false; t.solve_lanes = v; return true;
     case 4: if (v != 4 && v != 8 && v != 16) return false; t.solve_bl = v; return true;
+    case 5: if (v != 4 && v != 8 && v != 16) return false; t.zinv_kx = v; return true;
+    case 6: if (v != 4 && v != 8) return false; t.zfwd_kx = v; return true;
+    case 7: if (v != 0 && v != 1) return false; t.xpass_prefetch = v; return true;
     default: return false;
   }
 }
@@ -360,6 +366,9 @@ int tuning_get(const Tuning& t, int key) {
     case 2: return t.zfwd_v2;
     case 3: return t.solve_lanes;
     case 4: return t.solve_bl;
+    case 5: return t.zinv_kx;
+    case 6: return t.zfwd_kx;
+    case 7: return t.xpass_prefetch;
     default: return -1;
   }
 }

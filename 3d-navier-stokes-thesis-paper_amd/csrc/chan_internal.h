@@ -75,6 +75,7 @@ struct XPassArgs {
   int rz;                 // rows per workgroup
   int wave;               // 1: wave-per-row kernel where the length allows it
   int maxthr;             // workgroup-kernel threads bound (64..640)
+  int prefetch;           // 1: direct global-to-LDS prefetch of the next plane's rows
 };
 
 // Launch-shape choices of one context (chan_set_tuning, chan_autotune);
@@ -86,6 +87,9 @@ struct Tuning {
   int zfwd_v2 = 1;        // sequential-A/B z-forward kernel
   int solve_lanes = 3;    // 3: component-parallel solve, 1: one lane per mode
   int solve_bl = 8;       // rows per prefetch block (4, 8, 16)
+  int zinv_kx = 8;        // z-inverse kx columns per workgroup (4, 8, 16)
+  int zfwd_kx = 8;        // z-forward kx columns per workgroup (4, 8)
+  int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -147,14 +151,14 @@ void xz_geom_init(XZGeom& g, int nx, int nz, int nx2, int nz2, int small, std::v
 
 // launchers (k_transform.hip)
 void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
-                 long out_ps, int nplanes, hipStream_t s);
+                 long out_ps, int nplanes, hipStream_t s, int kx = 8);
 void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps, double* out,
                  long out_ps, int nplanes, hipStream_t s);
 void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const double* fx,
                  const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s);
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s, int v2 = 1);
+                 hipStream_t s, int v2 = 1, int kx = 8);
 // (k_nonlinear.hip)
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
                   hipStream_t s);

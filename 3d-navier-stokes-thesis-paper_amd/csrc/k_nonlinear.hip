@@ -263,18 +263,27 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
 // v-interp w} give Bw, Av, Bv.  The four wavefronts of a workgroup share
 // only the root table.
 // ---------------------------------------------------------------------------
-template <bool LIN, int M>
+//
+// With PF the raw rows of the next plane are streamed into a per-wavefront
+// LDS buffer by direct global-to-LDS loads (global_load_lds_dwordx4: lane i
+// writes cells base + i) as soon as the current rows have been consumed, so
+// their latency overlaps a whole plane of work instead of stalling the
+// wavefront at the start of every plane.
+template <bool LIN, int M, bool PF>
 __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
                                                   XPassArgs a) {
   extern __shared__ double2 lds[];
   constexpr int SEQ = fft_seq_cells(M), PPL = (M + 63) / 64;
+  constexpr int RAWN = ((M + 1 + 63) / 64) * 64;          // cells per raw row (nx/2+1 <= M+1)
+  constexpr int WSZ = 4 * SEQ + (PF ? 3 * RAWN : 0);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double2* rl = lds;
   load_tw2<M>(rl, tw + g.px.tw, tid, 256);
   __syncthreads();
   const int zz = blockIdx.x * 4 + wv;
   if (zz >= g.nz2) return;  // after the only workgroup barrier
-  double2* W = lds + M + wv * 4 * SEQ;
+  double2* W = lds + M + wv * WSZ;
+  double2* RAW = W + 4 * SEQ;
   const double2* rg = tw + g.px.tw;
   const double2* rootx = tw + g.twx;
   const int ny = a.ny, nyp = ny + 1;
@@ -297,11 +306,34 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
     pout[q] = fft2_out_pos<M>(n);
   }
 
+  // raw rows of plane p (1-based) into RAW[c][0..nxh) (PF only); every
+  // global address is clamped to an element of the row
+  auto fetch = [&](int p) {
+    if constexpr (PF) {
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): earlier reads of RAW are done
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
+        for (int i = 0; 64 * i < nxh; ++i) {
+          const int idx = min(64 * i + lane, nxh - 1);
+          __builtin_amdgcn_global_load_lds((const void*)(row + idx),
+                                           (__attribute__((address_space(3))) void*)(RAW + c * RAWN + 64 * i),
+                                           16, 0, 0);
+        }
+      }
+    }
+  };
+  auto ready = [&]() {
+    if constexpr (PF) {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the raw rows have landed
+      wave_sync();
+    }
+  };
   // c2r of plane p (1-based) into U, V, Wv at the owned points
   auto c2r_plane = [&](int p, double2* U, double2* V, double2* Wv, bool needv) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
+      const double2* row = PF ? RAW + c * RAWN : a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
 #pragma unroll
       for (int q = 0; q < PPL; ++q) {
         if (!vq[q]) continue;
@@ -350,9 +382,14 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   const double half = 0.5, two = 2.0;
 
   // prologue: planes ka-1 and ka, fluxes at y(ka-1) and yh(ka)
+  fetch(ka - 1);
+  ready();
   c2r_plane(ka - 1, U0, V0, W0, true);
+  fetch(ka);
+  ready();
   load_base(ka - 1, BU0, BV0, BW0);
   c2r_plane(ka, U1, V1, W1, true);
+  fetch(ka + 1);
   load_base(ka, BU1, BV1, BW1);
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
@@ -381,7 +418,9 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
 
   for (int k = ka; k <= kb; ++k) {
     const bool top = (k == ny);
+    ready();
     c2r_plane(k + 1, U1, V1, W1, !top);
+    if (k < kb) fetch(k + 2);
     load_base(k + 1, BU1, BV1, BW1);
     const int kk = k - 1;  // 0-based plane index
     const double dh = ops.dh[kk];
@@ -553,11 +592,18 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
   int nch = (nplanes + a.chunk - 1) / a.chunk;
   if (a.wave && ct && M <= 192) {
     dim3 gw((g.nz2 + 3) / 4, nch);
-    size_t ldw = sizeof(double2) * (M + 4 * 4 * fft_seq_cells(M));
-#define XPASSW_CASE(N)                                                                            \
-  if constexpr (N > 0 && N <= 192) {                                                              \
-    if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N>), gw, dim3(256), ldw, s, g, tw, ops, a);   \
-    else hipLaunchKernelGGL((k_xpass_w<false, N>), gw, dim3(256), ldw, s, g, tw, ops, a);         \
+    const bool pf = a.prefetch && M <= 144;
+    const int rawn = ((M + 1 + 63) / 64) * 64;
+    size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
+#define XPASSW_CASE(N)                                                                                  \
+  if constexpr (N > 0 && N <= 192) {                                                                    \
+    if (pf) {                                                                                           \
+      if constexpr (N <= 144) {                                                                         \
+        if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a); \
+        else hipLaunchKernelGGL((k_xpass_w<false, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a);       \
+      }                                                                                                 \
+    } else if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N, false>), gw, dim3(256), ldw, s, g, tw, ops, a); \
+    else hipLaunchKernelGGL((k_xpass_w<false, N, false>), gw, dim3(256), ldw, s, g, tw, ops, a);        \
   }
     FFT_DISPATCH(M, XPASSW_CASE)
 #undef XPASSW_CASE

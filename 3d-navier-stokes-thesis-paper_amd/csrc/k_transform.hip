@@ -227,8 +227,52 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
 // before A's transform so that their latency overlaps it; A's folded and
 // scaled values wait in registers for B's.  Used for the compile-time
 // lengths with a second field (the nonlinear pass).
-template <int NC>
-__global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restrict__ tw,
+// Compile-time variant of k_zinv with KX columns per workgroup of 32*KX threads.
+template <int NC, int KX>
+__global__ __launch_bounds__(32 * KX) void k_zinv2(XZGeom g, const double2* __restrict__ tw,
+                                                   const double2* __restrict__ in, long in_ps,
+                                                   double2* __restrict__ out, long out_ps) {
+  extern __shared__ double2 lds[];
+  constexpr int NT = 32 * KX, NZ2 = NC, SS = fft_seq_cells(NC);
+  const int tid = threadIdx.x;
+  const int kx0 = blockIdx.x * KX;
+  const long P = blockIdx.y;
+  const int nz = g.nz;
+  double2* rl = lds + KX * SS;
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);
+  const int ncol = min(KX, g.nxh - kx0);
+  const double2* src = in + P * in_ps;
+  const bool pad = (NZ2 > nz) && !g.small;
+  const bool halve = g.nx2 > g.nx;
+  const int h = nz / 2;
+  for (int e = tid; e < KX * NZ2; e += NT) {
+    const int s = e / NZ2, t = e - s * NZ2;
+    double2 v = make_double2(0.0, 0.0);
+    if (s < ncol) {
+      const double2* col = src + (long)(kx0 + s) * nz;
+      if (pad) {
+        if (t < h) v = col[t];
+        else if (t == h) v = cscale(0.5, col[h]);
+        else if (t == NZ2 - h) v = cscale(0.5, cconj(col[h]));
+        else if (t > NZ2 - h) v = col[t - (NZ2 - nz)];
+        if (halve && kx0 + s == g.nxh - 1) v = cscale(0.5, v);
+      } else {
+        v = col[t];
+      }
+    }
+    lds[s * SS + fft2_in_pos<NC>(t)] = v;
+  }
+  __syncthreads();
+  fft2p<NC, 1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
+  double2* dst = out + P * out_ps;
+  for (int e = tid; e < NZ2 * KX; e += NT) {
+    const int t = e / KX, s = e - t * KX;
+    if (s < ncol) dst[(long)t * g.kxpad + kx0 + s] = lds[s * SS + fft2_out_pos<NC>(t)];
+  }
+}
+
+template <int NC, int KX>
+__global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __restrict__ tw,
                                                const double2* __restrict__ A,
                                                const double2* __restrict__ B, long ab_ps,
                                                const double* __restrict__ kzv,
@@ -236,12 +280,13 @@ __global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restri
                                                int nl_zero_ny) {
   extern __shared__ double2 lds[];
   constexpr int NZ2 = NC, SS = fft_seq_cells(NC);
-  constexpr int NLD = (NZ2 * ZT_KX + 255) / 256;      // loads per thread per field
+  constexpr int NT = 32 * KX;
+  constexpr int NLD = (NZ2 * KX + NT - 1) / NT;       // loads per thread per field
   const int tid = threadIdx.x;
-  const int kx0 = blockIdx.x * ZT_KX;
+  const int kx0 = blockIdx.x * KX;
   const long P = blockIdx.y;
   const int nz = g.nz;
-  const int ncol = min(ZT_KX, g.nxh - kx0);
+  const int ncol = min(KX, g.nxh - kx0);
   double2* dst = out + P * out_ps;
   if (nl_zero_ny > 0) {
     // NL planes that the reference leaves zero: plane 1, plane ny+1, and v at
@@ -249,32 +294,32 @@ __global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restri
     const int ny = nl_zero_ny;
     const int c = (int)(P / (ny + 1)), j = (int)(P % (ny + 1));
     if (j == 0 || j == ny || (c == 1 && j == ny - 1)) {
-      for (int e = tid; e < ncol * nz; e += 256) {
+      for (int e = tid; e < ncol * nz; e += NT) {
         const int s = e / nz, q = e - s * nz;
         dst[(long)(kx0 + s) * nz + q] = make_double2(0.0, 0.0);
       }
       return;
     }
   }
-  double2* rl = lds + ZT_KX * SS;
-  load_tw2<NC>(rl, tw + g.pz.tw, tid, 256);
+  double2* rl = lds + KX * SS;
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);
   const double2* Ap = A + P * ab_ps;
   const double2* Bp = B + P * ab_ps;
   double2 rb[NLD];
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
-    const int e = tid + 256 * k;
-    const int t = e / ZT_KX, s = e - t * ZT_KX;
+    const int e = tid + NT * k;
+    const int t = e / KX, s = e - t * KX;
     double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-    if (e < NZ2 * ZT_KX && s < ncol) {
+    if (e < NZ2 * KX && s < ncol) {
       va = Ap[(long)t * g.kxpad + kx0 + s];
       vb = Bp[(long)t * g.kxpad + kx0 + s];
     }
-    if (e < NZ2 * ZT_KX) lds[s * SS + fft2_in_pos<NC>(t)] = va;
+    if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = va;
     rb[k] = vb;
   }
   __syncthreads();
-  fft2p<NC, -1>(lds, SS, ZT_KX, rl, tw + g.pz.tw, tid, 256);
+  fft2p<NC, -1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
   const int h = nz / 2, off = NZ2 - nz;
   const double scale = 1.0 / (double)(g.nx2 * NZ2);
   auto fold = [&](const double2* a, int q) {
@@ -282,11 +327,11 @@ __global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restri
     if (q == h) return cadd(a[fft2_out_pos<NC>(h)], cconj(a[fft2_out_pos<NC>(off + h)]));
     return a[fft2_out_pos<NC>(q + off)];
   };
-  constexpr int NOUT = (ZT_KX * 192 + 255) / 256;      // outputs per thread for nz <= 192
+  constexpr int NOUT = (KX * 192 + NT - 1) / NT;     // outputs per thread for nz <= 192
   double2 acc[NOUT];
 #pragma unroll
   for (int k = 0; k < NOUT; ++k) {
-    const int e = tid + 256 * k;
+    const int e = tid + NT * k;
     acc[k] = make_double2(0.0, 0.0);
     if (e < ncol * nz) {
       const int s = e / nz, q = e - s * nz;
@@ -297,15 +342,15 @@ __global__ __launch_bounds__(256) void k_zfwd2(XZGeom g, const double2* __restri
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
-    const int e = tid + 256 * k;
-    const int t = e / ZT_KX, s = e - t * ZT_KX;
-    if (e < NZ2 * ZT_KX) lds[s * SS + fft2_in_pos<NC>(t)] = rb[k];
+    const int e = tid + NT * k;
+    const int t = e / KX, s = e - t * KX;
+    if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = rb[k];
   }
   __syncthreads();
-  fft2p<NC, -1>(lds, SS, ZT_KX, rl, tw + g.pz.tw, tid, 256);
+  fft2p<NC, -1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
 #pragma unroll
   for (int k = 0; k < NOUT; ++k) {
-    const int e = tid + 256 * k;
+    const int e = tid + NT * k;
     if (e < ncol * nz) {
       const int s = e / nz, q = e - s * nz;
       const double f = (kx0 + s == g.nxh - 1) ? 2.0 * scale : scale;
@@ -329,10 +374,23 @@ static void check_stage_fit(const FFTPlan& p, int nseq, int nthr) {
 }
 
 void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
-                 long out_ps, int nplanes, hipStream_t s) {
+                 long out_ps, int nplanes, hipStream_t s, int kx) {
+  const bool ct = fft2_split(g.nz2).n1 > 0;
+  if (ct && (kx == 4 || kx == 16)) {
+    dim3 gk((g.nxh + kx - 1) / kx, nplanes);
+    const size_t lk = sizeof(double2) * (kx * fft_seq_cells(g.nz2) + g.nz2);
+#define ZINV2_CASE(N)                                                                                      \
+  if constexpr (N > 0) {                                                                                   \
+    if (kx == 4) hipLaunchKernelGGL((k_zinv2<N, 4>), gk, dim3(128), lk, s, g, tw, in, in_ps, out, out_ps);  \
+    else hipLaunchKernelGGL((k_zinv2<N, 16>), gk, dim3(512), lk, s, g, tw, in, in_ps, out, out_ps);          \
+  }
+    FFT_DISPATCH(g.nz2, ZINV2_CASE)
+#undef ZINV2_CASE
+    LAUNCH_CHECK("k_zinv2");
+    return;
+  }
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nthr = 256;
-  const bool ct = fft2_split(g.nz2).n1 > 0;
   if (!ct) check_stage_fit(g.pz, ZT_KX, nthr);
   size_t lds = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
 #define ZINV_CASE(N) \
@@ -376,16 +434,22 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
 
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s, int v2) {
+                 hipStream_t s, int v2, int kx) {
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nf = B ? 2 : 1;
   const int nthr = 256 * nf;
   const bool ct = fft2_split(g.nz2).n1 > 0;
   if (v2 && ct && B && !g.small && g.nz <= 192) {
-    size_t l2 = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + g.nz2);
-#define ZFWD2_CASE(N)                                                                                \
-  if constexpr (N > 0) hipLaunchKernelGGL(k_zfwd2<N>, grid, dim3(256), l2, s, g, tw, A, B, ab_ps, kz, out, \
-                                          out_ps, nl_zero_ny)
+    const int k2 = (kx == 4) ? 4 : 8;
+    dim3 g2((g.nxh + k2 - 1) / k2, nplanes);
+    size_t l2 = sizeof(double2) * (k2 * fft_seq_cells(g.nz2) + g.nz2);
+#define ZFWD2_CASE(N)                                                                                       \
+  if constexpr (N > 0) {                                                                                    \
+    if (k2 == 4) hipLaunchKernelGGL((k_zfwd2<N, 4>), g2, dim3(128), l2, s, g, tw, A, B, ab_ps, kz, out, out_ps, \
+                                    nl_zero_ny);                                                            \
+    else hipLaunchKernelGGL((k_zfwd2<N, 8>), g2, dim3(256), l2, s, g, tw, A, B, ab_ps, kz, out, out_ps,       \
+                            nl_zero_ny);                                                                    \
+  }
     FFT_DISPATCH(g.nz2, ZFWD2_CASE)
 #undef ZFWD2_CASE
     LAUNCH_CHECK("k_zfwd2");

@@ -200,7 +200,10 @@ int chan_plane_moments(chan_ctx* ctx, double* out);
  * Keys: 0 x-pass kernel (1: one wavefront per z-row, 0: workgroup per rows),
  * 1 x-pass workgroup thread bound (64..640), 2 z-forward kernel (1: A and B
  * transformed in turn, 0: together), 3 solve lanes per mode (3 or 1),
- * 4 solve rows per prefetch block (4, 8 or 16).  chan_autotune times `steps`
+ * 4 solve rows per prefetch block (4, 8 or 16), 5 z-inverse kx columns per
+ * workgroup (4, 8 or 16), 6 z-forward kx columns per workgroup (4 or 8),
+ * 7 x-pass prefetch of the next plane's rows into LDS (1 on, 0 off).
+ * chan_autotune times `steps`
  * timesteps of every candidate, keeps the fastest and restores the state. */
 int chan_set_tuning(chan_ctx* ctx, int key, int value);
 int chan_get_tuning(chan_ctx* ctx, int key, int* value);
