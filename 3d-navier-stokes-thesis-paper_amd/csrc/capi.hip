@@ -192,7 +192,7 @@ void solve_pass(chan_ctx* c, int rk) {
   a.sc = c->d_sc; a.scr = c->d_scr; a.rk = rk; a.ny = c->prm.ny; a.nxh = c->nxh; a.nz = c->prm.nz;
   a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
   a.fix_mass = c->prm.fix_mass;
-  a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl;
+  a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl; a.dma = c->tune.solve_dma;
   launch_solve(c->ops, a, c->stream);
 }
 
@@ -327,7 +327,8 @@ void prepare_capture(chan_ctx* c) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl.
+// Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
+// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma.
 Tuning tuning_defaults() {
   Tuning t;
   auto env = [](const char* n, int& v) {
@@ -342,6 +343,7 @@ Tuning tuning_defaults() {
   v = t.zinv_kx; env("CHANFLOW_ZINV_KX", v); tuning_set(t, 5, v);
   v = t.zfwd_kx; env("CHANFLOW_ZFWD_KX", v); tuning_set(t, 6, v);
   v = t.xpass_prefetch; env("CHANFLOW_XPASS_PREFETCH", v); tuning_set(t, 7, v);
+  v = t.solve_dma; env("CHANFLOW_SOLVE_DMA", v); tuning_set(t, 8, v);
   return t;
 }
 
@@ -355,6 +357,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 5: if (v != 4 && v != 8 && v != 16) return false; t.zinv_kx = v; return true;
     case 6: if (v != 4 && v != 8) return false; t.zfwd_kx = v; return true;
     case 7: if (v != 0 && v != 1) return false; t.xpass_prefetch = v; return true;
+    case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
     default: return false;
   }
 }
@@ -369,6 +372,7 @@ int tuning_get(const Tuning& t, int key) {
     case 5: return t.zinv_kx;
     case 6: return t.zfwd_kx;
     case 7: return t.xpass_prefetch;
+    case 8: return t.solve_dma;
     default: return -1;
   }
 }
@@ -1195,11 +1199,13 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
     for (int w : {1, 0})
       for (int z : {1, 0})
         for (int sl : {3, 1})
-          for (int bl : {8, 16}) {
-            Tuning t = c->tune;
-            t.xpass_wave = w; t.zfwd_v2 = z; t.solve_lanes = sl; t.solve_bl = bl;
-            cand.push_back(t);
-          }
+          for (int bl : {8, 16})
+            for (int dm : {1, 0}) {
+              if (dm && (sl != 3 || bl != 8)) continue;  // the ring variant has no block size
+              Tuning t = c->tune;
+              t.xpass_wave = w; t.zfwd_v2 = z; t.solve_lanes = sl; t.solve_bl = bl; t.solve_dma = dm;
+              cand.push_back(t);
+            }
     hipEvent_t a, b;
     CHAN_CHECK(hipEventCreate(&a));
     CHAN_CHECK(hipEventCreate(&b));

@@ -46,6 +46,14 @@ struct DevScalars {
   int pad;
 };
 
+// Load of data that no kernel writes while it runs, through the constant
+// address space: with a wavefront-uniform index it becomes a scalar load,
+// which is counted by lgkmcnt and so never waits on vector loads in flight.
+template <class T>
+__device__ __forceinline__ T ldro(const T* p, long i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Operators uploaded from the driver (layouts of the reference's globals).
 struct DevOps {
   const double* d2h;   // d2h(3, ny+1)
@@ -90,6 +98,7 @@ struct Tuning {
   int zinv_kx = 8;        // z-inverse kx columns per workgroup (4, 8, 16)
   int zfwd_kx = 8;        // z-forward kx columns per workgroup (4, 8)
   int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
+  int solve_dma = 0;      // 3-lane solve streams its rows through an LDS ring
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -177,6 +186,7 @@ struct SolveArgs {
   double re, cosb, sinb;
   int fix_mass;
   int lanes, bl;      // kernel variant (Tuning::solve_lanes, solve_bl)
+  int dma;            // 3-lane variant: rows streamed through an LDS ring (Tuning::solve_dma)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,

@@ -366,6 +366,21 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
       Wv[q] = *reinterpret_cast<const double2*>(b + 2 * cs + o);
     }
   };
+  // Everything the plane loop reads from global memory besides the raw rows
+  // is loaded ahead of the wait for those rows: with PF a later vector load
+  // would make its consumer wait for the in-flight prefetch as well.
+  // The columns kx < 2M/3+1 (all of them with 3/2 dealiasing) have their
+  // root and wavenumber in registers; wider rows load the rest in place.
+  constexpr int KQ = (M + 64) / 64;                        // output columns per lane (nx/2+1 <= M+1)
+  constexpr int KP = ((2 * M) / 3 + 1 + 63) / 64 < KQ ? ((2 * M) / 3 + 1 + 63) / 64 : KQ;
+  double2 rxl[KP];
+  double kxl[KP];
+#pragma unroll
+  for (int i = 0; i < KP; ++i) {
+    const int kq = min(lane + 64 * i, nxh - 1);
+    rxl[i] = rootx[kq];
+    kxl[i] = ops.kx[kq];
+  }
   auto bufv = [&](int q) {
     return *reinterpret_cast<const double2*>(ops.buffer + (long)zz * g.nx2 + 2 * nq[q]);
   };
@@ -386,11 +401,11 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   ready();
   c2r_plane(ka - 1, U0, V0, W0, true);
   fetch(ka);
-  ready();
   load_base(ka - 1, BU0, BV0, BW0);
+  load_base(ka, BU1, BV1, BW1);
+  ready();
   c2r_plane(ka, U1, V1, W1, true);
   fetch(ka + 1);
-  load_base(ka, BU1, BV1, BW1);
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     if (!LIN) {
@@ -418,14 +433,16 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
 
   for (int k = ka; k <= kb; ++k) {
     const bool top = (k == ny);
+    const int kk = k - 1;  // 0-based plane index
+    const double dh = ldro(ops.dh, kk);
+    const double dnk = top ? 0.0 : ldro(ops.dn, kk);
+    const double i1 = top ? 0.0 : ldro(ops.iu, 2 * kk), i2 = top ? 0.0 : ldro(ops.iu, 2 * kk + 1);
+    double tgu = 0.0, tgv = 0.0, tgw = 0.0;
+    if (hasbuf) { tgu = ldro(ops.target, 0 * nyp + kk); tgv = ldro(ops.target, 1 * nyp + kk); tgw = ldro(ops.target, 2 * nyp + kk); }
+    load_base(k + 1, BU1, BV1, BW1);
     ready();
     c2r_plane(k + 1, U1, V1, W1, !top);
     if (k < kb) fetch(k + 2);
-    load_base(k + 1, BU1, BV1, BW1);
-    const int kk = k - 1;  // 0-based plane index
-    const double dh = ops.dh[kk];
-    const double dnk = top ? 0.0 : ops.dn[kk];
-    const double i1 = top ? 0.0 : ops.iu[2 * kk], i2 = top ? 0.0 : ops.iu[2 * kk + 1];
     double2 Fu[PPL], Fw[PPL], G1[PPL];
     // batch 1: u^2, d/dy flux_u, w u, d/dy flux_w (:138-155)
 #pragma unroll
@@ -452,7 +469,7 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
       double2 yfw = psc(dh, psub(Fw[q], FW[q]));
       if (hasbuf) {
         const double2 bf = bufv(q);
-        const double tu = ops.target[0 * nyp + kk], tw_ = ops.target[2 * nyp + kk];
+        const double tu = tgu, tw_ = tgw;
         yfu = padd(yfu, pmul(bf, make_double2(U0[q].x - tu, U0[q].y - tu)));
         yfw = padd(yfw, pmul(bf, make_double2(W0[q].x - tw_, W0[q].y - tw_)));
       }
@@ -493,9 +510,12 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
     const long ab0 = ((long)((0 * nyp + kk) * 2) * nz2 + zz) * kxp;
     const long ab1 = ((long)((1 * nyp + kk) * 2) * nz2 + zz) * kxp;
     const long ab2 = ((long)((2 * nyp + kk) * 2) * nz2 + zz) * kxp;
-    for (int kq = lane; kq < nxh; kq += 64) {
-      const double2 wk = rootx[kq];
-      const double kxv = ops.kx[kq];
+#pragma unroll
+    for (int i = 0; i < KQ; ++i) {
+      const int kq = lane + 64 * i;
+      if (kq >= nxh) break;
+      const double2 wk = i < KP ? rxl[i] : rootx[kq];
+      const double kxv = i < KP ? kxl[i] : ops.kx[kq];
       auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
       const double2 X0 = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk), X1 = r2c_post_nl<M>(W + 1 * SEQ, M, kq, wk);
       const double2 X2 = r2c_post_nl<M>(W + 2 * SEQ, M, kq, wk), X3 = r2c_post_nl<M>(W + 3 * SEQ, M, kq, wk);
@@ -534,7 +554,7 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
         }
         double2 yfv = psc(dnk, psub(G1[q], GV[q]));                                    // :188-189
         if (hasbuf) {
-          const double tv = ops.target[1 * nyp + kk];
+          const double tv = tgv;
           yfv = padd(yfv, pmul(bufv(q), make_double2(V0[q].x - tv, V0[q].y - tv)));
         }
         W[1 * SEQ + pin[q]] = f5;
@@ -544,9 +564,12 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
     }
     wave_sync();
     fft2p_wave<M, -1>(W, SEQ, nb2, rl, rg, lane);
-    for (int kq = lane; kq < nxh; kq += 64) {
-      const double2 wk = rootx[kq];
-      const double kxv = ops.kx[kq];
+#pragma unroll
+    for (int i = 0; i < KQ; ++i) {
+      const int kq = lane + 64 * i;
+      if (kq >= nxh) break;
+      const double2 wk = i < KP ? rxl[i] : rootx[kq];
+      const double kxv = i < KP ? kxl[i] : ops.kx[kq];
       auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
       a.AB[ab2 + bst + kq] = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk);
       if (!top) {

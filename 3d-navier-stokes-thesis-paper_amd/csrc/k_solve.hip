@@ -439,8 +439,115 @@ __device__ __forceinline__ double2 shfl2(double2 v, int src) {
   return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-template <int BL>
+// ---- direct global-to-LDS row ring (the DMA variant of k_solve3) ----
+//
+// Each pass streams its rows through a ring of R slots in the wavefront's
+// LDS: global_load_lds writes lane i's 16 (or 4) bytes at slot base + 16i
+// (4i), so every lane reads back exactly what it fetched.  A row is consumed
+// R rows after its loads were issued; vector memory operations of a
+// wavefront complete in issue order, so once at most (R-1)*nld are still
+// outstanding (nld = loads per row) the oldest row has landed.  Stores issued
+// in between only make that wait stricter.  Rows past the end are replaced by
+// loads of each array's first element (one cache line) to keep the count
+// uniform.
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt has 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
+template <int R>
+__device__ __forceinline__ void wait_ring(int nld) {
+  constexpr int M1 = R - 1;
+  switch (nld) {  // a count above 63 is clamped, which only waits for more
+    case 2: wait_vm<(2 * M1 < 63 ? 2 * M1 : 63)>(); break;
+    case 3: wait_vm<(3 * M1 < 63 ? 3 * M1 : 63)>(); break;
+    case 4: wait_vm<(4 * M1 < 63 ? 4 * M1 : 63)>(); break;
+    case 5: wait_vm<(5 * M1 < 63 ? 5 * M1 : 63)>(); break;
+    case 6: wait_vm<(6 * M1 < 63 ? 6 * M1 : 63)>(); break;
+    default: wait_vm<0>(); break;
+  }
+}
+__device__ __forceinline__ void dma16(const void* g, char* lds_uniform) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_uniform, 16, 0, 0);
+}
+__device__ __forceinline__ void dma4(const void* g, char* lds_uniform) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_uniform, 4, 0, 0);
+}
+// n rows; row(t) is the t-th row; fetch(row or -1, slot) issues nld loads;
+// read(slot) returns the row data of this lane; comp(row, data) consumes it
+template <int R, class RowF, class FetchF, class ReadF, class CompF>
+__device__ __forceinline__ void ring_rows(int n, int nld, RowF row, FetchF fetch, ReadF read, CompF comp) {
+#pragma unroll
+  for (int t = 0; t < R; ++t) fetch(t < n ? row(t) : -1, t);
+  int slot = 0;
+  for (int t = 0; t < n; ++t) {
+    wait_ring<R>(nld);
+    const auto d = read(slot);
+    comp(row(t), d);
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this slot has been read
+    fetch(t + R < n ? row(t + R) : -1, slot);
+    slot = (slot + 1 == R) ? 0 : slot + 1;
+  }
+  wait_vm<0>();  // nothing lands in the ring after the pass
+}
+// Ring reads.  The compiler cannot tell ring slots apart, so a plain LDS
+// load would make it wait for every DMA in flight (vmcnt(0)) and serialise
+// the ring; these reads are issued from inline assembly together with their
+// own lgkmcnt(0), after the explicit vmcnt wait above has ordered them.
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+__device__ __forceinline__ double2 d2(v2d v) { return make_double2(v.x, v.y); }
+// three 16-byte cells at a16 + 0, 1024, 2048
+__device__ __forceinline__ void ring_rd_a(uint32_t a16, v2d& x0, v2d& x1, v2d& x2) {
+  asm volatile(
+      "ds_read_b128 %0, %3\n\t"
+      "ds_read_b128 %1, %3 offset:1024\n\t"
+      "ds_read_b128 %2, %3 offset:2048\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(x0), "=&v"(x1), "=&v"(x2)
+      : "v"(a16)
+      : "memory");
+}
+// one 16-byte cell at a16 and two 4-byte cells at a4 + 1024, 1280
+__device__ __forceinline__ void ring_rd_b(uint32_t a16, uint32_t a4, v2d& x0, int& lo, int& hi) {
+  asm volatile(
+      "ds_read_b128 %0, %3\n\t"
+      "ds_read_b32 %1, %4 offset:1024\n\t"
+      "ds_read_b32 %2, %4 offset:1280\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(x0), "=&v"(lo), "=&v"(hi)
+      : "v"(a16), "v"(a4)
+      : "memory");
+}
+// four 16-byte cells at a16 + 0 .. 3072 and two 4-byte cells at a4 + 4096, 4352
+__device__ __forceinline__ void ring_rd_c(uint32_t a16, uint32_t a4, v2d& x0, v2d& x1, v2d& x2, v2d& x3,
+                                          int& lo, int& hi) {
+  asm volatile(
+      "ds_read_b128 %0, %6\n\t"
+      "ds_read_b128 %1, %6 offset:1024\n\t"
+      "ds_read_b128 %2, %6 offset:2048\n\t"
+      "ds_read_b128 %3, %6 offset:3072\n\t"
+      "ds_read_b32 %4, %7 offset:4096\n\t"
+      "ds_read_b32 %5, %7 offset:4352\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(x0), "=&v"(x1), "=&v"(x2), "=&v"(x3), "=&v"(lo), "=&v"(hi)
+      : "v"(a16), "v"(a4)
+      : "memory");
+}
+
+// ring depths and LDS bytes of the DMA variant (one wavefront per workgroup)
+constexpr int SOLVE_RA = 12, SOLVE_RB = 21, SOLVE_RC = 8;
+constexpr int SOLVE_SA = 3072, SOLVE_SB = 1536, SOLVE_SC = 4608;   // slot bytes
+constexpr int SOLVE_RING_BYTES = 36864;
+static_assert(SOLVE_RA * SOLVE_SA <= SOLVE_RING_BYTES && SOLVE_RB * SOLVE_SB <= SOLVE_RING_BYTES &&
+              SOLVE_RC * SOLVE_SC <= SOLVE_RING_BYTES, "solve ring exceeds its LDS");
+
+template <int BL, bool DMA>
 __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
+  extern __shared__ __align__(16) char solve_ring[];
+  const uint32_t ring0 = lds_addr(solve_ring);
   const int lane = threadIdx.x;
   const int sub = lane & 15, g5 = sub / 3, c = sub - 3 * g5;
   const long nm = (long)a.nxh * a.nz;
@@ -495,7 +602,7 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
   if (mean && a.fix_mass == 1) {
     double tmp = 0.0;
 #pragma unroll 8
-    for (int j = 1; j < ny; ++j) tmp = tmp + ops.dyh[j] * (-ldo(NL, OM(j)).x);
+    for (int j = 1; j < ny; ++j) tmp = tmp + ldro(ops.dyh, j) * (-ldo(NL, OM(j)).x);
     p_grad = -tmp + 0.5 * (a.dudy[ny - 1] - a.dudy[0]) / re;
     if (own) a.sc->p_grad = p_grad;
   }
@@ -515,8 +622,8 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     return d;
   };
   auto compA = [&](int j, const RowA3& d) {
-    const double h0 = d2h[3 * j], h1 = d2h[3 * j + 1], h2 = d2h[3 * j + 2];
-    const double n0 = d2n[3 * j], n1 = d2n[3 * j + 1], n2 = d2n[3 * j + 2];
+    const double h0 = ldro(d2h, 3 * j), h1 = ldro(d2h, 3 * j + 1), h2 = ldro(d2h, 3 * j + 2);
+    const double n0 = ldro(d2n, 3 * j), n1 = ldro(d2n, 3 * j + 1), n2 = ldro(d2n, 3 * j + 2);
     const double a0 = cv1 ? n0 : h0, a1 = cv1 ? n1 : h1, a2 = cv1 ? n2 : h2;
     double2 v = cadd(Rc, rmul(coef, neg(d.n)));
     if (first) {
@@ -526,7 +633,7 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
       if (j < jt) v = ve;
       if (!mean) {  // subtract_pressure_gradient(cv, cp, rk_press) (:54-77)
         if (cv1) {
-          if (j <= ny - 2) v = csub(v, rmul(rk_press * ops.dn[j], csub(d.p, Pc)));
+          if (j <= ny - 2) v = csub(v, rmul(rk_press * ldro(ops.dn, j), csub(d.p, Pc)));
         } else {
           v = csub(v, imul(rk_press * kc, Pc));
         }
@@ -547,7 +654,27 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     if (valid) { sto(SX, OC(j), f); sto1(SC, OC(j) >> 1, cp); }
     Rm = Rc; Rc = d.r; Pc = d.p;
   };
-  rows_up<BL>(1, ny - 1, loadA, compA);
+  if constexpr (DMA) {
+    auto fetchA = [&](int j, int slot) {
+      char* sb = solve_ring + slot * SOLVE_SA;
+      const uint32_t on = j < 0 ? 0u : OC(j), orr = j < 0 ? 0u : OC(j + 1), op = j < 0 ? 0u : OM(j + 1);
+      dma16(reinterpret_cast<const char*>(NL) + on, sb);
+      dma16(reinterpret_cast<const char*>(R) + orr, sb + 1024);
+      if (first) dma16(reinterpret_cast<const char*>(P) + op, sb + 2048);
+    };
+    auto readA = [&](int slot) {
+      v2d x0, x1, x2;
+      ring_rd_a(ring0 + slot * SOLVE_SA + 16 * lane, x0, x1, x2);
+      RowA3 d;
+      d.n = d2(x0);
+      d.r = d2(x1);
+      d.p = first ? d2(x2) : cz();
+      return d;
+    };
+    ring_rows<SOLVE_RA>(ny - 1, first ? 3 : 2, [](int t) { return 1 + t; }, fetchA, readA, compA);
+  } else {
+    rows_up<BL>(1, ny - 1, loadA, compA);
+  }
   // row ny: Dirichlet row of u and w; v has no row ny (its value there is zero)
   if (valid) { sto(SX, OC(ny), cv1 ? cz() : topc); sto1(SC, OC(ny) >> 1, 0.0); }
 
@@ -557,15 +684,16 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
   double ap = 0.0;
   double2 fp = cz();
   auto poisson_elim = [&](int i, double2 rhs) {  // reversed Thomas, row i (descending)
-    const double* dh = ops.dh; const double* dn = ops.dn;
+    auto dh = [&](int k) { return ldro(ops.dh, k); };
+    auto dn = [&](int k) { return ldro(ops.dn, k); };
     double a_, b_, c_;
-    if (i == 0) { a_ = 0.0; b_ = -dn[0]; c_ = dn[0]; }
-    else if (i == 1) { a_ = dh[1] * dn[0]; b_ = -dh[1] * dn[0] - dh[1] * dn[1] + alp; c_ = dh[1] * dn[1]; }
+    if (i == 0) { a_ = 0.0; b_ = -dn(0); c_ = dn(0); }
+    else if (i == 1) { a_ = dh(1) * dn(0); b_ = -dh(1) * dn(0) - dh(1) * dn(1) + alp; c_ = dh(1) * dn(1); }
     else if (i == ny - 1) {
-      a_ = dh[ny - 1] * dn[ny - 2]; b_ = -dh[ny - 1] * dn[ny - 2] - dh[ny - 1] * dn[ny - 1] + alp;
-      c_ = dh[ny - 1] * dn[ny - 1];
-    } else if (i == ny) { a_ = -dn[ny - 1]; b_ = dn[ny - 1]; c_ = 0.0; }
-    else { a_ = d2h[3 * i]; b_ = d2h[3 * i + 1] + alp; c_ = d2h[3 * i + 2]; }
+      a_ = dh(ny - 1) * dn(ny - 2); b_ = -dh(ny - 1) * dn(ny - 2) - dh(ny - 1) * dn(ny - 1) + alp;
+      c_ = dh(ny - 1) * dn(ny - 1);
+    } else if (i == ny) { a_ = -dn(ny - 1); b_ = dn(ny - 1); c_ = 0.0; }
+    else { a_ = ldro(d2h, 3 * i); b_ = ldro(d2h, 3 * i + 1) + alp; c_ = ldro(d2h, 3 * i + 2); }
     const double bp2 = (i == ny) ? b_ : b_ - c_ * ap;
     const double2 gg = (i == ny) ? rhs : csub(rhs, rmul(c_, fp));
     const double rb2 = 1.0 / bp2;
@@ -588,7 +716,7 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     // divergence of the provisional velocity at row i = j+1 (strip_divergence rhs, :28-31)
     const int i = j + 1;
     double2 t;
-    if (cv1) t = rmul(ops.dh[i <= ny - 1 ? i : ny - 1], csub(x1, x));
+    if (cv1) t = rmul(ldro(ops.dh, i <= ny - 1 ? i : ny - 1), csub(x1, x));
     else t = imul(kc, x1);
     const double2 t0 = shfl2(t, gbase), t1 = shfl2(t, gbase + 1), t2 = shfl2(t, gbase + 2);
     if (!mean) {
@@ -598,7 +726,28 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     }
     x1 = x;
   };
-  rows_down<BL>(ny - 1, 0, loadB, compB);
+  if constexpr (DMA) {
+    auto fetchB = [&](int j, int slot) {
+      char* sb = solve_ring + slot * SOLVE_SB;
+      const uint32_t o = j < 0 ? 0u : OC(j);
+      dma16(reinterpret_cast<const char*>(SX) + o, sb);
+      dma4(reinterpret_cast<const char*>(SC) + (o >> 1), sb + 1024);
+      dma4(reinterpret_cast<const char*>(SC) + (o >> 1) + 4, sb + 1280);
+    };
+    auto readB = [&](int slot) {
+      v2d x0;
+      int lo, hi;
+      const uint32_t sb = ring0 + slot * SOLVE_SB;
+      ring_rd_b(sb + 16 * lane, sb + 4 * lane, x0, lo, hi);
+      RowB3 d;
+      d.s = d2(x0);
+      d.cc = __hiloint2double(hi, lo);
+      return d;
+    };
+    ring_rows<SOLVE_RB>(ny, 3, [ny](int t) { return ny - 1 - t; }, fetchB, readB, compB);
+  } else {
+    rows_down<BL>(ny - 1, 0, loadB, compB);
+  }
   if (!mean) poisson_elim(0, cz());
 
   // ---------------- pass C: pressure correction, projection, next RK register ----------------
@@ -613,18 +762,18 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
   // D2H/D2N stencil rows s0..s2, P_new at j, j+1 and NL at j)
   auto buildR = [&](int j, double2 cvj, double2 s0, double2 s1, double2 s2, double2 Pj, double2 Pj1, double2 nj) {
     double2 out = cadd(cvj, rmul(coef2, neg(nj)));
-    const double h0 = d2h[3 * j], h1 = d2h[3 * j + 1], h2 = d2h[3 * j + 2];
+    const double h0 = ldro(d2h, 3 * j), h1 = ldro(d2h, 3 * j + 1), h2 = ldro(d2h, 3 * j + 2);
     if (!cv1) {
       double2 dd = cadd(cadd(rmul(h0, s0), rmul(h1, s1)), rmul(h2, s2));
       out = cadd(out, rmul(vis_exp2, csub(dd, rmul(k2, cvj))));
     } else if (j < ny) {
-      const double b0 = d2n[3 * j], b1 = d2n[3 * j + 1], b2 = d2n[3 * j + 2];
+      const double b0 = ldro(d2n, 3 * j), b1 = ldro(d2n, 3 * j + 1), b2 = ldro(d2n, 3 * j + 2);
       double2 dd = cadd(cadd(rmul(b0, s0), rmul(b1, s1)), rmul(b2, s2));
       out = cadd(out, rmul(vis_exp2, csub(dd, rmul(k2, cvj))));
     }
     if (!mean) {
       if (cv1) {
-        if (j >= 1 && j <= ny - 2) out = csub(out, rmul(rk_press2 * ops.dn[j], csub(Pj1, Pj)));
+        if (j >= 1 && j <= ny - 2) out = csub(out, rmul(rk_press2 * ldro(ops.dn, j), csub(Pj1, Pj)));
       } else if (j >= 1 && j <= ny - 1) {
         out = csub(out, imul(rk_press2 * kc, Pj));
       }
@@ -639,11 +788,11 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     double2 pn = cz();
     if (!mean) {
       if (cv1) {
-        if (i >= 1 && i <= ny - 2) cvv = csub(cvv, rmul(rk_press * ops.dn[i], csub(fn, f0)));
+        if (i >= 1 && i <= ny - 2) cvv = csub(cvv, rmul(rk_press * ldro(ops.dn, i), csub(fn, f0)));
       } else if (i >= 1 && i <= ny - 1) {
         cvv = csub(cvv, imul(rk_press * kc, f0));
       }
-      double2 d2 = cadd(cadd(rmul(d2h[3 * i], st0), rmul(d2h[3 * i + 1], st1)), rmul(d2h[3 * i + 2], st2));
+      double2 d2 = cadd(cadd(rmul(ldro(d2h, 3 * i), st0), rmul(ldro(d2h, 3 * i + 1), st1)), rmul(ldro(d2h, 3 * i + 2), st2));
       pn = csub(cadd(Pold, rmul(alpha2, f0)), rmul(vis_imp, d2));
     }
     if (valid) sto(U, OC(i), cvv);
@@ -651,9 +800,9 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     w0 = w1; w1 = w2; w2 = w3; w3 = cvv;
     Pw0 = Pw1; Pw1 = pn;
     if (mean && c == 0) {  // d_um_dy update from the new mean profile (:84-89)
-      if (i >= 2 && i <= ny - 1) a.dudy[i - 1] = (ops.dn[i - 1] * (w3.x - w2.x));
-      if (i == 2) a.dudy[0] = dm[0] * w1.x + dm[1] * w2.x + dm[2] * w3.x;
-      if (i == ny) a.dudy[ny - 1] = dm[3] * w1.x + dm[4] * w2.x + dm[5] * w3.x;
+      if (i >= 2 && i <= ny - 1) a.dudy[i - 1] = (ldro(ops.dn, i - 1) * (w3.x - w2.x));
+      if (i == 2) a.dudy[0] = ldro(dm, 0) * w1.x + ldro(dm, 1) * w2.x + ldro(dm, 2) * w3.x;
+      if (i == ny) a.dudy[ny - 1] = ldro(dm, 3) * w1.x + ldro(dm, 4) * w2.x + ldro(dm, 5) * w3.x;
     }
     if (!more) {
       if (valid) sto(R, OC(i), cvv);
@@ -673,8 +822,8 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
   const double2 ph0 = lsp(0);
   const double2 ph1 = csub(lsp(1), rmul(lsa(1), ph0));
   const double2 ph2 = csub(lsp(2), rmul(lsa(2), ph1));
-  const double2 pb = make_double2((0.0 - dm[1] * ph1.x - dm[2] * ph2.x) / dm[0],
-                                  (0.0 - dm[1] * ph1.y - dm[2] * ph2.y) / dm[0]);
+  const double2 pb = make_double2((0.0 - ldro(dm, 1) * ph1.x - ldro(dm, 2) * ph2.x) / ldro(dm, 0),
+                                  (0.0 - ldro(dm, 1) * ph1.y - ldro(dm, 2) * ph2.y) / ldro(dm, 0));
   c2row(0, ldo(SX, OC(0)), ldo(P, OM(0)), pb, cz(), pb, ph1, ph2, cz());
   c2row(1, ldo(SX, OC(1)), ldo(P, OM(1)), ph1, ph2, pb, ph1, ph2, cz());
   double2 q0 = pb, q1 = ph1, q2 = ph2, ph = ph2;
@@ -691,10 +840,39 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
     q0 = q1; q1 = q2; q2 = ph;
     c2row(i - 1, d.sx, d.p, q1, q2, q0, q1, q2, d.n);
   };
-  rows_up<BL>(3, ny - 1, loadC, compC);
+  if constexpr (DMA) {
+    auto fetchC = [&](int i, int slot) {
+      char* sb = solve_ring + slot * SOLVE_SC;
+      const bool dmy = i < 0;
+      const uint32_t om = dmy ? 0u : OM(i), ox = dmy ? 0u : OC(i - 1), opp = dmy ? 0u : OM(i - 1),
+                     on = dmy ? 0u : OC(i - 2);
+      dma16(reinterpret_cast<const char*>(SP) + om, sb);
+      dma16(reinterpret_cast<const char*>(SX) + ox, sb + 1024);
+      dma16(reinterpret_cast<const char*>(P) + opp, sb + 2048);
+      if (more) dma16(reinterpret_cast<const char*>(NL) + on, sb + 3072);
+      dma4(reinterpret_cast<const char*>(SA) + (om >> 1), sb + 4096);
+      dma4(reinterpret_cast<const char*>(SA) + (om >> 1) + 4, sb + 4352);
+    };
+    auto readC = [&](int slot) {
+      v2d x0, x1, x2, x3;
+      int lo, hi;
+      const uint32_t sb = ring0 + slot * SOLVE_SC;
+      ring_rd_c(sb + 16 * lane, sb + 4 * lane, x0, x1, x2, x3, lo, hi);
+      RowC3 d;
+      d.sp = mean ? cz() : d2(x0);
+      d.sx = d2(x1);
+      d.p = d2(x2);
+      d.n = more ? d2(x3) : cz();
+      d.sa = mean ? 0.0 : __hiloint2double(hi, lo);
+      return d;
+    };
+    ring_rows<SOLVE_RC>(ny - 3, more ? 6 : 5, [](int t) { return 3 + t; }, fetchC, readC, compC);
+  } else {
+    rows_up<BL>(3, ny - 1, loadC, compC);
+  }
   // top wall: phi(ny) from phi(ny-2), phi(ny-1)
-  const double2 pt = make_double2((0.0 - dm[3] * q1.x - dm[4] * q2.x) / dm[5],
-                                  (0.0 - dm[3] * q1.y - dm[4] * q2.y) / dm[5]);
+  const double2 pt = make_double2((0.0 - ldro(dm, 3) * q1.x - ldro(dm, 4) * q2.x) / ldro(dm, 5),
+                                  (0.0 - ldro(dm, 3) * q1.y - ldro(dm, 4) * q2.y) / ldro(dm, 5));
   c2row(ny - 1, ldo(SX, OC(ny - 1)), ldo(P, OM(ny - 1)), q2, pt, q1, q2, pt, more ? ldo(NL, OC(ny - 2)) : cz());
   c2row(ny, ldo(SX, OC(ny)), ldo(P, OM(ny)), pt, cz(), q1, q2, pt, more ? ldo(NL, OC(ny - 1)) : cz());
   if (more) buildR(ny, w3, w1, w2, w3, Pw1, cz(), ldo(NL, OC(ny)));
@@ -707,9 +885,10 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   // the component-parallel kernel needs the byte offsets of one array to fit 32 bits
   if (lanes == 3 && 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9) {
     dim3 g3((unsigned)((nm + 19) / 20));
-    if (bl == 4) hipLaunchKernelGGL(k_solve3<4>, g3, dim3(64), 0, s, ops, a);
-    else if (bl == 16) hipLaunchKernelGGL(k_solve3<16>, g3, dim3(64), 0, s, ops, a);
-    else hipLaunchKernelGGL(k_solve3<8>, g3, dim3(64), 0, s, ops, a);
+    if (a.dma) hipLaunchKernelGGL((k_solve3<8, true>), g3, dim3(64), SOLVE_RING_BYTES, s, ops, a);
+    else if (bl == 4) hipLaunchKernelGGL((k_solve3<4, false>), g3, dim3(64), 0, s, ops, a);
+    else if (bl == 16) hipLaunchKernelGGL((k_solve3<16, false>), g3, dim3(64), 0, s, ops, a);
+    else hipLaunchKernelGGL((k_solve3<8, false>), g3, dim3(64), 0, s, ops, a);
     LAUNCH_CHECK("k_solve3");
     return;
   }
@@ -855,14 +1034,17 @@ void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* i
 // ---------------------------------------------------------------------------
 // CFL number and dt adaptation (check_cfl PFC/pass1.f90:202-250, :106-124)
 // ---------------------------------------------------------------------------
-__global__ void k_cfl_finalize(DevOps ops, chan_params prm, const unsigned long long* cfl,
-                               DevScalars* sc, int step) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(64) void k_cfl_finalize(DevOps ops, chan_params prm,
+                                                     const unsigned long long* cfl, DevScalars* sc,
+                                                     int step) {
+  // one wavefront: lane l takes the planes k = 2 + l, 2 + l + 64, ...; the
+  // maximum and the abort flag do not depend on the order of the planes
+  const int lane = threadIdx.x;
   const int ny = prm.ny;
   const double dt = sc->dt;
-  double cmax = 0.0;
+  double cmax = 0.0;   // every plane's CFL number is >= 0
   int err = 0;
-  for (int k = 2; k <= ny; ++k) {
+  for (int k = 2 + lane; k <= ny; k += 64) {
     const int kk = k - 1;
     double amax = __longlong_as_double((long long)cfl[3 * kk + 0]);
     double vmax = __longlong_as_double((long long)cfl[3 * kk + 1]);
@@ -871,10 +1053,16 @@ __global__ void k_cfl_finalize(DevOps ops, chan_params prm, const unsigned long 
     double tempb = vmax / (1.73 * fmin(ops.hn[k - 1], ops.hn[k - 2]));
     double tempc = 0.5 * wmax * (double)prm.nz / (0.55 * prm.lz);
     double c = (tempa + tempb + tempc) * dt;
-    if (prm.dt_flag == 1) { if (c >= 3.0e2 && !err) err = 2; }
-    else if (c >= 1.15 && !err) err = 1;
-    if (k == 2 || c > cmax) cmax = c;
+    if (prm.dt_flag == 1) { if (c >= 3.0e2) err = 2; }
+    else if (c >= 1.15) err = 1;
+    if (!(c <= cmax)) cmax = c;   // a NaN plane makes the maximum NaN
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double oc = __shfl_xor(cmax, o, 64);
+    if (!(oc <= cmax)) cmax = oc;
+    err = max(err, __shfl_xor(err, o, 64));
+  }
+  if (lane != 0) return;
   sc->cfl = cmax;
   if (err && !sc->err) sc->err = err;
   if (step >= 0 && prm.dt_flag == 1) {
