@@ -98,7 +98,7 @@ struct Tuning {
   int zinv_kx = 8;        // z-inverse kx columns per workgroup (4, 8, 16)
   int zfwd_kx = 8;        // z-forward kx columns per workgroup (4, 8)
   int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
-  int solve_dma = 0;      // 3-lane solve streams its rows through an LDS ring
+  int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
