@@ -145,7 +145,8 @@ def test_rhs_solve_single(g0, rk):
 def test_rhs_solve_variants(g0, variant):
     """Every solve kernel variant (lanes per mode, row block, LDS ring) against
     the oracle for the three substeps; the ring and the register-blocked loads
-    of the 3-lane kernel run the same arithmetic and agree bit for bit."""
+    of the 3-lane kernel run the same recurrences (the compiler may contract
+    multiply-adds differently in the two) and agree to round-off."""
     lanes, bl, dma = variant
     rng = np.random.default_rng(40)
     st = O.initial_state(g0, init_cfg0(O, g0, seed=9, amp=1e-2), dt=0.01)
@@ -172,7 +173,7 @@ def test_rhs_solve_variants(g0, variant):
         assert rel_l2(d2, O.to_fortran_data2(ref.R)) < TOL1, rk
         assert rel_l2(d4, O.to_fortran_data4(ref.P)) < 1e-11, rk
         for x, y in zip(outs[0], outs[-1]):
-            assert np.array_equal(x, y), rk
+            assert rel_l2(x, y) <= 1e-14, rk
 
 
 def test_initialize_flow(g0):
