@@ -55,6 +55,8 @@ ENTRY_POINTS = (
     "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_num_params", "chan_mlp_set_params",
     "chan_mlp_get_params", "chan_mlp_infer", "chan_mlp_forward_fp32", "chan_mlp_gradient", "chan_mlp_train",
     "chan_vtk_write", "chan_write_vtk",
+    "chan_les_configure", "chan_les_update", "chan_les_set_tau", "chan_les_get_tau", "chan_les_get_coefficient",
+    "chan_les_get_fields", "chan_les_features", "chan_les_gated_points",
 )
 
 _lib = None
@@ -139,6 +141,14 @@ def load_library(path: str = LIBPATH):
         "chan_mlp_train": (C.c_int, [vp, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_double, _dp]),
         "chan_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
         "chan_write_vtk": (C.c_int, [vp, C.c_char_p, _dp]),
+        "chan_les_configure": (C.c_int, [vp, C.c_int, vp, C.c_double]),
+        "chan_les_update": (C.c_int, [vp]),
+        "chan_les_set_tau": (C.c_int, [vp, _dp]),
+        "chan_les_get_tau": (C.c_int, [vp, _dp]),
+        "chan_les_get_coefficient": (C.c_int, [vp, _dp]),
+        "chan_les_get_fields": (C.c_int, [vp, _dp]),
+        "chan_les_features": (C.c_int, [vp, _fp, _dp]),
+        "chan_les_gated_points": (C.c_int, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -246,6 +256,55 @@ class Channel:
     def set_tuning(self, key, value):
         k = self.TUNING_KEYS.index(key) if isinstance(key, str) else int(key)
         self._chk(self.lib.chan_set_tuning(self.h, k, int(value)))
+
+    # -- subgrid closure (LES)
+    LES_MODES = {"off": 0, "dynamic": 1, "learned": 2, "prescribed": 3}
+
+    def les_configure(self, mode, net=None, beta=1.0):
+        """mode: 'off', 'dynamic' (dynamic Smagorinsky), 'learned' (Mlp `net`
+        with the Smagorinsky fallback gate) or 'prescribed' (les_set_tau)."""
+        m = self.LES_MODES[mode] if isinstance(mode, str) else int(mode)
+        self._les_net = net
+        self._chk(self.lib.chan_les_configure(self.h, m, net.h if net is not None else None, float(beta)))
+
+    def les_update(self):
+        self._chk(self.lib.chan_les_update(self.h))
+
+    def _phys_shape(self, nf):
+        return (nf, self.nyp, self.nz2, self.nx2)
+
+    def les_set_tau(self, tau):
+        t = _f64(tau)
+        if t.shape != self._phys_shape(6):
+            raise ValueError(f"tau must have shape {self._phys_shape(6)}")
+        self._chk(self.lib.chan_les_set_tau(self.h, _ptr(t)))
+
+    def les_tau(self):
+        t = np.empty(self._phys_shape(6))
+        self._chk(self.lib.chan_les_get_tau(self.h, _ptr(t)))
+        return t
+
+    def les_coefficient(self):
+        c = np.empty(self.nyp)
+        self._chk(self.lib.chan_les_get_coefficient(self.h, _ptr(c)))
+        return c
+
+    def les_fields(self):
+        f = np.empty(self._phys_shape(12))
+        self._chk(self.lib.chan_les_get_fields(self.h, _ptr(f)))
+        return f
+
+    def les_features(self):
+        n = (self.ny - 1) * self.nz2 * self.nx2
+        x = np.empty((n, 10), np.float32)
+        sc = np.empty(n)
+        self._chk(self.lib.chan_les_features(self.h, x.ctypes.data_as(_fp), _ptr(sc)))
+        return x, sc
+
+    def les_gated_points(self):
+        a, b = C.c_long(), C.c_long()
+        self._chk(self.lib.chan_les_gated_points(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def get_tuning(self):
         out = {}

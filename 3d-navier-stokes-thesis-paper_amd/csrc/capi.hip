@@ -164,6 +164,11 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
   const long zps = (long)g.nz2 * g.kxpad;
   if (lin && !c->d_base) throw chan_error("base flow not set (chan_set_baseflow)");
   if (cfl) launch_fill_zero(c->d_cfl, sizeof(unsigned long long) * 3 * nyp, c->stream);
+  const double* tau = nullptr;
+  if (!lin && c->les) {
+    KTimer t(c, KT_OTHER);
+    tau = les_tau_for_pass(c);
+  }
   {
     KTimer t(c, KT_ZINV);
     launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream, c->tune.zinv_kx);
@@ -175,6 +180,7 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     a.cflbits = cfl ? c->d_cfl : nullptr;
     a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->xpass_chunk; a.rz = 0;
     a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads; a.prefetch = c->tune.xpass_prefetch;
+    a.tau = tau;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
   }
   {
@@ -325,6 +331,10 @@ void prepare_capture(chan_ctx* c) {
 }
 
 }  // namespace
+
+void ctx_set_error(chan_ctx* c, const std::string& m) { fail(c, m); }
+void check_ops_les(chan_ctx* c) { check_ops(c); }
+void les_prepare_state(chan_ctx* c) { ensure_vel_in_vel(c); }
 
 // ---------------------------------------------------------------------------
 // Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
@@ -541,6 +551,7 @@ int chan_destroy(chan_ctx* c) {
   dfree(c->d_tw); dfree(c->d_ops); dfree(c->d_vel); dfree(c->d_nl); dfree(c->d_R); dfree(c->d_P);
   dfree(c->d_cbc); dfree(c->d_dudy); dfree(c->d_sc); dfree(c->d_cfl); dfree(c->d_Z); dfree(c->d_AB);
   dfree(c->d_scr); dfree(c->d_base);
+  les_destroy(c->les);
   if (c->d_stage) (void)hipFree(c->d_stage);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -1248,6 +1259,7 @@ int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int line
   try {
     for (int i = 0; i < n; ++i) {
       if (!members[i]) throw chan_error("null ensemble member");
+      if (members[i]->les) throw chan_error("ensemble members run without a subgrid closure");
       for (int k = 0; k < i; ++k)
         if (members[k] == members[i]) throw chan_error("ensemble member given twice");
       e->m.push_back(members[i]);

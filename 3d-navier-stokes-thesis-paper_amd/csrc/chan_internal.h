@@ -84,6 +84,8 @@ struct XPassArgs {
   int wave;               // 1: wave-per-row kernel where the length allows it
   int maxthr;             // workgroup-kernel threads bound (64..640)
   int prefetch;           // 1: direct global-to-LDS prefetch of the next plane's rows
+  const double* tau;      // subgrid stress [6][ny+1][nz2][nx2] at the cell centres yh
+                          // (11, 12, 13, 22, 23, 33; zero on the wall planes) or nullptr
 };
 
 // Launch-shape choices of one context (chan_set_tuning, chan_autotune);
@@ -103,6 +105,8 @@ struct Tuning {
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
 int tuning_get(const Tuning& t, int key);
+
+struct LesState;
 
 struct chan_ctx {
   chan_params prm;
@@ -141,6 +145,7 @@ struct chan_ctx {
   void* h_pin = nullptr;         // pinned host staging (hipHostMalloc)
   size_t pin_bytes = 0;
   bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
+  LesState* les = nullptr;       // subgrid closure (chan_les_configure)
   // options
   int xpass_chunk = 8;
   Tuning tune;
@@ -217,3 +222,30 @@ void launch_ctrl_law(int npts, const double* p2, const double* p3, double ca, do
 void launch_absmax(long n, const double* x, unsigned long long* out, hipStream_t s);
 void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, const double* dyh,
                           const unsigned long long* vmax, double eps, int dir, hipStream_t s);
+
+// Error capture shared by the entry points of every translation unit: the
+// message goes to the context (chan_last_error) and the status is 1.
+void ctx_set_error(chan_ctx* c, const std::string& m);
+template <class F>
+int guarded_les(chan_ctx* c, F&& f) {
+  if (!c) return 1;
+  try {
+    f();
+    return 0;
+  } catch (const chan_error& e) {
+    ctx_set_error(c, e.msg);
+  } catch (const std::exception& e) {
+    ctx_set_error(c, e.what());
+  }
+  return 1;
+}
+void check_ops_les(chan_ctx* c);        // operators uploaded, else throws
+void les_prepare_state(chan_ctx* c);    // the velocity (not the NL term) is in d_vel
+
+// subgrid closure (k_les.hip)
+void les_destroy(LesState* s);
+const double* les_tau_for_pass(chan_ctx* c);  // updates and returns tau, nullptr when off
+
+// learned closure network (k_mlp.hip)
+void mlp_prepare_device(chan_mlp* m);   // packs the bf16 weights, synchronised
+void mlp_infer_device(chan_mlp* m, long n, const float* x, float* y, hipStream_t s);

@@ -1,0 +1,536 @@
+// Subgrid-scale closure for large-eddy simulation on the channel grid (no
+// counterpart in the reference: it carries only the switches `subgrid`,
+// `filter_dim`, `lagr_avg` (PFC/flow.dat:19-22) and commented-out calls of a
+// dynamic model, PFC/main_mpi.f90:10-13,74,77-78).
+//
+// The closure gives the subgrid stress tau_ij at the cell centres yh of the
+// dealiased physical grid; the x-pass adds it to the flux u_i u_j wherever the
+// flux is formed, so the nonlinear term becomes div(u u + tau) with the
+// reference's own discrete operators.
+//
+// Fields of one update, all at the cell centres of planes 1..ny-1 (zero on the
+// wall planes 0 and ny):
+//   G[0..8]  = du_i/dx_j  (i = u, v, w; j = x, y, z), G[9..11] = u, v, w
+//   d/dx, d/dz spectrally (i kx, i kz); v at yh is the mean of its two faces;
+//   du/dy, dw/dy by the central difference over yh(k-1), yh(k+1); dv/dy by the
+//   face difference dh(k) (v(k) - v(k-1)).
+// Models:
+//   1  dynamic Smagorinsky: tau_ij = -2 C(y) D^2 |S| S_ij, |S| = sqrt(2 S:S),
+//      D = (dx dz (y(k)-y(k-1)))^(1/3), C(y) from the Germano identity with a
+//      sharp test filter at half the resolved x and z wavenumbers (width ratio
+//      2), least squares over each plane (Lilly), clipped at zero;
+//   2  learned: an MLP maps (G[0..8]/|G|, 1-|yh|) to tau_ij / (D^2 |G|^2);
+//      a point whose learned stress removes energy at a rate below -beta times
+//      the dynamic Smagorinsky rate, or is not finite, takes the Smagorinsky
+//      stress instead (the fallback gate);
+//   3  prescribed tau (chan_les_set_tau).
+#include "chan_internal.h"
+#include <cmath>
+
+namespace {
+
+constexpr int LES_NF = 12;  // fields of G
+
+// symmetric-tensor component order 11, 12, 13, 22, 23, 33 and its (i, j) pairs
+__device__ __constant__ int c_ti[6] = {0, 0, 0, 1, 1, 2};
+__device__ __constant__ int c_tj[6] = {0, 1, 2, 1, 2, 2};
+
+__device__ __forceinline__ double2 ikm(double k, double2 a) { return make_double2(-k * a.y, k * a.x); }
+
+__global__ void k_les_grad(DevOps ops, int ny, int nxh, int nz, const double2* __restrict__ vel,
+                           double2* __restrict__ G) {
+  const long nm = (long)nxh * nz;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = blockIdx.y, nyp = ny + 1;
+  if (m >= nm) return;
+  auto out = [&](int f, double2 v) { G[((long)f * nyp + kk) * nm + m] = v; };
+  if (kk == 0 || kk == ny) {
+    for (int f = 0; f < LES_NF; ++f) out(f, make_double2(0.0, 0.0));
+    return;
+  }
+  const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
+  const double kx = ops.kx[ix], kz = ops.kz[iz];
+  auto V = [&](int c, int k) { return vel[((long)c * nyp + k) * nm + m]; };
+  const double2 u = V(0, kk), w = V(2, kk), vm = V(1, kk - 1), v0 = V(1, kk);
+  const double2 vc = make_double2(0.5 * (vm.x + v0.x), 0.5 * (vm.y + v0.y));
+  const double dyc = 1.0 / (ops.hn[kk] + ops.hn[kk - 1]);
+  const double dhk = ops.dh[kk];
+  const double2 up = V(0, kk + 1), um = V(0, kk - 1), wp = V(2, kk + 1), wm = V(2, kk - 1);
+  out(0, ikm(kx, u));
+  out(1, make_double2((up.x - um.x) * dyc, (up.y - um.y) * dyc));
+  out(2, ikm(kz, u));
+  out(3, ikm(kx, vc));
+  out(4, make_double2(dhk * (v0.x - vm.x), dhk * (v0.y - vm.y)));
+  out(5, ikm(kz, vc));
+  out(6, ikm(kx, w));
+  out(7, make_double2((wp.x - wm.x) * dyc, (wp.y - wm.y) * dyc));
+  out(8, ikm(kz, w));
+  out(9, u);
+  out(10, vc);
+  out(11, w);
+}
+
+// sharp test filter: keeps kx index < kxc and |kz index| < kzc; src -> dst
+__global__ void k_les_mask(const double2* __restrict__ src, double2* __restrict__ dst, int nxh, int nz, int kxc,
+                           int kzc) {
+  const long nm = (long)nxh * nz;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const long o = (long)blockIdx.y * nm + m;
+  const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
+  const int kzi = iz <= nz / 2 ? iz : iz - nz;
+  const bool keep = ix < kxc && (kzi < 0 ? -kzi : kzi) < kzc;
+  dst[o] = keep ? src[o] : make_double2(0.0, 0.0);
+}
+
+struct Strain {
+  double s[6];  // S_ij in the symmetric order
+  double mag;   // |S| = sqrt(2 S:S)
+};
+__device__ __forceinline__ Strain strain(const double g[9]) {
+  Strain r;
+  r.s[0] = g[0];
+  r.s[1] = 0.5 * (g[1] + g[3]);
+  r.s[2] = 0.5 * (g[2] + g[6]);
+  r.s[3] = g[4];
+  r.s[4] = 0.5 * (g[5] + g[7]);
+  r.s[5] = g[8];
+  const double ss = r.s[0] * r.s[0] + r.s[3] * r.s[3] + r.s[5] * r.s[5] +
+                    2.0 * (r.s[1] * r.s[1] + r.s[2] * r.s[2] + r.s[4] * r.s[4]);
+  r.mag = sqrt(2.0 * ss);
+  return r;
+}
+// sum_ij a_ij b_ij of two symmetric tensors in the symmetric order
+__device__ __forceinline__ double contract(const double* a, const double* b) {
+  return a[0] * b[0] + a[3] * b[3] + a[5] * b[5] + 2.0 * (a[1] * b[1] + a[2] * b[2] + a[4] * b[4]);
+}
+
+// products for the Germano identity: P[0..5] = u_i u_j, P[6..11] = |S| S_ij
+__global__ void k_les_prod(int ny, long ps, const double* __restrict__ Ph, double* __restrict__ P) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = blockIdx.y, nyp = ny + 1;
+  if (i >= ps) return;
+  auto at = [&](const double* b, int f) { return b[((long)f * nyp + kk) * ps + i]; };
+  double* o[12];
+  for (int f = 0; f < 12; ++f) o[f] = P + ((long)f * nyp + kk) * ps + i;
+  if (kk == 0 || kk == ny) {
+    for (int f = 0; f < 12; ++f) *o[f] = 0.0;
+    return;
+  }
+  double g[9], u[3];
+  for (int f = 0; f < 9; ++f) g[f] = at(Ph, f);
+  for (int c = 0; c < 3; ++c) u[c] = at(Ph, 9 + c);
+  const Strain S = strain(g);
+  for (int t = 0; t < 6; ++t) {
+    *o[t] = u[c_ti[t]] * u[c_tj[t]];
+    *o[6 + t] = S.mag * S.s[t];
+  }
+}
+
+// Germano identity per point, partial plane sums of L:M and M:M per block
+// (T = test-filtered fields: T[0..11] = T(P), T[12..23] = T(G))
+__global__ __launch_bounds__(256) void k_les_germano(int ny, long ps, const double* __restrict__ T,
+                                                     const double* __restrict__ delta2, double* __restrict__ part) {
+  const int kk = blockIdx.y, nyp = ny + 1;
+  double lm = 0.0, mm = 0.0;
+  if (kk > 0 && kk < ny) {
+    const double d2 = delta2[kk];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ps; i += (long)gridDim.x * blockDim.x) {
+      auto at = [&](int f) { return T[((long)f * nyp + kk) * ps + i]; };
+      double g[9], u[3], L[6], M[6];
+      for (int f = 0; f < 9; ++f) g[f] = at(12 + f);
+      for (int c = 0; c < 3; ++c) u[c] = at(21 + c);
+      const Strain S = strain(g);
+      for (int t = 0; t < 6; ++t) {
+        L[t] = at(t) - u[c_ti[t]] * u[c_tj[t]];
+        M[t] = 2.0 * d2 * (at(6 + t) - 4.0 * S.mag * S.s[t]);
+      }
+      lm += contract(L, M);
+      mm += contract(M, M);
+    }
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = lm;
+  red[1][threadIdx.x] = mm;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[((long)kk * gridDim.x + blockIdx.x) * 2 + 0] = red[0][0];
+    part[((long)kk * gridDim.x + blockIdx.x) * 2 + 1] = red[1][0];
+  }
+}
+
+__global__ void k_les_cs2(int ny, int nblk, const double* __restrict__ part, double* __restrict__ cs2) {
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk > ny) return;
+  double lm = 0.0, mm = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    lm += part[((long)kk * nblk + b) * 2 + 0];
+    mm += part[((long)kk * nblk + b) * 2 + 1];
+  }
+  double c = (mm > 0.0) ? lm / mm : 0.0;
+  cs2[kk] = (kk == 0 || kk == ny || !(c > 0.0)) ? 0.0 : c;
+}
+
+// Smagorinsky stress at the interior points (zero on the wall planes)
+__global__ void k_les_tau_smag(int ny, long ps, const double* __restrict__ Ph, const double* __restrict__ cs2,
+                               const double* __restrict__ delta2, double* __restrict__ tau) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = blockIdx.y, nyp = ny + 1;
+  if (i >= ps) return;
+  double* o = tau + (long)kk * ps + i;
+  const long cs = (long)nyp * ps;
+  if (kk == 0 || kk == ny) {
+    for (int t = 0; t < 6; ++t) o[t * cs] = 0.0;
+    return;
+  }
+  double g[9];
+  for (int f = 0; f < 9; ++f) g[f] = Ph[((long)f * nyp + kk) * ps + i];
+  const Strain S = strain(g);
+  const double c = -2.0 * cs2[kk] * delta2[kk] * S.mag;
+  for (int t = 0; t < 6; ++t) o[t * cs] = c * S.s[t];
+}
+
+// MLP inputs of the interior points p = ((k-1) nz2 + z) nx2 + x and the stress
+// scale D^2 |G|^2
+__global__ void k_les_feat(int ny, long ps, const double* __restrict__ Ph, const double* __restrict__ ydist,
+                           const double* __restrict__ delta2, float* __restrict__ X, double* __restrict__ scale) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = blockIdx.y + 1, nyp = ny + 1;
+  if (i >= ps) return;
+  double g[9], n2 = 0.0;
+  for (int f = 0; f < 9; ++f) {
+    g[f] = Ph[((long)f * nyp + kk) * ps + i];
+    n2 += g[f] * g[f];
+  }
+  const double gn = sqrt(n2);
+  const double inv = gn > 0.0 ? 1.0 / gn : 0.0;
+  const long p = (long)(kk - 1) * ps + i;
+  for (int f = 0; f < 9; ++f) X[p * 10 + f] = (float)(g[f] * inv);
+  X[p * 10 + 9] = (float)ydist[kk];
+  scale[p] = delta2[kk] * n2;
+}
+
+// learned stress with the fallback gate
+__global__ __launch_bounds__(256) void k_les_tau_mlp(int ny, long ps, const double* __restrict__ Ph,
+                                                     const float* __restrict__ Y, const double* __restrict__ scale,
+                                                     const double* __restrict__ cs2,
+                                                     const double* __restrict__ delta2, double beta,
+                                                     double* __restrict__ tau, unsigned int* __restrict__ gated) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kk = blockIdx.y + 1, nyp = ny + 1;
+  bool fb = false;
+  if (i < ps) {
+    double g[9];
+    for (int f = 0; f < 9; ++f) g[f] = Ph[((long)f * nyp + kk) * ps + i];
+    const Strain S = strain(g);
+    const long p = (long)(kk - 1) * ps + i;
+    double tm[6], ts[6];
+    const double c = -2.0 * cs2[kk] * delta2[kk] * S.mag;
+    for (int t = 0; t < 6; ++t) {
+      tm[t] = scale[p] * (double)Y[p * 6 + t];
+      ts[t] = c * S.s[t];
+    }
+    const double em = -contract(tm, S.s), es = -contract(ts, S.s);
+    fb = !(em >= -beta * es) || !isfinite(em);
+    const long cs = (long)nyp * ps;
+    double* o = tau + (long)kk * ps + i;
+    for (int t = 0; t < 6; ++t) o[t * cs] = fb ? ts[t] : tm[t];
+  }
+  const unsigned long long b = __ballot(fb);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(gated, (unsigned int)__popcll(b));
+}
+
+__global__ void k_les_zero_walls(int ny, long ps, double* __restrict__ tau) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ps) return;
+  const long cs = (long)(ny + 1) * ps;
+  for (int t = 0; t < 6; ++t) {
+    tau[t * cs + i] = 0.0;
+    tau[t * cs + (long)ny * ps + i] = 0.0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct LesState {
+  int mode = 0;
+  double beta = 1.0;
+  chan_mlp* net = nullptr;
+  int kxc = 0, kzc = 0;        // test-filter cut-offs
+  long ps = 0, nm = 0, npts = 0;
+  int nblk = 0;                // Germano partial sums per plane
+  double2* G = nullptr;        // [24][nyp][nm]: G, then its test-filtered copy
+  double2* Gt = nullptr;       // [12][nyp][nm]: transformed products
+  double2* Zl = nullptr;       // [12 nyp][nz2][kxpad] transform intermediate
+  double* Ph = nullptr;        // [12][nyp][ps]
+  double* P = nullptr;         // [12][nyp][ps]
+  double* T = nullptr;         // [24][nyp][ps]
+  double* tau = nullptr;       // [6][nyp][ps]
+  double* cs2 = nullptr;       // [nyp]
+  double* delta2 = nullptr;    // [nyp]
+  double* ydist = nullptr;     // [nyp]
+  double* part = nullptr;      // [nyp][nblk][2]
+  float* X = nullptr;          // [npts][10]
+  float* Y = nullptr;          // [npts][6]
+  double* scale = nullptr;     // [npts]
+  unsigned int* gated = nullptr;
+  bool fresh = false;          // tau is current for the state (mode 3: set by the caller)
+};
+
+namespace {
+
+template <class T>
+void lalloc(T*& p, size_t n) {
+  CHAN_CHECK(hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)));
+}
+template <class T>
+void lfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void les_c2r(chan_ctx* c, LesState* L, const double2* in, int np, double* out) {
+  const XZGeom& g = c->gd;
+  const long zps = (long)g.nz2 * g.kxpad;
+  launch_zinv(g, c->d_tw, in, L->nm, L->Zl, zps, np, c->stream);
+  launch_xc2r(g, c->d_tw, L->Zl, zps, out, L->ps, np, c->stream);
+}
+
+void les_r2c(chan_ctx* c, LesState* L, const double* in, int np, double2* out) {
+  const XZGeom& g = c->gd;
+  const long zps = (long)g.nz2 * g.kxpad;
+  launch_xr2c(g, c->d_tw, in, nullptr, c->ops.kx, L->ps, L->Zl, zps, np, c->stream);
+  launch_zfwd(g, c->d_tw, L->Zl, nullptr, zps, c->ops.kz, out, L->nm, np, 0, c->stream);
+}
+
+// G, its physical fields Ph and (models 1, 2) the dynamic coefficient
+void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
+  const int ny = c->prm.ny, nyp = c->nyp;
+  dim3 gm((unsigned)((L->nm + 255) / 256), nyp);
+  hipLaunchKernelGGL(k_les_grad, gm, dim3(256), 0, c->stream, c->ops, ny, c->nxh, c->prm.nz, c->d_vel, L->G);
+  LAUNCH_CHECK("k_les_grad");
+  les_c2r(c, L, L->G, LES_NF * nyp, L->Ph);
+  if (!dynamic) return;
+  dim3 gp((unsigned)((L->ps + 255) / 256), nyp);
+  hipLaunchKernelGGL(k_les_prod, gp, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->P);
+  LAUNCH_CHECK("k_les_prod");
+  les_r2c(c, L, L->P, 12 * nyp, L->Gt);
+  dim3 gk((unsigned)((L->nm + 255) / 256), 12 * nyp);
+  double2* Gf = L->G + (long)LES_NF * nyp * L->nm;
+  hipLaunchKernelGGL(k_les_mask, gk, dim3(256), 0, c->stream, L->Gt, L->Gt, c->nxh, c->prm.nz, L->kxc, L->kzc);
+  hipLaunchKernelGGL(k_les_mask, gk, dim3(256), 0, c->stream, L->G, Gf, c->nxh, c->prm.nz, L->kxc, L->kzc);
+  LAUNCH_CHECK("k_les_mask");
+  les_c2r(c, L, L->Gt, 12 * nyp, L->T);
+  les_c2r(c, L, Gf, 12 * nyp, L->T + (long)12 * nyp * L->ps);
+  hipLaunchKernelGGL(k_les_germano, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, L->ps, L->T, L->delta2,
+                     L->part);
+  LAUNCH_CHECK("k_les_germano");
+  hipLaunchKernelGGL(k_les_cs2, dim3((nyp + 63) / 64), dim3(64), 0, c->stream, ny, L->nblk, L->part, L->cs2);
+  LAUNCH_CHECK("k_les_cs2");
+}
+
+void les_compute(chan_ctx* c, LesState* L) {
+  const int ny = c->prm.ny, nyp = c->nyp;
+  if (L->mode == 3) return;
+  les_fields(c, L, true);
+  dim3 gp((unsigned)((L->ps + 255) / 256), nyp);
+  if (L->mode == 1) {
+    hipLaunchKernelGGL(k_les_tau_smag, gp, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->cs2, L->delta2, L->tau);
+    LAUNCH_CHECK("k_les_tau_smag");
+    return;
+  }
+  dim3 gi((unsigned)((L->ps + 255) / 256), ny - 1);
+  hipLaunchKernelGGL(k_les_feat, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->ydist, L->delta2, L->X,
+                     L->scale);
+  LAUNCH_CHECK("k_les_feat");
+  mlp_infer_device(L->net, L->npts, L->X, L->Y, c->stream);
+  launch_fill_zero(L->gated, sizeof(unsigned long long), c->stream);
+  hipLaunchKernelGGL(k_les_tau_mlp, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->Y, L->scale, L->cs2,
+                     L->delta2, L->beta, L->tau, L->gated);
+  LAUNCH_CHECK("k_les_tau_mlp");
+  hipLaunchKernelGGL(k_les_zero_walls, dim3((unsigned)((L->ps + 255) / 256)), dim3(256), 0, c->stream, ny, L->ps,
+                     L->tau);
+  LAUNCH_CHECK("k_les_zero_walls");
+}
+
+LesState* les_need(chan_ctx* c) {
+  if (!c->les || c->les->mode == 0) throw chan_error("LES closure not configured (chan_les_configure)");
+  return c->les;
+}
+
+}  // namespace
+
+void les_destroy(LesState* L) {
+  if (!L) return;
+  lfree(L->G); lfree(L->Gt); lfree(L->Zl); lfree(L->Ph); lfree(L->P); lfree(L->T); lfree(L->tau);
+  lfree(L->cs2); lfree(L->delta2); lfree(L->ydist); lfree(L->part); lfree(L->X); lfree(L->Y);
+  lfree(L->scale); lfree(L->gated);
+  delete L;
+}
+
+const double* les_tau_for_pass(chan_ctx* c) {
+  LesState* L = c->les;
+  if (!L || L->mode == 0) return nullptr;
+  if (L->mode == 3) {
+    if (!L->fresh) throw chan_error("prescribed subgrid stress not set (chan_les_set_tau)");
+    return L->tau;
+  }
+  les_compute(c, L);
+  return L->tau;
+}
+
+extern "C" {
+
+int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta) {
+  return guarded_les(c, [&] {
+    if (mode < 0 || mode > 3) throw chan_error("LES mode must be 0 (off), 1, 2 or 3");
+    if (mode == 2 && !net) throw chan_error("the learned closure needs a chan_mlp");
+    if (!(beta >= 0.0)) throw chan_error("gate beta must be >= 0");
+    check_ops_les(c);
+    if (mode == 0) {
+      les_destroy(c->les);
+      c->les = nullptr;
+      return;
+    }
+    if ((int)c->h_y.size() != c->prm.ny) throw chan_error("LES closure needs chan_set_coordinates");
+    const XZGeom& g = c->gd;
+    if (!(c->tune.xpass_wave && fft2_split(g.nx2 / 2).n1 > 0 && g.nx2 / 2 <= 192))
+      throw chan_error("the subgrid closure needs the wavefront x-pass (nx2/2 <= 192 of the form 2^a 3^b)");
+    les_destroy(c->les);
+    c->les = nullptr;
+    LesState* L = new LesState();
+    c->les = L;
+    const int ny = c->prm.ny, nyp = c->nyp;
+    L->mode = mode;
+    L->beta = beta;
+    L->net = net;
+    L->kxc = c->prm.nx / 4;
+    L->kzc = c->prm.nz / 4;
+    L->ps = (long)g.nz2 * g.nx2;
+    L->nm = (long)c->nxh * c->prm.nz;
+    L->npts = (long)(ny - 1) * L->ps;
+    L->nblk = (int)std::min<long>(64, (L->ps + 255) / 256);
+    const long zps = (long)g.nz2 * g.kxpad;
+    lalloc(L->G, (size_t)2 * LES_NF * nyp * L->nm);
+    lalloc(L->Gt, (size_t)12 * nyp * L->nm);
+    lalloc(L->Zl, (size_t)12 * nyp * zps);
+    lalloc(L->Ph, (size_t)LES_NF * nyp * L->ps);
+    lalloc(L->P, (size_t)12 * nyp * L->ps);
+    lalloc(L->T, (size_t)24 * nyp * L->ps);
+    lalloc(L->tau, (size_t)6 * nyp * L->ps);
+    lalloc(L->cs2, nyp);
+    lalloc(L->delta2, nyp);
+    lalloc(L->ydist, nyp);
+    lalloc(L->part, (size_t)2 * nyp * L->nblk);
+    lalloc(L->gated, 2);
+    if (mode == 2) {
+      lalloc(L->X, (size_t)10 * L->npts);
+      lalloc(L->Y, (size_t)6 * L->npts);
+      lalloc(L->scale, L->npts);
+      mlp_prepare_device(net);
+    }
+    // filter width D^2 and wall distance 1 - |yh| per plane (zero on the walls)
+    const std::vector<double>& y = c->h_y;
+    const double dx = c->prm.lx / c->prm.nx, dz = c->prm.lz / c->prm.nz;
+    std::vector<double> d2(nyp, 0.0), yd(nyp, 0.0);
+    for (int k = 1; k < ny; ++k) {
+      const double d = std::cbrt(dx * dz * (y[k] - y[k - 1]));
+      d2[k] = d * d;
+      yd[k] = 1.0 - std::fabs(0.5 * (y[k - 1] + y[k]));
+    }
+    CHAN_CHECK(hipMemcpy(L->delta2, d2.data(), sizeof(double) * nyp, hipMemcpyHostToDevice));
+    CHAN_CHECK(hipMemcpy(L->ydist, yd.data(), sizeof(double) * nyp, hipMemcpyHostToDevice));
+    launch_fill_zero(L->cs2, sizeof(double) * nyp, c->stream);
+    launch_fill_zero(L->tau, sizeof(double) * 6 * nyp * L->ps, c->stream);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_les_update(chan_ctx* c) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (L->mode == 3) return;
+    les_prepare_state(c);
+    les_compute(c, L);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_les_set_tau(chan_ctx* c, const double* tau) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (L->mode != 3) throw chan_error("chan_les_set_tau needs LES mode 3 (prescribed stress)");
+    if (!tau) throw chan_error("null tau");
+    CHAN_CHECK(hipMemcpy(L->tau, tau, sizeof(double) * 6 * c->nyp * L->ps, hipMemcpyHostToDevice));
+    L->fresh = true;
+  });
+}
+
+int chan_les_get_tau(chan_ctx* c, double* tau) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (!tau) throw chan_error("null tau");
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    CHAN_CHECK(hipMemcpy(tau, L->tau, sizeof(double) * 6 * c->nyp * L->ps, hipMemcpyDeviceToHost));
+  });
+}
+
+int chan_les_get_coefficient(chan_ctx* c, double* cs2) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (!cs2) throw chan_error("null cs2");
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    CHAN_CHECK(hipMemcpy(cs2, L->cs2, sizeof(double) * c->nyp, hipMemcpyDeviceToHost));
+  });
+}
+
+int chan_les_get_fields(chan_ctx* c, double* fields) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (!fields) throw chan_error("null fields");
+    les_prepare_state(c);
+    les_fields(c, L, false);
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    CHAN_CHECK(hipMemcpy(fields, L->Ph, sizeof(double) * LES_NF * c->nyp * L->ps, hipMemcpyDeviceToHost));
+  });
+}
+
+int chan_les_features(chan_ctx* c, float* x, double* scale) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    if (L->mode != 2) throw chan_error("chan_les_features needs LES mode 2 (learned closure)");
+    if (!x || !scale) throw chan_error("null output");
+    les_prepare_state(c);
+    les_fields(c, L, false);
+    dim3 gi((unsigned)((L->ps + 255) / 256), c->prm.ny - 1);
+    hipLaunchKernelGGL(k_les_feat, gi, dim3(256), 0, c->stream, c->prm.ny, L->ps, L->Ph, L->ydist, L->delta2, L->X,
+                       L->scale);
+    LAUNCH_CHECK("k_les_feat");
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    CHAN_CHECK(hipMemcpy(x, L->X, sizeof(float) * 10 * L->npts, hipMemcpyDeviceToHost));
+    CHAN_CHECK(hipMemcpy(scale, L->scale, sizeof(double) * L->npts, hipMemcpyDeviceToHost));
+  });
+}
+
+int chan_les_gated_points(chan_ctx* c, long* count, long* total) {
+  return guarded_les(c, [&] {
+    LesState* L = les_need(c);
+    unsigned int g = 0;
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    CHAN_CHECK(hipMemcpy(&g, L->gated, sizeof(g), hipMemcpyDeviceToHost));
+    if (count) *count = (L->mode == 2) ? (long)g : 0;
+    if (total) *total = L->npts;
+  });
+}
+
+}  // extern "C"

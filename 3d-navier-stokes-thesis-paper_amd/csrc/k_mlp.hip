@@ -391,6 +391,23 @@ int mguard(chan_mlp* m, F&& f) {
 
 }  // namespace
 
+void mlp_prepare_device(chan_mlp* m) {
+  pack(m);
+  mchk(hipStreamSynchronize(m->stream), "sync");
+}
+
+// inference of n points already on the device, on the caller's stream
+// (weights changed since the last packing are packed first, synchronised)
+void mlp_infer_device(chan_mlp* m, long n, const float* x, float* y, hipStream_t s) {
+  if (!m->packed_ok) mlp_prepare_device(m);
+  if (n <= 0) return;
+  if (n > 0x7fffffffL) throw chan_error("too many points for one inference");
+  const long tiles = (n + 16 * MLP_NT - 1) / (16 * MLP_NT);
+  const int blocks = (int)std::max<long>(1, std::min<long>(1024, (tiles + 3) / 4));
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, m->d_packed, m->d_p, (int)n, x, y);
+  LAUNCH_CHECK("k_mlp_infer");
+}
+
 extern "C" {
 
 int chan_mlp_create(chan_mlp** out) {

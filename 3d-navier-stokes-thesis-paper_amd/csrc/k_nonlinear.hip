@@ -269,7 +269,7 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
 // writes cells base + i) as soon as the current rows have been consumed, so
 // their latency overlaps a whole plane of work instead of stalling the
 // wavefront at the start of every plane.
-template <bool LIN, int M, bool PF>
+template <bool LIN, int M, bool PF, bool LES = false>
 __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
                                                   XPassArgs a) {
   extern __shared__ double2 lds[];
@@ -384,6 +384,14 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   auto bufv = [&](int q) {
     return *reinterpret_cast<const double2*>(ops.buffer + (long)zz * g.nx2 + 2 * nq[q]);
   };
+  // Subgrid stress (LES): tau_ij is added to the flux u_i u_j wherever the
+  // flux is formed, so the divergence of the fluxes carries -div(tau) with the
+  // same discrete operators.  tau lives at the cell centres yh; at the v-points
+  // y(k) tau_12 and tau_23 are interpolated with the iu weights, as u and w are.
+  constexpr bool les = LES;
+  auto tauv = [&](int c, int kk, int q) {
+    return *reinterpret_cast<const double2*>(a.tau + (((long)c * nyp + kk) * nz2 + zz) * g.nx2 + 2 * nq[q]);
+  };
 
   const double2 z2 = make_double2(0.0, 0.0);
   double2 U0[PPL], V0[PPL], W0[PPL], U1[PPL], V1[PPL], W1[PPL];
@@ -427,6 +435,15 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
       double2 t = psc(half, padd(V0[q], V1[q])), tb = psc(half, padd(BV0[q], BV1[q]));
       GV[q] = pmul(psc(two, t), tb);
     }
+    if (les && vq[q]) {
+      const int k0 = ka - 2;  // 0-based plane ka-1
+      if (ka - 1 != 1) {      // the bottom wall carries no subgrid stress
+        const double j1 = ldro(ops.iu, 2 * k0), j2 = ldro(ops.iu, 2 * k0 + 1);
+        FU[q] = padd(FU[q], padd(psc(j1, tauv(1, k0, q)), psc(j2, tauv(1, k0 + 1, q))));
+        FW[q] = padd(FW[q], padd(psc(j1, tauv(4, k0, q)), psc(j2, tauv(4, k0 + 1, q))));
+      }
+      GV[q] = padd(GV[q], tauv(3, k0 + 1, q));
+    }
     U0[q] = U1[q]; V0[q] = V1[q]; W0[q] = W1[q];
     BU0[q] = BU1[q]; BV0[q] = BV1[q]; BW0[q] = BW1[q];
   }
@@ -465,6 +482,10 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
         }
       }
       if (!vq[q]) continue;
+      if (les && !top) {
+        Fu[q] = padd(Fu[q], padd(psc(i1, tauv(1, kk, q)), psc(i2, tauv(1, kk + 1, q))));
+        Fw[q] = padd(Fw[q], padd(psc(i1, tauv(4, kk, q)), psc(i2, tauv(4, kk + 1, q))));
+      }
       double2 yfu = psc(dh, psub(Fu[q], FU[q]));
       double2 yfw = psc(dh, psub(Fw[q], FW[q]));
       if (hasbuf) {
@@ -476,6 +497,7 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
       double2 f0, f2;
       if (!LIN) { f0 = pmul(U0[q], U0[q]); f2 = pmul(W0[q], U0[q]); }
       else { f0 = pmul(psc(two, BU0[q]), U0[q]); f2 = padd(pmul(BW0[q], U0[q]), pmul(W0[q], BU0[q])); }
+      if (les) { f0 = padd(f0, tauv(0, kk, q)); f2 = padd(f2, tauv(2, kk, q)); }
       W[0 * SEQ + pin[q]] = f0;
       W[1 * SEQ + pin[q]] = yfu;
       W[2 * SEQ + pin[q]] = f2;
@@ -540,7 +562,10 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
         }
       }
       if (!vq[q]) continue;
-      W[0 * SEQ + pin[q]] = LIN ? pmul(psc(two, BW0[q]), W0[q]) : pmul(W0[q], W0[q]);
+      if (les && !top) G1[q] = padd(G1[q], tauv(3, kk + 1, q));
+      double2 f4 = LIN ? pmul(psc(two, BW0[q]), W0[q]) : pmul(W0[q], W0[q]);
+      if (les) f4 = padd(f4, tauv(5, kk, q));
+      W[0 * SEQ + pin[q]] = f4;
       if (!top) {
         double2 f5, f7;
         if (!LIN) {
@@ -551,6 +576,10 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
                     pmul(V0[q], padd(psc(i1, BU0[q]), psc(i2, BU1[q]))));
           f7 = padd(pmul(BV0[q], padd(psc(i1, W0[q]), psc(i2, W1[q]))),
                     pmul(V0[q], padd(psc(i1, BW0[q]), psc(i2, BW1[q]))));
+        }
+        if (les) {
+          f5 = padd(f5, padd(psc(i1, tauv(1, kk, q)), psc(i2, tauv(1, kk + 1, q))));
+          f7 = padd(f7, padd(psc(i1, tauv(4, kk, q)), psc(i2, tauv(4, kk + 1, q))));
         }
         double2 yfv = psc(dnk, psub(G1[q], GV[q]));                                    // :188-189
         if (hasbuf) {
@@ -613,6 +642,8 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     }
   int nplanes = a.kb - a.ka + 1;
   int nch = (nplanes + a.chunk - 1) / a.chunk;
+  if (a.tau && !(a.wave && ct && M <= 192))
+    throw chan_error("the subgrid closure needs the wavefront x-pass (nx2/2 <= 192 of the form 2^a 3^b)");
   if (a.wave && ct && M <= 192) {
     dim3 gw((g.nz2 + 3) / 4, nch);
     const bool pf = a.prefetch && M <= 144;
@@ -620,7 +651,15 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
 #define XPASSW_CASE(N)                                                                                  \
   if constexpr (N > 0 && N <= 192) {                                                                    \
-    if (pf) {                                                                                           \
+    if (a.tau) {                                                                                        \
+      if (a.base) throw chan_error("the subgrid closure is not defined for the linearized pass");       \
+      if constexpr (N <= 144) {                                                                         \
+        if (pf) hipLaunchKernelGGL((k_xpass_w<false, N, true, true>), gw, dim3(256), ldw, s, g, tw, ops, a); \
+        else hipLaunchKernelGGL((k_xpass_w<false, N, false, true>), gw, dim3(256), ldw, s, g, tw, ops, a);   \
+      } else {                                                                                          \
+        hipLaunchKernelGGL((k_xpass_w<false, N, false, true>), gw, dim3(256), ldw, s, g, tw, ops, a);     \
+      }                                                                                                 \
+    } else if (pf) {                                                                                    \
       if constexpr (N <= 144) {                                                                         \
         if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a); \
         else hipLaunchKernelGGL((k_xpass_w<false, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a);       \

@@ -251,6 +251,36 @@ int chan_mlp_gradient(chan_mlp* net, int n, const float* x, const float* t, floa
 int chan_mlp_train(chan_mlp* net, int n, const float* x, const float* t, int epochs, int batch, double lr,
                    double* loss_out);
 
+/* Subgrid-scale closure for large-eddy simulation (no counterpart in the
+ * reference beyond its unused switches subgrid / filter_dim / lagr_avg,
+ * PFC/flow.dat:19-22, and commented-out calls of a dynamic model,
+ * PFC/main_mpi.f90:10-13,74,77-78).  With a closure configured, every
+ * chan_nonlinear (and chan_step) adds the subgrid stress tau_ij to the flux
+ * u_i u_j, so the nonlinear term becomes div(u u + tau) with the discrete
+ * operators of PFC/pass1.f90:138-195; tau_12, tau_23 are interpolated to the
+ * v-points with iu.  tau lives at the cell centres yh of the dealiased
+ * physical grid, layout double [6][ny+1][nz2][nx2] (components 11, 12, 13,
+ * 22, 23, 33), zero on the wall planes.
+ * Modes: 0 off; 1 dynamic Smagorinsky (Germano identity, sharp test filter at
+ * half the resolved kx and kz, plane least squares, clipped at 0); 2 learned
+ * closure `net` with a fallback to dynamic Smagorinsky wherever the learned
+ * stress's energy transfer -tau:S is below -beta times Smagorinsky's or not
+ * finite; 3 prescribed tau (chan_les_set_tau).  Needs chan_set_coordinates.
+ * chan_les_update evaluates the closure on the current state;
+ * chan_les_get_coefficient returns C(y) [ny+1]; chan_les_get_fields the
+ * physical closure inputs [12][ny+1][nz2][nx2] (du_i/dx_j row-major, then u,
+ * v, w at yh); chan_les_features the network inputs [(ny-1) nz2 nx2][10] and
+ * the stress scale D^2 |du/dx|^2 per point (mode 2); chan_les_gated_points the
+ * number of points of the last evaluation that fell back, and the total. */
+int chan_les_configure(chan_ctx* ctx, int mode, chan_mlp* net, double beta);
+int chan_les_update(chan_ctx* ctx);
+int chan_les_set_tau(chan_ctx* ctx, const double* tau);
+int chan_les_get_tau(chan_ctx* ctx, double* tau);
+int chan_les_get_coefficient(chan_ctx* ctx, double* cs2);
+int chan_les_get_fields(chan_ctx* ctx, double* fields);
+int chan_les_features(chan_ctx* ctx, float* x, double* scale);
+int chan_les_gated_points(chan_ctx* ctx, long* count, long* total);
+
 /* Legacy VTK output as output_hdf_dfsd writes it with xskip = yskip = zskip = 1
  * (PFC/read_write.f90:296-500): big-endian RECTILINEAR_GRID, nx x (ny+1) x nz
  * points (x = lx i/nx, y = yh, z = lz k/nz), float32 vectors "flowvec".

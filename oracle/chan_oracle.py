@@ -484,11 +484,17 @@ def _cfl_and_dt(g: Grid, step, rk_step, dt, amax, vmax, wmax):
     return dt, cfl_temp
 
 
-def nonlinear(g: Grid, U, step=1, rk_step=1, dt=0.0, return_cfl=False):
+def nonlinear(g: Grid, U, step=1, rk_step=1, dt=0.0, return_cfl=False, tau=None):
     """nonlinear (PFC/pass1.f90:26-252), vectorised over planes.  The
     reference overwrites data1 plane k only after planes k and k+1 have been
     transformed (:65-83), so the result equals a computation from the
-    original field, which is what this does.  Returns (NL, dt[, cfl])."""
+    original field, which is what this does.  Returns (NL, dt[, cfl]).
+
+    tau (no counterpart in the reference): a subgrid stress [6, ny+1, nz2,
+    nx2] at the cell centres (components 11, 12, 13, 22, 23, 33, zero on the
+    wall planes) added to each flux u_i u_j where the flux is formed;
+    tau_12 and tau_23 reach the interior v-points y(k) with the iu weights
+    and are zero at the walls."""
     ny = g.ny
     ph = xz_c2r(g, U)                       # [3, ny+1, nz2, nx2]
     u, v, w = ph[0], ph[1], ph[2]
@@ -506,20 +512,37 @@ def nonlinear(g: Grid, U, step=1, rk_step=1, dt=0.0, return_cfl=False):
     # v flux at yh(j), j = 2..ny: G[j-2]  (:54-55,184-185)
     tmp = half * (v[:ny - 1] + v[1:ny])
     G = tmp * tmp
+    t12y = t23y = 0.0
+    if tau is not None:
+        tau = np.asarray(tau, dtype=np.float64)
+        jv = np.arange(1, ny - 1)
+        a1 = g.iu[0, jv][:, None, None]; a2 = g.iu[1, jv][:, None, None]
+        t12y = np.zeros_like(Fu); t23y = np.zeros_like(Fu)
+        t12y[jv] = a1 * tau[1, jv] + a2 * tau[1, jv + 1]
+        t23y[jv] = a1 * tau[4, jv] + a2 * tau[4, jv + 1]
+        Fu = Fu + t12y
+        Fw = Fw + t23y
+        G = G + tau[3, 1:ny]
     NL = np.zeros_like(U)
     ks = np.arange(2, ny + 1)              # planes 2..ny for u,w
     kk = ks - 1                            # array index of plane k
     dh = g.dh[kk][:, None, None]
     yfu = dh * (Fu[kk] - Fu[kk - 1]) + buf * (u[kk] - tgt[kk, 0][:, None, None])
     yfw = dh * (Fw[kk] - Fw[kk - 1]) + buf * (w[kk] - tgt[kk, 2][:, None, None])
-    NL[0, kk] = div_fluxes(g, u[kk] * u[kk], yfu, w[kk] * u[kk])    # :158-162
-    NL[2, kk] = div_fluxes(g, u[kk] * w[kk], yfw, w[kk] * w[kk])    # :163-167
+    fuu, fwu, fww = u[kk] * u[kk], w[kk] * u[kk], w[kk] * w[kk]
+    if tau is not None:
+        fuu = fuu + tau[0, kk]; fwu = fwu + tau[2, kk]; fww = fww + tau[5, kk]
+    NL[0, kk] = div_fluxes(g, fuu, yfu, fwu)                         # :158-162
+    NL[2, kk] = div_fluxes(g, fwu, yfw, fww)                         # :163-167
     kv = np.arange(2, ny) - 1              # planes 2..ny-1 for v
     dn = g.dn[kv][:, None, None]
     yfv = dn * (G[kv] - G[kv - 1]) + buf * (v[kv] - tgt[kv, 1][:, None, None])   # :188-189
     i1 = g.iu[0, kv][:, None, None]; i2 = g.iu[1, kv][:, None, None]
-    NL[1, kv] = div_fluxes(g, v[kv] * (i1 * u[kv] + i2 * u[kv + 1]), yfv,
-                           v[kv] * (i1 * w[kv] + i2 * w[kv + 1]))   # :190-195
+    fvu = v[kv] * (i1 * u[kv] + i2 * u[kv + 1])
+    fvw = v[kv] * (i1 * w[kv] + i2 * w[kv + 1])
+    if tau is not None:
+        fvu = fvu + t12y[kv]; fvw = fvw + t23y[kv]
+    NL[1, kv] = div_fluxes(g, fvu, yfv, fvw)                         # :190-195
     # planes 1, ny (v) and ny+1 are zero (:62,96-99)
     if rk_step == 1:
         amax = np.abs(u[1:ny] + u[2:ny + 1]).max(axis=(1, 2))

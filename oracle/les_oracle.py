@@ -131,3 +131,173 @@ def smagorinsky_tau(grad, delta, cs):
     nu = (cs * np.asarray(delta, dtype=np.float64)) ** 2 * Smag
     idx = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
     return np.stack([-2.0 * nu * S[:, i, j] for i, j in idx], axis=1)
+
+
+# ---------------------------------------------------------------------------
+# The closure on the channel grid (the device's LES module)
+#
+# Fields at the cell centres yh of planes 1..ny-1 (zero on the wall planes):
+# G[0..8] = du_i/dx_j (row-major, i = u, v, w; j = x, y, z), G[9..11] = u, v,
+# w; d/dx and d/dz spectral, v at yh the mean of its faces, du/dy and dw/dy
+# central over yh(k-1), yh(k+1), dv/dy the face difference dh(k) (v(k) -
+# v(k-1)).  Pinned by closed forms (linear shear, single Fourier modes), by
+# the Germano identity vanishing for x-z uniform fields and by second-order
+# convergence of the subgrid-stress divergence (tests/test_les_closure.py).
+# ---------------------------------------------------------------------------
+SYM = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def _co():
+    from oracle import chan_oracle as co
+    return co
+
+
+def les_spectral_fields(g, U):
+    """The 12 spectral fields [12, ny+1, nxh, nz] of the closure."""
+    ny = g.ny
+    G = np.zeros((12, ny + 1, g.nxh, g.nz), dtype=np.complex128)
+    k = np.arange(1, ny)
+    ikx = 1j * g.kx[:, None]
+    ikz = 1j * g.kz[None, :]
+    u, w = U[0, k], U[2, k]
+    vm, v0 = U[1, k - 1], U[1, k]
+    vc = 0.5 * (vm + v0)
+    dyc = (1.0 / (g.hn[k] + g.hn[k - 1]))[:, None, None]
+    dhk = g.dh[k][:, None, None]
+    G[0, k] = ikx * u
+    G[1, k] = (U[0, k + 1] - U[0, k - 1]) * dyc
+    G[2, k] = ikz * u
+    G[3, k] = ikx * vc
+    G[4, k] = dhk * (v0 - vm)
+    G[5, k] = ikz * vc
+    G[6, k] = ikx * w
+    G[7, k] = (U[2, k + 1] - U[2, k - 1]) * dyc
+    G[8, k] = ikz * w
+    G[9, k], G[10, k], G[11, k] = u, vc, w
+    return G
+
+
+def les_fields(g, U):
+    """Physical closure inputs [12, ny+1, nz2, nx2]."""
+    return _co().xz_c2r(g, les_spectral_fields(g, U))
+
+
+def strain(gf):
+    """S_ij in the order 11, 12, 13, 22, 23, 33 and |S| = sqrt(2 S:S) from
+    gradient fields gf[9, ...]."""
+    s = np.stack([gf[0], 0.5 * (gf[1] + gf[3]), 0.5 * (gf[2] + gf[6]), gf[4], 0.5 * (gf[5] + gf[7]), gf[8]])
+    ss = s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.0 * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4])
+    return s, np.sqrt(2.0 * ss)
+
+
+def contract(a, b):
+    """sum_ij a_ij b_ij of symmetric tensors stored as 6 components."""
+    return a[0] * b[0] + a[3] * b[3] + a[5] * b[5] + 2.0 * (a[1] * b[1] + a[2] * b[2] + a[4] * b[4])
+
+
+def les_delta2(g):
+    """Filter width squared per plane, D = (dx dz (y(k)-y(k-1)))^(1/3)."""
+    d2 = np.zeros(g.ny + 1)
+    dx, dz = g.lx / g.nx, g.lz / g.nz
+    for k in range(1, g.ny):
+        d = np.cbrt(dx * dz * (g.y[k] - g.y[k - 1]))
+        d2[k] = d * d
+    return d2
+
+
+def les_ydist(g):
+    """Wall distance 1 - |yh| per plane (zero on the walls)."""
+    yd = np.zeros(g.ny + 1)
+    k = np.arange(1, g.ny)
+    yd[k] = 1.0 - np.abs(0.5 * (g.y[k - 1] + g.y[k]))
+    return yd
+
+
+def test_filter_mask(g):
+    """Sharp test filter at half the resolved wavenumbers: kx index < nx/4,
+    |kz index| < nz/4.  Returns a boolean [nxh, nz]."""
+    ix = np.arange(g.nxh)[:, None]
+    iz = np.arange(g.nz)[None, :]
+    kzi = np.where(iz <= g.nz // 2, iz, iz - g.nz)
+    return (ix < g.nx // 4) & (np.abs(kzi) < g.nz // 4)
+
+
+def dynamic_coefficient(g, U):
+    """Dynamic Smagorinsky C(y): Germano identity L = C M with
+    L_ij = T(u_i u_j) - T(u_i) T(u_j), M_ij = 2 D^2 (T(|S| S_ij) - 4 |S^| S^_ij),
+    plane least squares <L:M>/<M:M> (Lilly 1992), clipped at zero."""
+    co = _co()
+    ny = g.ny
+    Gs = les_spectral_fields(g, U)
+    Ph = co.xz_c2r(g, Gs)
+    s, mag = strain(Ph[:9])
+    u = Ph[9:12]
+    P = np.zeros((12,) + Ph.shape[1:])
+    for t, (i, j) in enumerate(SYM):
+        P[t] = u[i] * u[j]
+        P[6 + t] = mag * s[t]
+    P[:, 0] = 0.0
+    P[:, ny] = 0.0
+    mask = test_filter_mask(g)
+    TP = co.xz_c2r(g, co.xz_r2c(g, P) * mask)
+    TG = co.xz_c2r(g, Gs * mask)
+    st, magt = strain(TG[:9])
+    ut = TG[9:12]
+    d2 = les_delta2(g)
+    cs2 = np.zeros(ny + 1)
+    for k in range(1, ny):
+        L = np.stack([TP[t, k] - ut[i, k] * ut[j, k] for t, (i, j) in enumerate(SYM)])
+        M = np.stack([2.0 * d2[k] * (TP[6 + t, k] - 4.0 * magt[k] * st[t, k]) for t in range(6)])
+        lm = contract(L, M).sum()
+        mm = contract(M, M).sum()
+        c = lm / mm if mm > 0.0 else 0.0
+        cs2[k] = c if c > 0.0 else 0.0
+    return cs2
+
+
+def tau_smagorinsky(g, Ph, cs2):
+    """tau_ij = -2 C D^2 |S| S_ij at the interior planes, [6, ny+1, nz2, nx2]."""
+    s, mag = strain(Ph[:9])
+    d2 = les_delta2(g)
+    tau = np.zeros((6,) + Ph.shape[1:])
+    k = np.arange(1, g.ny)
+    c = (-2.0 * cs2[k] * d2[k])[:, None, None] * mag[k]
+    for t in range(6):
+        tau[t, k] = c * s[t, k]
+    return tau
+
+
+def features(g, Ph):
+    """Network inputs [(ny-1) nz2 nx2, 10] (float32) and the stress scale
+    D^2 |G|^2 per point, points ordered ((k-1) nz2 + z) nx2 + x."""
+    ny = g.ny
+    gf = Ph[:9, 1:ny]
+    n2 = (gf * gf).sum(axis=0)
+    gn = np.sqrt(n2)
+    inv = np.where(gn > 0.0, 1.0 / np.where(gn > 0.0, gn, 1.0), 0.0)
+    X = np.empty(gf.shape[1:] + (10,), dtype=np.float32)
+    for f in range(9):
+        X[..., f] = (gf[f] * inv).astype(np.float32)
+    X[..., 9] = np.float32(0.0)
+    X[..., 9] = les_ydist(g)[1:ny, None, None].astype(np.float32)
+    scale = les_delta2(g)[1:ny, None, None] * n2
+    return X.reshape(-1, 10), scale.reshape(-1)
+
+
+def tau_learned(g, Ph, Y, scale, cs2, beta=1.0):
+    """Learned stress tau = scale * Y with the fallback gate: a point whose
+    energy transfer -tau:S is below -beta times Smagorinsky's, or is not
+    finite, takes the Smagorinsky stress.  Returns (tau, fallback count)."""
+    ny = g.ny
+    s, mag = strain(Ph[:9])
+    ts = tau_smagorinsky(g, Ph, cs2)
+    sh = (ny - 1,) + Ph.shape[2:]
+    tm = np.stack([(scale * Y[:, t].astype(np.float64)).reshape(sh) for t in range(6)])
+    si = s[:, 1:ny]
+    em = -contract(tm, si)
+    es = -contract(ts[:, 1:ny], si)
+    with np.errstate(invalid="ignore"):
+        fb = ~(em >= -beta * es) | ~np.isfinite(em)
+    tau = np.zeros_like(ts)
+    tau[:, 1:ny] = np.where(fb[None], ts[:, 1:ny], tm)
+    return tau, int(fb.sum())

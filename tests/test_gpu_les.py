@@ -1,0 +1,125 @@
+"""GPU parity of the subgrid closure (LES module, through the C-ABI) against
+oracle/les_oracle.py and the oracle's nonlinear term with a subgrid stress.
+
+Tolerances (FP64 unless noted):
+  * closure inputs (12 physical fields), Smagorinsky stress, nonlinear term
+    with a prescribed stress: relative L2 <= 1e-12;
+  * dynamic coefficient C(y): relative L2 <= 1e-10 (plane sums in a
+    different order);
+  * network inputs (float32): max abs difference <= 2e-6;
+  * learned stress with the gate, from the device's own network outputs:
+    relative L2 <= 1e-12, same number of fallback points;
+  * 5 RK3 steps with the dynamic model: relative L2 <= 1e-10.
+"""
+import numpy as np
+import pytest
+
+from helpers import chanflow, grid_cfg0, init_cfg0, make_channel, rel_l2
+from oracle import chan_oracle as O
+from oracle import les_oracle as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g0():
+    return grid_cfg0(O)
+
+
+def _chan_with(g, U, mode, net=None, beta=1.0):
+    ch = make_channel(g)
+    ch.set_state(data1=U)
+    ch.les_configure(mode, net=net, beta=beta)
+    return ch
+
+
+def test_les_fields_and_dynamic_smagorinsky(g0):
+    U = init_cfg0(O, g0, seed=11, amp=0.05)
+    ch = _chan_with(g0, U, "dynamic")
+    Ph = ch.les_fields()
+    ref = L.les_fields(g0, U)
+    assert rel_l2(Ph, ref) < 1e-12
+    ch.les_update()
+    cs2 = ch.les_coefficient()
+    cref = L.dynamic_coefficient(g0, U)
+    assert cref.max() > 0.0
+    assert rel_l2(cs2, cref) < 1e-10
+    tau = ch.les_tau()
+    tref = L.tau_smagorinsky(g0, ref, cref)
+    assert rel_l2(tau, tref) < 1e-10
+    assert np.all(tau[:, 0] == 0.0) and np.all(tau[:, g0.ny] == 0.0)
+    ch.close()
+
+
+def test_les_prescribed_stress_nonlinear(g0):
+    """The x-pass adds tau to every flux: NL(U, tau) against the oracle."""
+    U = init_cfg0(O, g0, seed=12, amp=0.05)
+    rng = np.random.default_rng(3)
+    tau = 1e-3 * rng.standard_normal((6, g0.ny + 1, g0.nz2, g0.nx2))
+    tau[:, 0] = 0.0
+    tau[:, g0.ny] = 0.0
+    ch = _chan_with(g0, U, "prescribed")
+    ch.les_set_tau(tau)
+    ch.nonlinear(1, 2)
+    d1 = ch.get_data1()
+    ref, _ = O.nonlinear(g0, U, rk_step=2, tau=tau)
+    base, _ = O.nonlinear(g0, U, rk_step=2)
+    assert rel_l2(ref, base) > 1e-3   # the stress matters
+    assert rel_l2(d1, ref) < 1e-12
+    ch.close()
+
+
+def test_les_learned_closure_and_gate(g0):
+    cf = chanflow()
+    W, b = L.init_params(seed=21)
+    net = cf.Mlp(cf.Mlp.flatten(W, b))
+    U = init_cfg0(O, g0, seed=13, amp=0.05)
+    ch = _chan_with(g0, U, "learned", net=net, beta=0.5)
+    X, sc = ch.les_features()
+    Ph = ch.les_fields()
+    Xr, scr = L.features(g0, Ph)
+    assert np.abs(X - Xr).max() <= 2e-6
+    assert rel_l2(sc, scr) < 1e-12
+    ch.les_update()
+    Y = net.infer(X)
+    cs2 = ch.les_coefficient()
+    tref, nref = L.tau_learned(g0, Ph, Y, sc, cs2, beta=0.5)
+    nfb, ntot = ch.les_gated_points()
+    assert ntot == X.shape[0]
+    assert nfb == nref
+    assert 0 < nfb < ntot          # a random network triggers the gate at some points only
+    assert rel_l2(ch.les_tau(), tref) < 1e-12
+    ch.close()
+    net.close()
+
+
+def test_les_dynamic_steps(g0):
+    """5 RK3 steps with the dynamic model against the oracle's time step with
+    the same closure evaluated at every substep."""
+    U0 = init_cfg0(O, g0, seed=14, amp=0.05)
+    dt = 0.01
+    ch = make_channel(g0)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=dt, p_grad=-2.0 / g0.re)
+    ch.les_configure("dynamic")
+    ch.initialize_flow()
+    ch.step(5, step0=1)
+    d1 = ch.get_data1()
+
+    orig = O.nonlinear
+
+    def nl_les(g, U, step=1, rk_step=1, dt=0.0):
+        tau = L.tau_smagorinsky(g, L.les_fields(g, U), L.dynamic_coefficient(g, U))
+        return orig(g, U, step=step, rk_step=rk_step, dt=dt, tau=tau)
+
+    try:
+        O.nonlinear = nl_les
+        st = O.initial_state(g0, U0, dt=dt)
+        for s in range(1, 6):
+            O.rk3_step(g0, st, s)
+    finally:
+        O.nonlinear = orig
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
+    assert ch.divergence_max() <= 1e-12
+    ch.close()
