@@ -167,7 +167,7 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
   const double* tau = nullptr;
   if (!lin && c->les) {
     KTimer t(c, KT_OTHER);
-    tau = les_tau_for_pass(c);
+    tau = les_tau_for_pass(c, rk);
   }
   {
     KTimer t(c, KT_ZINV);

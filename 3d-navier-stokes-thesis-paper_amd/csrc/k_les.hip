@@ -18,7 +18,9 @@
 //   1  dynamic Smagorinsky: tau_ij = -2 C(y) D^2 |S| S_ij, |S| = sqrt(2 S:S),
 //      D = (dx dz (y(k)-y(k-1)))^(1/3), C(y) from the Germano identity with a
 //      sharp test filter at half the resolved x and z wavenumbers (width ratio
-//      2), least squares over each plane (Lilly), clipped at zero;
+//      2), least squares over each plane (Lilly), clipped at zero; C(y) is
+//      evaluated at the first substep of each time step and held for the
+//      other two;
 //   2  learned: an MLP maps (G[0..8]/|G|, 1-|yh|) to tau_ij / (D^2 |G|^2);
 //      a point whose learned stress removes energy at a rate below -beta times
 //      the dynamic Smagorinsky rate, or is not finite, takes the Smagorinsky
@@ -243,8 +245,12 @@ __global__ __launch_bounds__(256) void k_les_tau_mlp(int ny, long ps, const doub
     double* o = tau + (long)kk * ps + i;
     for (int t = 0; t < 6; ++t) o[t * cs] = fb ? ts[t] : tm[t];
   }
+  // fallback count of the block (one slot per block: no contended atomics)
+  __shared__ unsigned int wc[4];
   const unsigned long long b = __ballot(fb);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(gated, (unsigned int)__popcll(b));
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (unsigned int)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) gated[(long)blockIdx.y * gridDim.x + blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
 __global__ void k_les_zero_walls(int ny, long ps, double* __restrict__ tau) {
@@ -283,8 +289,10 @@ struct LesState {
   float* X = nullptr;          // [npts][10]
   float* Y = nullptr;          // [npts][6]
   double* scale = nullptr;     // [npts]
-  unsigned int* gated = nullptr;
+  unsigned int* gated = nullptr;  // fallback points per block of k_les_tau_mlp
+  long ngated = 0;
   bool fresh = false;          // tau is current for the state (mode 3: set by the caller)
+  bool cs2_valid = false;      // C(y) computed for the current step
 };
 
 namespace {
@@ -313,7 +321,7 @@ void les_r2c(chan_ctx* c, LesState* L, const double* in, int np, double2* out) {
   launch_zfwd(g, c->d_tw, L->Zl, nullptr, zps, c->ops.kz, out, L->nm, np, 0, c->stream);
 }
 
-// G, its physical fields Ph and (models 1, 2) the dynamic coefficient
+// G, its physical fields Ph and (dynamic) the dynamic coefficient
 void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
   const int ny = c->prm.ny, nyp = c->nyp;
   dim3 gm((unsigned)((L->nm + 255) / 256), nyp);
@@ -339,10 +347,14 @@ void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
   LAUNCH_CHECK("k_les_cs2");
 }
 
-void les_compute(chan_ctx* c, LesState* L) {
+// the closure on the current state; C(y) is recomputed when `dyn` (the first
+// substep of a time step, an explicit update) or not yet known
+void les_compute(chan_ctx* c, LesState* L, bool dyn) {
   const int ny = c->prm.ny, nyp = c->nyp;
   if (L->mode == 3) return;
-  les_fields(c, L, true);
+  const bool d = dyn || !L->cs2_valid;
+  les_fields(c, L, d);
+  L->cs2_valid = true;
   dim3 gp((unsigned)((L->ps + 255) / 256), nyp);
   if (L->mode == 1) {
     hipLaunchKernelGGL(k_les_tau_smag, gp, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->cs2, L->delta2, L->tau);
@@ -354,7 +366,6 @@ void les_compute(chan_ctx* c, LesState* L) {
                      L->scale);
   LAUNCH_CHECK("k_les_feat");
   mlp_infer_device(L->net, L->npts, L->X, L->Y, c->stream);
-  launch_fill_zero(L->gated, sizeof(unsigned long long), c->stream);
   hipLaunchKernelGGL(k_les_tau_mlp, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->Y, L->scale, L->cs2,
                      L->delta2, L->beta, L->tau, L->gated);
   LAUNCH_CHECK("k_les_tau_mlp");
@@ -378,14 +389,14 @@ void les_destroy(LesState* L) {
   delete L;
 }
 
-const double* les_tau_for_pass(chan_ctx* c) {
+const double* les_tau_for_pass(chan_ctx* c, int rk) {
   LesState* L = c->les;
   if (!L || L->mode == 0) return nullptr;
   if (L->mode == 3) {
     if (!L->fresh) throw chan_error("prescribed subgrid stress not set (chan_les_set_tau)");
     return L->tau;
   }
-  les_compute(c, L);
+  les_compute(c, L, rk == 1);
   return L->tau;
 }
 
@@ -432,7 +443,8 @@ int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta) {
     lalloc(L->delta2, nyp);
     lalloc(L->ydist, nyp);
     lalloc(L->part, (size_t)2 * nyp * L->nblk);
-    lalloc(L->gated, 2);
+    L->ngated = ((L->ps + 255) / 256) * (long)(ny - 1);
+    lalloc(L->gated, (L->ngated + 1) & ~1L);
     if (mode == 2) {
       lalloc(L->X, (size_t)10 * L->npts);
       lalloc(L->Y, (size_t)6 * L->npts);
@@ -452,6 +464,7 @@ int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta) {
     CHAN_CHECK(hipMemcpy(L->ydist, yd.data(), sizeof(double) * nyp, hipMemcpyHostToDevice));
     launch_fill_zero(L->cs2, sizeof(double) * nyp, c->stream);
     launch_fill_zero(L->tau, sizeof(double) * 6 * nyp * L->ps, c->stream);
+    launch_fill_zero(L->gated, sizeof(unsigned int) * ((L->ngated + 1) & ~1L), c->stream);
     CHAN_CHECK(hipStreamSynchronize(c->stream));
   });
 }
@@ -461,7 +474,7 @@ int chan_les_update(chan_ctx* c) {
     LesState* L = les_need(c);
     if (L->mode == 3) return;
     les_prepare_state(c);
-    les_compute(c, L);
+    les_compute(c, L, true);
     CHAN_CHECK(hipStreamSynchronize(c->stream));
   });
 }
@@ -525,10 +538,13 @@ int chan_les_features(chan_ctx* c, float* x, double* scale) {
 int chan_les_gated_points(chan_ctx* c, long* count, long* total) {
   return guarded_les(c, [&] {
     LesState* L = les_need(c);
-    unsigned int g = 0;
+    std::vector<unsigned int> g(L->ngated, 0u);
     CHAN_CHECK(hipStreamSynchronize(c->stream));
-    CHAN_CHECK(hipMemcpy(&g, L->gated, sizeof(g), hipMemcpyDeviceToHost));
-    if (count) *count = (L->mode == 2) ? (long)g : 0;
+    if (L->mode == 2)
+      CHAN_CHECK(hipMemcpy(g.data(), L->gated, sizeof(unsigned int) * L->ngated, hipMemcpyDeviceToHost));
+    long sum = 0;
+    for (unsigned int v : g) sum += v;
+    if (count) *count = sum;
     if (total) *total = L->npts;
   });
 }

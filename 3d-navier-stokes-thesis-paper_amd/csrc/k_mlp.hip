@@ -67,10 +67,21 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ pac
   packed[blk * 64 + l] = v;
 }
 
+// GELU for the bf16 inference path: erf by Abramowitz & Stegun 7.1.26
+// (|error| <= 1.5e-7, with the hardware exp and reciprocal), far inside the
+// 2^-8 relative rounding of the bf16 result it feeds.
+__device__ __forceinline__ float gelu_bf(float x) {
+  const float z = fabsf(x) * 0.70710678118654752f;
+  const float t = __fdividef(1.0f, 1.0f + 0.3275911f * z);
+  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float e = 1.0f - poly * __expf(-z * z);
+  return 0.5f * x * (1.0f + copysignf(e, x));
+}
+
 // Bias + GELU on a result tile, rounded to bf16 as the next B operand.
 __device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ bias, int ot, int lane) {
   bf16x4 r;
-  for (int i = 0; i < 4; ++i) r[i] = (short)f2bf(gelu_f(acc[i] + bias[ot * 16 + 4 * (lane >> 4) + i]));
+  for (int i = 0; i < 4; ++i) r[i] = (short)f2bf(gelu_bf(acc[i] + bias[ot * 16 + 4 * (lane >> 4) + i]));
   return r;
 }
 

@@ -83,16 +83,19 @@ __device__ __forceinline__ double2 r2c_post(const double2* __restrict__ Zs, int 
   return cadd(E, cmul(wk, O));
 }
 
+template <int NC>
 __global__ __launch_bounds__(512) void k_xc2r(XZGeom g, const double2* __restrict__ tw,
                                                const double2* __restrict__ Z, long z_ps,
                                                double* __restrict__ out, long out_ps, int rows) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int M = g.nx2 / 2, ss = M + 1;
+  const int M = NC > 0 ? NC : g.nx2 / 2, ss = NC > 0 ? fft_seq_cells(NC) : M + 1;
   const long P = blockIdx.y;
   const int z0 = blockIdx.x * rows;
   const int nr = min(rows, g.nz2 - z0);
   const double2* rootx = tw + g.twx;
+  double2* rl = lds + rows * ss;
+  load_tw2<NC>(rl, tw + g.px.tw, tid, nthr);
   for (int e = tid; e < rows * M; e += nthr) {
     int r = e / M, n = e - r * M;
     double2 v = make_double2(0.0, 0.0);
@@ -100,18 +103,19 @@ __global__ __launch_bounds__(512) void k_xc2r(XZGeom g, const double2* __restric
       const double2* row = Z + P * z_ps + (long)(z0 + r) * g.kxpad;
       v = c2r_pack(row, g.nxh, M, n, cconj(rootx[n]));
     }
-    lds[r * ss + n] = v;
+    lds[r * ss + fft_in_pos<NC>(n)] = v;
   }
   __syncthreads();
-  fft_batch<1>(lds, ss, rows, g.px, tw, tid, nthr);
+  fft_run<NC, 1, FFT_MAXV>(lds, ss, rows, g.px, tw, rl, tid, nthr);
   for (int e = tid; e < nr * M; e += nthr) {
     int r = e / M, n = e - r * M;
-    double2 v = lds[r * ss + n];
+    double2 v = lds[r * ss + fft_out_pos<NC>(n)];
     double* o = out + P * out_ps + (long)(z0 + r) * g.nx2 + 2 * n;
     *reinterpret_cast<double2*>(o) = v;
   }
 }
 
+template <int NC>
 __global__ __launch_bounds__(512) void k_xr2c(XZGeom g, const double2* __restrict__ tw,
                                                const double* __restrict__ fy,
                                                const double* __restrict__ fx,
@@ -119,12 +123,14 @@ __global__ __launch_bounds__(512) void k_xr2c(XZGeom g, const double2* __restric
                                                double2* __restrict__ out, long out_ps, int rows) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int M = g.nx2 / 2, ss = M + 1;
+  const int M = NC > 0 ? NC : g.nx2 / 2, ss = NC > 0 ? fft_seq_cells(NC) : M + 1;
   const long P = blockIdx.y;
   const int z0 = blockIdx.x * rows;
   const int nr = min(rows, g.nz2 - z0);
   const int nf = fx ? 2 : 1;
   const double2* rootx = tw + g.twx;
+  double2* rl = lds + nf * rows * ss;
+  load_tw2<NC>(rl, tw + g.px.tw, tid, nthr);
   for (int e = tid; e < nf * rows * M; e += nthr) {
     int q = e / M, n = e - q * M;
     int f = q / rows, r = q - f * rows;
@@ -133,16 +139,25 @@ __global__ __launch_bounds__(512) void k_xr2c(XZGeom g, const double2* __restric
       const double* src = (f == 0 ? fy : fx) + P * in_ps + (long)(z0 + r) * g.nx2;
       v = *reinterpret_cast<const double2*>(src + 2 * n);
     }
-    lds[q * ss + n] = v;
+    lds[q * ss + fft_in_pos<NC>(n)] = v;
   }
   __syncthreads();
-  fft_batch<-1>(lds, ss, nf * rows, g.px, tw, tid, nthr);
+  fft_run<NC, -1, FFT_MAXV>(lds, ss, nf * rows, g.px, tw, rl, tid, nthr);
+  // X_k = E_k + w^-k O_k from the half-length transform (r2c_post), read
+  // through the two-pass output order
+  auto post = [&](const double2* Zs, int k, double2 wk) {
+    double2 a = Zs[fft_out_pos<NC>(k % M)], b = cconj(Zs[fft_out_pos<NC>((M - k) % M)]);
+    double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    double2 d = csub(a, b);
+    double2 O = make_double2(0.5 * d.y, -0.5 * d.x);
+    return cadd(E, cmul(wk, O));
+  };
   for (int e = tid; e < nr * g.nxh; e += nthr) {
     int r = e / g.nxh, k = e - r * g.nxh;
     double2 wk = rootx[k];
-    double2 X = r2c_post(lds + r * ss, M, k, wk);
+    double2 X = post(lds + r * ss, k, wk);
     if (fx) {
-      double2 Xx = r2c_post(lds + (rows + r) * ss, M, k, wk);
+      double2 Xx = post(lds + (rows + r) * ss, k, wk);
       double kk = kxv[k];
       X = make_double2(X.x - kk * Xx.y, X.y + kk * Xx.x);
     }
@@ -412,10 +427,13 @@ void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps
   const int M = g.nx2 / 2;
   int rows = rows_for(M, g.nz2, 1);
   int nthr = std::min(512, std::max(64, ((rows * M + 63) / 64) * 64));
-  check_stage_fit(g.px, rows, nthr);
+  const bool ct = fft2_split(M).n1 > 0;
+  if (!ct) check_stage_fit(g.px, rows, nthr);
   dim3 grid(g.nz2 / rows, nplanes);
-  size_t lds = sizeof(double2) * rows * (M + 1);
-  hipLaunchKernelGGL(k_xc2r, grid, dim3(nthr), lds, s, g, tw, Z, z_ps, out, out_ps, rows);
+  size_t lds = sizeof(double2) * (ct ? rows * fft_seq_cells(M) + M : rows * (M + 1));
+#define XC2R_CASE(N) hipLaunchKernelGGL(k_xc2r<N>, grid, dim3(nthr), lds, s, g, tw, Z, z_ps, out, out_ps, rows)
+  FFT_DISPATCH(M, XC2R_CASE)
+#undef XC2R_CASE
   LAUNCH_CHECK("k_xc2r");
 }
 
@@ -425,10 +443,14 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
   const int nf = fx ? 2 : 1;
   int rows = rows_for(M, g.nz2, nf);
   int nthr = std::min(512, std::max(64, ((nf * rows * M + 63) / 64) * 64));
-  check_stage_fit(g.px, nf * rows, nthr);
+  const bool ct = fft2_split(M).n1 > 0;
+  if (!ct) check_stage_fit(g.px, nf * rows, nthr);
   dim3 grid(g.nz2 / rows, nplanes);
-  size_t lds = sizeof(double2) * nf * rows * (M + 1);
-  hipLaunchKernelGGL(k_xr2c, grid, dim3(nthr), lds, s, g, tw, fy, fx, kx, in_ps, out, out_ps, rows);
+  size_t lds = sizeof(double2) * (ct ? nf * rows * fft_seq_cells(M) + M : nf * rows * (M + 1));
+#define XR2C_CASE(N) \
+  hipLaunchKernelGGL(k_xr2c<N>, grid, dim3(nthr), lds, s, g, tw, fy, fx, kx, in_ps, out, out_ps, rows)
+  FFT_DISPATCH(M, XR2C_CASE)
+#undef XR2C_CASE
   LAUNCH_CHECK("k_xr2c");
 }
 

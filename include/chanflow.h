@@ -262,7 +262,8 @@ int chan_mlp_train(chan_mlp* net, int n, const float* x, const float* t, int epo
  * physical grid, layout double [6][ny+1][nz2][nx2] (components 11, 12, 13,
  * 22, 23, 33), zero on the wall planes.
  * Modes: 0 off; 1 dynamic Smagorinsky (Germano identity, sharp test filter at
- * half the resolved kx and kz, plane least squares, clipped at 0); 2 learned
+ * half the resolved kx and kz, plane least squares, clipped at 0; C(y) is
+ * evaluated at the first substep of each time step); 2 learned
  * closure `net` with a fallback to dynamic Smagorinsky wherever the learned
  * stress's energy transfer -tau:S is below -beta times Smagorinsky's or not
  * finite; 3 prescribed tau (chan_les_set_tau).  Needs chan_set_coordinates.
