@@ -83,6 +83,50 @@ def cpu_baseline(c, U0, nsteps=2):
     return nsteps / (t1 - t0), t1 - t0
 
 
+LES_CFG = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.01)
+
+
+def les_rate(cf, synth, c, steps):
+    """RK3 steps/s of the 64 x 65 x 64 LES (same box and Reynolds number) with
+    the trained learned closure and its dynamic-Smagorinsky gate, and the
+    fraction of points that fell back at the last evaluation."""
+    import math
+    pi = math.atan2(0.0, -1.0)
+    lx = c["lx_over_pi"] * pi
+    lz = lx / c["aspect"]
+    L = LES_CFG
+    ch = cf.Channel(L["nx"], L["ny"], L["nz"], L["nx2"], L["nz2"], lx, lz, c["re"], fix_mass=1, dt_flag=0)
+    y, yh = ch.build_operators(c["stretch"])
+    params = np.load(os.path.join(PKG_DIR, "les_net_64x65x64.npy"), allow_pickle=False)
+    net = cf.Mlp(params)
+    U0 = synth.turbulent_like(L["nx"], L["ny"], L["nz"], yh, y, seed=c["seed"], rms=c["rms"], re_tau=c["re_tau"])
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=L["dt"], t_total=0.0, p_grad=-2.0 / c["re"])
+    ch.les_configure("learned", net=net, beta=1.0)
+    ch.initialize_flow()
+    ch.step(5, step0=1)
+    ch.sync()
+    t0 = time.perf_counter()
+    ch.step(steps, step0=6)
+    ch.sync()
+    t1 = time.perf_counter()
+    g, n = ch.les_gated_points()
+    ch.close()
+    net.close()
+    return steps / (t1 - t0), g / n
+
+
+def les_eval_results():
+    """Errors of the committed filtered-DNS evaluation (les_eval.py)."""
+    path = os.path.join(ROOT, "profiles", "les_eval_64x65x64.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    if "```json" not in txt:
+        return None
+    return json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +134,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--les-steps", type=int, default=200,
+                    help="RK3 steps of the learned-closure LES timed for the LES speed-up (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprof), print no JSON")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -164,6 +210,21 @@ def main():
     step_gbs = alg_step * steps_per_s / 1e9
     traffic, traffic_src = measured_traffic(name)
 
+    les = None
+    if args.les_steps > 0:
+        les_sps, fb = les_rate(cf, synth, c, args.les_steps)
+        dns_tu = steps_per_s * c["dt"]
+        les_tu = les_sps * LES_CFG["dt"]
+        ev = les_eval_results()
+        les = {"metric": "learned-LES speed-up (simulated time per wall second, LES / DNS)",
+               "config": "LES 64x65x64 (96x96 dealiased), dt 0.01, learned closure with the dynamic-"
+                         "Smagorinsky gate (beta 1) vs the 192x129x192 DNS above, same box and Re",
+               "les_steps_per_s": les_sps, "les_time_units_per_s": les_tu, "dns_time_units_per_s": dns_tu,
+               "speedup": les_tu / dns_tu, "fallback_fraction": fb,
+               "error": None if ev is None else ev.get("learned_error"),
+               "error_source": None if ev is None else "profiles/les_eval_64x65x64.md (les_eval.py, "
+                                                       "a posteriori against the filtered DNS)"}
+
     cpu = None
     if not args.no_cpu_baseline:
         sps, secs = cpu_baseline(c, U0, args.cpu_steps)
@@ -197,6 +258,7 @@ def main():
         "kernel_launches_measured": {k: v[1] for k, v in kt.items()},
         "max_divergence": div,
         "cpu_baseline": cpu,
+        "les": les,
     }
     print(json.dumps(out), flush=True)
 
