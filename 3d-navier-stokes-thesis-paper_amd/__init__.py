@@ -141,7 +141,7 @@ def load_library(path: str = LIBPATH):
         "chan_mlp_train": (C.c_int, [vp, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_double, _dp]),
         "chan_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
         "chan_write_vtk": (C.c_int, [vp, C.c_char_p, _dp]),
-        "chan_les_configure": (C.c_int, [vp, C.c_int, vp, C.c_double]),
+        "chan_les_configure": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_int]),
         "chan_les_update": (C.c_int, [vp]),
         "chan_les_set_tau": (C.c_int, [vp, _dp]),
         "chan_les_get_tau": (C.c_int, [vp, _dp]),
@@ -260,12 +260,14 @@ class Channel:
     # -- subgrid closure (LES)
     LES_MODES = {"off": 0, "dynamic": 1, "learned": 2, "prescribed": 3}
 
-    def les_configure(self, mode, net=None, beta=1.0):
+    def les_configure(self, mode, net=None, beta=1.0, every_substep=True):
         """mode: 'off', 'dynamic' (dynamic Smagorinsky), 'learned' (Mlp `net`
-        with the Smagorinsky fallback gate) or 'prescribed' (les_set_tau)."""
+        with the Smagorinsky fallback gate) or 'prescribed' (les_set_tau);
+        every_substep=False evaluates the stress once per step."""
         m = self.LES_MODES[mode] if isinstance(mode, str) else int(mode)
         self._les_net = net
-        self._chk(self.lib.chan_les_configure(self.h, m, net.h if net is not None else None, float(beta)))
+        self._chk(self.lib.chan_les_configure(self.h, m, net.h if net is not None else None, float(beta),
+                                              1 if every_substep else 0))
 
     def les_update(self):
         self._chk(self.lib.chan_les_update(self.h))

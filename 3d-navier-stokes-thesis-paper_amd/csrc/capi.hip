@@ -327,6 +327,7 @@ void prepare_capture(chan_ctx* c) {
   ensure_vel_in_vel(c);
   if (c->prm.ctrlu != 0.0 || c->prm.ctrlw != 0.0) (void)stage(c, sizeof(double) * 6 * (size_t)c->gd.nz2 * c->gd.nx2);
   if (c->timing) throw chan_error("disable timing before batching or tuning");
+  les_prepare_capture(c);
   CHAN_CHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -1259,7 +1260,6 @@ int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int line
   try {
     for (int i = 0; i < n; ++i) {
       if (!members[i]) throw chan_error("null ensemble member");
-      if (members[i]->les) throw chan_error("ensemble members run without a subgrid closure");
       for (int k = 0; k < i; ++k)
         if (members[k] == members[i]) throw chan_error("ensemble member given twice");
       e->m.push_back(members[i]);

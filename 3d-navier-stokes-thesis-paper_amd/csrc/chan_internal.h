@@ -245,6 +245,7 @@ void les_prepare_state(chan_ctx* c);    // the velocity (not the NL term) is in 
 // subgrid closure (k_les.hip)
 void les_destroy(LesState* s);
 const double* les_tau_for_pass(chan_ctx* c, int rk);  // updates and returns tau, nullptr when off
+void les_prepare_capture(chan_ctx* c);  // everything the closure may synchronise for, before a capture
 
 // learned closure network (k_mlp.hip)
 void mlp_prepare_device(chan_mlp* m);   // packs the bf16 weights, synchronised

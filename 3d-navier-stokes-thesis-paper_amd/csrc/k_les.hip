@@ -26,6 +26,8 @@
 //      the dynamic Smagorinsky rate, or is not finite, takes the Smagorinsky
 //      stress instead (the fallback gate);
 //   3  prescribed tau (chan_les_set_tau).
+// The stress is evaluated at every substep, or (every_substep = 0) at the
+// first substep of each step and held for the other two.
 #include "chan_internal.h"
 #include <cmath>
 
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(256) void k_les_tau_mlp(int ny, long ps, const doub
       ts[t] = c * S.s[t];
     }
     const double em = -contract(tm, S.s), es = -contract(ts, S.s);
-    fb = !(em >= -beta * es) || !isfinite(em);
+    fb = !isfinite(em) || (beta >= 0.0 && !(em >= -beta * es));
     const long cs = (long)nyp * ps;
     double* o = tau + (long)kk * ps + i;
     for (int t = 0; t < 6; ++t) o[t * cs] = fb ? ts[t] : tm[t];
@@ -293,6 +295,8 @@ struct LesState {
   long ngated = 0;
   bool fresh = false;          // tau is current for the state (mode 3: set by the caller)
   bool cs2_valid = false;      // C(y) computed for the current step
+  int every = 1;               // 1: tau at every substep; 0: at the first substep of a step only
+  bool tau_valid = false;
 };
 
 namespace {
@@ -396,17 +400,24 @@ const double* les_tau_for_pass(chan_ctx* c, int rk) {
     if (!L->fresh) throw chan_error("prescribed subgrid stress not set (chan_les_set_tau)");
     return L->tau;
   }
-  les_compute(c, L, rk == 1);
+  if (L->every || rk == 1 || !L->tau_valid) {
+    les_compute(c, L, rk == 1);
+    L->tau_valid = true;
+  }
   return L->tau;
+}
+
+void les_prepare_capture(chan_ctx* c) {
+  if (c->les && c->les->mode == 2) mlp_prepare_device(c->les->net);
 }
 
 extern "C" {
 
-int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta) {
+int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta, int every_substep) {
   return guarded_les(c, [&] {
     if (mode < 0 || mode > 3) throw chan_error("LES mode must be 0 (off), 1, 2 or 3");
     if (mode == 2 && !net) throw chan_error("the learned closure needs a chan_mlp");
-    if (!(beta >= 0.0)) throw chan_error("gate beta must be >= 0");
+    if (beta != beta) throw chan_error("gate beta must be a number (negative: no energy gate)");
     check_ops_les(c);
     if (mode == 0) {
       les_destroy(c->les);
@@ -424,6 +435,7 @@ int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta) {
     const int ny = c->prm.ny, nyp = c->nyp;
     L->mode = mode;
     L->beta = beta;
+    L->every = every_substep ? 1 : 0;
     L->net = net;
     L->kxc = c->prm.nx / 4;
     L->kzc = c->prm.nz / 4;

@@ -7,8 +7,10 @@ reference, which has no working LES path).
    stretch 2.75, dt 0.005) from the seeded synthetic field, run `--spinup`
    steps, then snapshots every `--every` steps.
 2. Filter to the LES grid 64 x 65 x 64 (3/2 dealiased to 96 x 96, same box and
-   stretching): a sharp cut-off to the LES modes (the LES Nyquist modes
-   dropped) and, in y, the finite-volume restriction of the staggered grid:
+   stretching): a box filter over 3 DNS points in x and z (the 3-point
+   average, transfer (1 + 2 cos(k dx))/3), a cut-off to the LES modes (the LES
+   Nyquist modes dropped) and, in y, the finite-volume restriction of the
+   staggered grid:
    the tanh grid of 65 faces is every other face of the 129-face grid, so u
    and w are averaged over the two fine cells of a coarse cell (weights
    y(k)-y(k-1)) and v is taken at the shared faces.  The restriction keeps the
@@ -18,18 +20,22 @@ reference, which has no working LES path).
    cell averages in y) applied to fields at the DNS cell centres (v averaged
    to them from its faces).
 4. Training set: (network inputs, tau / (D^2 |du/dx|^2)) at every interior LES
-   point of the first snapshots; the network is trained on the GPU (float32
+   point of 80 % of the snapshots; the network is trained on the GPU (float32
    Adam) on targets standardised per component, and the standard deviations
    are folded into its last layer.
-5. A priori (held-out snapshot): correlation and relative L2 error of the
+5. A priori (the held-out 20 %): correlation and relative L2 error of the
    deviatoric stress and of the plane-averaged subgrid dissipation
-   <-tau:S>(y), for the learned closure (with its gate) and for dynamic
-   Smagorinsky.
+   <-tau:S>(y), for the learned closure (with its gate), dynamic Smagorinsky
+   and no model.  The learned closure is kept only when its a-priori error is
+   below the threshold (default: dynamic Smagorinsky's own error); otherwise
+   dynamic Smagorinsky is the closure of the a-posteriori run.
 6. A posteriori: LES runs from the filtered last snapshot (no model, dynamic
    Smagorinsky, learned) against the DNS continued over the same time, both
    sampled every `--sample` time units: relative L2 errors of the
-   time-averaged mean profile and of the rms profiles of the filtered DNS,
-   and the speed-up in simulated time per wall second.
+   time-averaged mean profile, the rms profiles and the Reynolds shear stress
+   <u'v'> of the filtered DNS, and the speed-up in simulated time per wall
+   second.  Both runs step through a captured one-member ensemble (graph
+   replay); the LES evaluates its closure once per time step.
 """
 import argparse
 import importlib.util
@@ -79,9 +85,24 @@ def kz_map(nz_from, nz_to):
     return src, keep
 
 
-def cutoff(C, nxh_to, nz_to):
-    """Sharp cut-off [..., nxh_from, nz_from] -> [..., nxh_to, nz_to] without the
-    coarse Nyquist modes."""
+def box3(C):
+    """3-point box filter in x and z of spectral fields [..., nxh, nz]:
+    transfer (1 + 2 cos(k dx)) / 3 per direction on the grid of the fields."""
+    nxh, nz = C.shape[-2], C.shape[-1]
+    nx = 2 * (nxh - 1)
+    ix = np.arange(nxh)
+    iz = np.arange(nz)
+    kzi = np.where(iz <= nz // 2, iz, iz - nz)
+    tx = (1.0 + 2.0 * np.cos(2.0 * PI * ix / nx)) / 3.0
+    tz = (1.0 + 2.0 * np.cos(2.0 * PI * kzi / nz)) / 3.0
+    return C * (tx[:, None] * tz[None, :])
+
+
+def cutoff(C, nxh_to, nz_to, box=True):
+    """(Box filter, then) cut-off [..., nxh_from, nz_from] -> [..., nxh_to, nz_to]
+    without the coarse Nyquist modes."""
+    if box:
+        C = box3(C)
     src, keep = kz_map(C.shape[-1], nz_to)
     out = C[..., :nxh_to, :][..., src] * keep
     out[..., nxh_to - 1, :] = 0.0
@@ -191,7 +212,8 @@ def apriori(tau_m, tau_x, Ph, ny):
 
 
 def profiles(d1):
-    """Mean u(y) and rms u', v', w' per plane from spectral data1 (Parseval)."""
+    """Mean u(y), rms u', v', w' and <u'v'> (v averaged to the cell centres)
+    per plane from spectral data1 (Parseval)."""
     nxh = d1.shape[2]
     w = np.full(nxh, 2.0)
     w[0] = 1.0
@@ -200,7 +222,11 @@ def profiles(d1):
     e[0] -= mean ** 2
     e[1] -= np.abs(d1[1, :, 0, 0]) ** 2
     e[2] -= np.abs(d1[2, :, 0, 0]) ** 2
-    return mean, np.sqrt(np.maximum(e, 0.0))
+    vc = np.zeros_like(d1[1])
+    vc[1:] = 0.5 * (d1[1, :-1] + d1[1, 1:])
+    uv = (np.real(d1[0] * np.conj(vc)) * w[None, :, None]).sum(axis=(1, 2)) - \
+        np.real(d1[0, :, 0, 0] * np.conj(vc[:, 0, 0]))
+    return mean, np.sqrt(np.maximum(e, 0.0)), uv
 
 
 def train_net(cf, X, T, epochs, batch, lr, seed=0):
@@ -229,24 +255,31 @@ def train_net(cf, X, T, epochs, batch, lr, seed=0):
     return net, losses, t_train, sd
 
 
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--spinup", type=int, default=3000)
-    ap.add_argument("--every", type=int, default=200)
-    ap.add_argument("--ntrain", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--spinup", type=int, default=4000)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--nsnap", type=int, default=10, help="snapshots; 80 %% train, 20 %% test")
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--post-time", type=float, default=10.0, help="a-posteriori horizon (time units)")
-    ap.add_argument("--sample", type=float, default=0.5, help="a-posteriori sampling interval (time units)")
+    ap.add_argument("--post-time", type=float, default=20.0, help="a-posteriori horizon (time units)")
+    ap.add_argument("--sample", type=float, default=1.0, help="a-posteriori sampling interval (time units)")
     ap.add_argument("--beta", type=float, default=1.0)
+    ap.add_argument("--threshold", type=float, default=-1.0,
+                    help="a-priori error above which dynamic Smagorinsky replaces the learned closure "
+                         "(default: dynamic Smagorinsky's own a-priori error)")
     ap.add_argument("--out", default="les_eval.json")
     ap.add_argument("--params-out", default="")
     args = ap.parse_args()
 
     cf = load_pkg()
     import chanflow.synth as synth
-    res = {"dns": DNS, "les": LES, "box": BOX}
+    res = {"dns": DNS, "les": LES, "box": BOX, "filter": "3-point box in x and z, cut-off, y cell average"}
 
     def log(msg):
         print(f"[les_eval] {msg}", flush=True)
@@ -258,21 +291,22 @@ def main():
     dns.set_state(data1=U0)
     dns.set_scalars(dt=DNS["dt"], t_total=0.0, p_grad=-2.0 / BOX["re"])
     dns.initialize_flow()
+    dens = cf.Ensemble([dns], nstreams=1)
     t0 = time.perf_counter()
-    dns.step(args.spinup, step0=1)
+    dens.step(args.spinup)
     dns.sync()
     res["dns_spinup_s"] = time.perf_counter() - t0
     log(f"DNS spin-up {args.spinup} steps in {res['dns_spinup_s']:.1f} s")
-    step = args.spinup + 1
 
-    # ---- snapshots: training inputs / targets
-    nsnap = args.ntrain + 1
-    Xs, Ts, held = [], [], None
-    les.les_configure("learned", net=cf.Mlp(), beta=args.beta)   # placeholder network for the inputs
-    for s in range(nsnap):
-        if s:
-            dns.step(args.every, step0=step)
-            step += args.every
+    # ---- snapshots: inputs / targets; the last 20 % are held out
+    ntest = max(1, int(round(0.2 * args.nsnap)))
+    ntrain = args.nsnap - ntest
+    Xs, Ts, held = [], [], []
+    placeholder = cf.Mlp()
+    les.les_configure("learned", net=placeholder, beta=args.beta)
+    for s_ in range(args.nsnap):
+        if s_:
+            dens.step(args.every)
         d1 = dns.get_data1()
         Ul, hh_f = filter_velocity(d1, y_f, LES["ny"], les.nxh, les.nz)
         les.set_state(data1=Ul)
@@ -282,89 +316,94 @@ def main():
         ok = sc > 0
         T = np.zeros_like(tx)
         T[ok] = tx[ok] / sc[ok, None]
-        if s < args.ntrain:
+        if s_ < ntrain:
             Xs.append(X[ok])
-            Ts.append(T[ok])
+            Ts.append(T[ok].astype(np.float32))
         else:
-            held = (Ul, tau_x, Ph, d1)
-        log(f"snapshot {s}: {X.shape[0]} points, |tau| rms {np.sqrt((tau_x ** 2).mean()):.3e}")
+            held.append((Ul, tau_x, Ph))
+        log(f"snapshot {s_}: {X.shape[0]} points, |tau| rms {np.sqrt((tau_x ** 2).mean()):.3e}")
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
+    del Xs, Ts
     net, losses, t_train, sd = train_net(cf, X, T, args.epochs, args.batch, args.lr)
-    res["training"] = {"points": int(X.shape[0]), "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
-                       "loss_per_epoch_standardised": [float(v) for v in losses], "seconds": t_train,
-                       "target_std": [float(v) for v in sd]}
+    res["training"] = {"snapshots": ntrain, "points": int(X.shape[0]), "epochs": args.epochs, "batch": args.batch,
+                       "lr": args.lr, "loss_per_epoch_standardised": [float(v) for v in losses],
+                       "seconds": t_train, "target_std": [float(v) for v in sd]}
     log(f"trained on {X.shape[0]} points in {t_train:.1f} s, losses {losses}")
+    del X, T
     if args.params_out:
         np.save(args.params_out, net.get_params())
 
-    # ---- a priori on the held-out snapshot
-    Ul, tau_x, Ph, d1 = held
-    les.les_configure("dynamic")
-    les.set_state(data1=Ul)
-    les.les_update()
-    tau_dyn = les.les_tau()
-    les.les_configure("learned", net=net, beta=args.beta)
-    les.set_state(data1=Ul)
-    les.les_update()
-    tau_ml = les.les_tau()
-    gated, total = les.les_gated_points()
-    res["apriori"] = {"dynamic_smagorinsky": apriori(tau_dyn, tau_x, Ph, LES["ny"]),
-                      "learned": apriori(tau_ml, tau_x, Ph, LES["ny"]),
-                      "learned_fallback_fraction": gated / total}
-    log(f"a priori: {json.dumps(res['apriori'])}")
+    # ---- a priori on the held-out snapshots
+    acc = {"dynamic_smagorinsky": [], "learned": [], "no_model": []}
+    fb = []
+    for Ul, tau_x, Ph in held:
+        les.les_configure("dynamic")
+        les.set_state(data1=Ul)
+        les.les_update()
+        acc["dynamic_smagorinsky"].append(apriori(les.les_tau(), tau_x, Ph, LES["ny"]))
+        les.les_configure("learned", net=net, beta=args.beta)
+        les.set_state(data1=Ul)
+        les.les_update()
+        acc["learned"].append(apriori(les.les_tau(), tau_x, Ph, LES["ny"]))
+        g, n = les.les_gated_points()
+        fb.append(g / n)
+        acc["no_model"].append(apriori(np.zeros_like(tau_x), tau_x, Ph, LES["ny"]))
+    pri = {k: {q: float(np.mean([e[q] for e in v])) for q in v[0]} for k, v in acc.items()}
+    pri["learned_fallback_fraction"] = float(np.mean(fb))
+    pri["test_snapshots"] = len(held)
+    thr = args.threshold if args.threshold >= 0 else pri["dynamic_smagorinsky"]["rel_l2_deviatoric"]
+    choice = "learned" if pri["learned"]["rel_l2_deviatoric"] <= thr else "dynamic"
+    pri["threshold"] = thr
+    pri["closure_choice"] = choice
+    res["apriori"] = pri
+    log(f"a priori: {json.dumps(pri)}")
 
-    # ---- a posteriori: LES runs against the continued DNS
+    # ---- a posteriori: LES runs against the continued DNS (graph replay for both)
     nsamp = int(round(args.post_time / args.sample))
     k_dns = int(round(args.sample / DNS["dt"]))
     k_les = int(round(args.sample / LES["dt"]))
-    Ul0 = Ul
-    hh = np.empty(DNS["ny"] + 1)
-    hh[1:DNS["ny"]] = y_f[1:] - y_f[:-1]
-    hh[0] = hh[DNS["ny"]] = 0.0
+    Ul0 = held[-1][0]
     dns.sync()
-    mean_d, rms_d = [], []
+    acc_d = []
     t_dns = 0.0
     for i in range(nsamp):
         t0 = time.perf_counter()
-        dns.step(k_dns, step0=step)
+        dens.step(k_dns)
         dns.sync()
         t_dns += time.perf_counter() - t0
-        step += k_dns
         Uf, _ = filter_velocity(dns.get_data1(), y_f, LES["ny"], les.nxh, les.nz)
-        m, r = profiles(Uf)
-        mean_d.append(m)
-        rms_d.append(r)
-    mean_d, rms_d = np.mean(mean_d, axis=0), np.mean(rms_d, axis=0)
+        acc_d.append(profiles(Uf))
+    mean_d, rms_d, uv_d = [np.mean([a[j] for a in acc_d], axis=0) for j in range(3)]
     post = {"time_units": nsamp * args.sample, "dns_seconds": t_dns,
             "dns_time_units_per_s": nsamp * args.sample / t_dns}
     for mode in ("off", "dynamic", "learned"):
-        les.les_configure(mode, net=net if mode == "learned" else None, beta=args.beta)
+        les.les_configure(mode, net=net if mode == "learned" else None, beta=args.beta, every_substep=False)
         les.set_state(data1=Ul0)
         les.set_scalars(dt=LES["dt"], t_total=0.0, p_grad=-2.0 / BOX["re"])
         les.initialize_flow()
+        lens = cf.Ensemble([les], nstreams=1)
+        lens.step(1)
         les.sync()
-        mean_l, rms_l = [], []
+        acc_l = []
         t_les = 0.0
-        s0 = 1
         for i in range(nsamp):
             t0 = time.perf_counter()
-            les.step(k_les, step0=s0)
+            lens.step(k_les if i else k_les - 1)
             les.sync()
             t_les += time.perf_counter() - t0
-            s0 += k_les
-            m, r = profiles(les.get_data1())
-            mean_l.append(m)
-            rms_l.append(r)
-        mean_l, rms_l = np.mean(mean_l, axis=0), np.mean(rms_l, axis=0)
+            acc_l.append(profiles(les.get_data1()))
+        lens.close()
+        mean_l, rms_l, uv_l = [np.mean([a[j] for a in acc_l], axis=0) for j in range(3)]
         k = slice(1, LES["ny"])
         ent = {
-            "rel_err_mean_profile": float(np.linalg.norm(mean_l[k] - mean_d[k]) / np.linalg.norm(mean_d[k])),
-            "rel_err_rms_u": float(np.linalg.norm(rms_l[0, k] - rms_d[0, k]) / np.linalg.norm(rms_d[0, k])),
-            "rel_err_rms_v": float(np.linalg.norm(rms_l[1, k] - rms_d[1, k]) / np.linalg.norm(rms_d[1, k])),
-            "rel_err_rms_w": float(np.linalg.norm(rms_l[2, k] - rms_d[2, k]) / np.linalg.norm(rms_d[2, k])),
+            "rel_err_mean_profile": rel(mean_l[k], mean_d[k]),
+            "rel_err_rms_u": rel(rms_l[0, k], rms_d[0, k]),
+            "rel_err_rms_v": rel(rms_l[1, k], rms_d[1, k]),
+            "rel_err_rms_w": rel(rms_l[2, k], rms_d[2, k]),
+            "rel_err_uv": rel(uv_l[k], uv_d[k]),
             "les_seconds": t_les,
-            "les_time_units_per_s": nsamp * args.sample / t_les,
+            "les_time_units_per_s": (nsamp * args.sample - LES["dt"]) / t_les,
         }
         ent["speedup_vs_dns"] = ent["les_time_units_per_s"] / post["dns_time_units_per_s"]
         if mode == "learned":
@@ -373,6 +412,7 @@ def main():
         post[mode] = ent
         log(f"a posteriori {mode}: {json.dumps(ent)}")
     res["aposteriori"] = post
+    dens.close()
     with open(args.out, "w") as f:
         json.dump(res, f, indent=1)
     log(f"wrote {args.out}")

@@ -266,14 +266,17 @@ int chan_mlp_train(chan_mlp* net, int n, const float* x, const float* t, int epo
  * evaluated at the first substep of each time step); 2 learned
  * closure `net` with a fallback to dynamic Smagorinsky wherever the learned
  * stress's energy transfer -tau:S is below -beta times Smagorinsky's or not
- * finite; 3 prescribed tau (chan_les_set_tau).  Needs chan_set_coordinates.
+ * finite (beta < 0: only the finiteness check); 3 prescribed tau
+ * (chan_les_set_tau).  With every_substep = 1 the
+ * stress is evaluated at every substep, with 0 at the first substep of each
+ * step and held for the other two.  Needs chan_set_coordinates.
  * chan_les_update evaluates the closure on the current state;
  * chan_les_get_coefficient returns C(y) [ny+1]; chan_les_get_fields the
  * physical closure inputs [12][ny+1][nz2][nx2] (du_i/dx_j row-major, then u,
  * v, w at yh); chan_les_features the network inputs [(ny-1) nz2 nx2][10] and
  * the stress scale D^2 |du/dx|^2 per point (mode 2); chan_les_gated_points the
  * number of points of the last evaluation that fell back, and the total. */
-int chan_les_configure(chan_ctx* ctx, int mode, chan_mlp* net, double beta);
+int chan_les_configure(chan_ctx* ctx, int mode, chan_mlp* net, double beta, int every_substep);
 int chan_les_update(chan_ctx* ctx);
 int chan_les_set_tau(chan_ctx* ctx, const double* tau);
 int chan_les_get_tau(chan_ctx* ctx, double* tau);

@@ -286,8 +286,9 @@ def features(g, Ph):
 
 def tau_learned(g, Ph, Y, scale, cs2, beta=1.0):
     """Learned stress tau = scale * Y with the fallback gate: a point whose
-    energy transfer -tau:S is below -beta times Smagorinsky's, or is not
-    finite, takes the Smagorinsky stress.  Returns (tau, fallback count)."""
+    energy transfer -tau:S is below -beta times Smagorinsky's (beta >= 0), or
+    is not finite, takes the Smagorinsky stress.  Returns (tau, fallback
+    count)."""
     ny = g.ny
     s, mag = strain(Ph[:9])
     ts = tau_smagorinsky(g, Ph, cs2)
@@ -297,7 +298,9 @@ def tau_learned(g, Ph, Y, scale, cs2, beta=1.0):
     em = -contract(tm, si)
     es = -contract(ts[:, 1:ny], si)
     with np.errstate(invalid="ignore"):
-        fb = ~(em >= -beta * es) | ~np.isfinite(em)
+        fb = ~np.isfinite(em)
+        if beta >= 0.0:
+            fb = fb | ~(em >= -beta * es)
     tau = np.zeros_like(ts)
     tau[:, 1:ny] = np.where(fb[None], ts[:, 1:ny], tm)
     return tau, int(fb.sum())

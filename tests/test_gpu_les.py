@@ -127,3 +127,52 @@ def test_les_dynamic_steps(g0):
         assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
     assert ch.divergence_max() <= 1e-12
     ch.close()
+
+
+def test_trained_network_gains_over_untrained_and_smagorinsky(g0):
+    """What training buys, on a target the inputs determine: the gradient
+    (Clark) model tau_ij = D^2 sum_k g_ik g_jk / 12, i.e. tau / (D^2 |G|^2) =
+    sum_k x_ik x_jk / 12 of the normalised inputs.  After a few epochs of the
+    GPU training kernels the learned stress (gate off) correlates with it far
+    better than the untrained network and than dynamic Smagorinsky."""
+    cf = chanflow()
+    U = init_cfg0(O, g0, seed=15, amp=0.2)
+    ch = make_channel(g0)
+    ch.set_state(data1=U)
+    W, b = L.init_params(seed=5)
+    net = cf.Mlp(cf.Mlp.flatten(W, b))
+    ch.les_configure("learned", net=net, beta=-1.0)   # energy gate off
+    X, sc = ch.les_features()
+    xg = X[:, :9].astype(np.float64).reshape(-1, 3, 3)
+    Tn = np.stack([(xg[:, i, :] * xg[:, j, :]).sum(axis=1) / 12.0 for i, j in L.SYM], axis=1)
+    tau_ref = np.zeros((6, g0.ny + 1, g0.nz2, g0.nx2))
+    tau_ref[:, 1:g0.ny] = (Tn * sc[:, None]).T.reshape(6, g0.ny - 1, g0.nz2, g0.nx2)
+
+    def corr12(t):
+        return np.corrcoef(t[1, 1:g0.ny].ravel(), tau_ref[1, 1:g0.ny].ravel())[0, 1]
+
+    ch.les_update()
+    c_untrained = corr12(ch.les_tau())
+    sd = Tn.std(axis=0)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(X.shape[0])
+    Xs = np.ascontiguousarray(X[perm])
+    Ts = np.ascontiguousarray((Tn[perm] / sd).astype(np.float32))
+    for e in range(6):
+        net.train(Xs, Ts, epochs=1, batch=256, lr=2e-3 * 0.5 ** (e // 2))
+    p = net.get_params().astype(np.float64)
+    off = 10 * 128 + 128 + 2 * (128 * 128 + 128)
+    p[off:off + 6 * 128] = (p[off:off + 6 * 128].reshape(6, 128) * sd[:, None]).ravel()
+    p[off + 6 * 128:off + 6 * 128 + 6] *= sd
+    net.set_params(p.astype(np.float32))
+    ch.les_update()
+    c_trained = corr12(ch.les_tau())
+    dyn = make_channel(g0)
+    dyn.set_state(data1=U)
+    dyn.les_configure("dynamic")
+    dyn.les_update()
+    c_dyn = corr12(dyn.les_tau())
+    assert c_trained > 0.9, (c_trained, c_untrained, c_dyn)
+    assert c_trained > abs(c_untrained) + 0.3, (c_trained, c_untrained)
+    assert c_trained > c_dyn + 0.05, (c_trained, c_dyn)
+    ch.close(); dyn.close(); net.close()
