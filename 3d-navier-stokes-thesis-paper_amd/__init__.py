@@ -56,7 +56,7 @@ ENTRY_POINTS = (
     "chan_mlp_get_params", "chan_mlp_infer", "chan_mlp_forward_fp32", "chan_mlp_gradient", "chan_mlp_train",
     "chan_vtk_write", "chan_write_vtk",
     "chan_les_configure", "chan_les_update", "chan_les_set_tau", "chan_les_get_tau", "chan_les_get_coefficient",
-    "chan_les_get_fields", "chan_les_features", "chan_les_gated_points",
+    "chan_les_get_fields", "chan_les_features", "chan_les_gated_points", "chan_jet_configure",
 )
 
 _lib = None
@@ -149,6 +149,8 @@ def load_library(path: str = LIBPATH):
         "chan_les_get_fields": (C.c_int, [vp, _dp]),
         "chan_les_features": (C.c_int, [vp, _fp, _dp]),
         "chan_les_gated_points": (C.c_int, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+        "chan_jet_configure": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -256,6 +258,12 @@ class Channel:
     def set_tuning(self, key, value):
         k = self.TUNING_KEYS.index(key) if isinstance(key, str) else int(key)
         self._chk(self.lib.chan_set_tuning(self.h, k, int(value)))
+
+    # -- wall jet
+    def jet_configure(self, wall, amplitude, radius, xjet, zjet, forcing=0.0, frequency=0.0):
+        """Gaussian jet at the bottom (wall=0) or top (wall=1) wall; amplitude 0 removes it."""
+        self._chk(self.lib.chan_jet_configure(self.h, int(wall), float(amplitude), float(radius), float(xjet),
+                                              float(zjet), float(forcing), float(frequency)))
 
     # -- subgrid closure (LES)
     LES_MODES = {"off": 0, "dynamic": 1, "learned": 2, "prescribed": 3}

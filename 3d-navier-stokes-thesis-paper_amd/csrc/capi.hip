@@ -199,6 +199,11 @@ void solve_pass(chan_ctx* c, int rk) {
   a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
   a.fix_mass = c->prm.fix_mass;
   a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl; a.dma = c->tune.solve_dma;
+  if (c->d_jet) {  // the jet overwrites the wall-normal velocity at its wall (PFC/pass2.f90:42-43)
+    const long nm = nmodes(c);
+    launch_jet_bc(c->d_sc, rk, c->jet_forcing, c->jet_freq, c->d_jet,
+                  c->d_cbc + ((c->jet_wall == 0 ? 1L : 0L) * 3 + 1) * nm, nm, c->stream);
+  }
   launch_solve(c->ops, a, c->stream);
 }
 
@@ -553,6 +558,7 @@ int chan_destroy(chan_ctx* c) {
   dfree(c->d_cbc); dfree(c->d_dudy); dfree(c->d_sc); dfree(c->d_cfl); dfree(c->d_Z); dfree(c->d_AB);
   dfree(c->d_scr); dfree(c->d_base);
   les_destroy(c->les);
+  dfree(c->d_jet);
   if (c->d_stage) (void)hipFree(c->d_stage);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -782,6 +788,44 @@ int chan_set_baseflow(chan_ctx* c, const double* base) {
     if (!c->d_base) c->d_base = (double*)dalloc(b);
     h2d(c, c->d_base, base, b);
     CHAN_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int chan_jet_configure(chan_ctx* c, int wall, double amplitude, double radius, double xjet, double zjet,
+                       double forcing, double frequency) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (wall != 0 && wall != 1) throw chan_error("jet wall must be 0 (bottom) or 1 (top)");
+    if (amplitude == 0.0) {
+      dfree(c->d_jet);
+      c->d_jet = nullptr;
+      return;
+    }
+    if (!(radius > 0.0)) throw chan_error("jet radius must be > 0");
+    const XZGeom& g = c->gs;  // the nx x nz grid of the wall profiles
+    const int nx = g.nx2, nz = g.nz2;
+    const double lx = c->prm.lx, lz = c->prm.lz, sign = (wall == 0) ? 1.0 : -1.0;
+    std::vector<double> f((size_t)nx * nz);
+    for (int z = 0; z < nz; ++z) {
+      double dz = (double)z * lz / (double)nz - zjet;
+      dz -= lz * std::floor(dz / lz + 0.5);
+      for (int x = 0; x < nx; ++x) {
+        double dx = (double)x * lx / (double)nx - xjet;
+        dx -= lx * std::floor(dx / lx + 0.5);
+        f[(size_t)z * nx + x] = sign * amplitude * std::exp(-(dx * dx + dz * dz) / (radius * radius));
+      }
+    }
+    const size_t cb = sizeof(double2) * (size_t)g.nxh * g.nz, rb = sizeof(double) * f.size();
+    if (!c->d_jet) c->d_jet = (double2*)dalloc(cb);
+    char* st = (char*)stage(c, rb);
+    h2d(c, st, f.data(), rb);
+    dev_r2c(c, g, (const double*)st, 1, c->d_jet);
+    // no net mass flux through the wall: the mean mode is removed (uniform suction)
+    CHAN_CHECK(hipMemsetAsync(c->d_jet, 0, sizeof(double2), c->stream));
+    CHAN_CHECK(hipStreamSynchronize(c->stream));
+    c->jet_wall = wall;
+    c->jet_forcing = forcing;
+    c->jet_freq = frequency;
   });
 }
 

@@ -146,6 +146,9 @@ struct chan_ctx {
   size_t pin_bytes = 0;
   bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
   LesState* les = nullptr;       // subgrid closure (chan_les_configure)
+  double2* d_jet = nullptr;      // wall jet: spectral v profile [nxh][nz] (chan_jet_configure)
+  int jet_wall = 0;              // 0 bottom, 1 top
+  double jet_forcing = 0.0, jet_freq = 0.0;
   // options
   int xpass_chunk = 8;
   Tuning tune;
@@ -194,6 +197,8 @@ struct SolveArgs {
   int dma;            // 3-lane variant: rows streamed through an LDS ring (Tuning::solve_dma)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
+void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
+                   long nm, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,
                       hipStream_t s);
 void launch_l2_final(const double* part, int n, double npts, const double* dudy, int ny, double* out, hipStream_t s);

@@ -878,6 +878,29 @@ __global__ __launch_bounds__(64) void k_solve3(DevOps ops, SolveArgs a) {
   if (more) buildR(ny, w3, w1, w2, w3, Pw1, cz(), ldo(NL, OC(ny)));
 }
 
+// Wall-normal velocity of a jet at one wall for substep rk: the spectral
+// profile times 1 + forcing sin(2 pi f t) at the substep's end time
+// t = t_total + dt sum_{r<=rk} (rkc(r) + rkd(r)) (the time rhs_solve hands to
+// jet_profile, PFC/pass2.f90:38-43).
+__global__ void k_jet_bc(const DevScalars* __restrict__ sc, int rk, double forcing, double freq,
+                         const double2* __restrict__ prof, double2* __restrict__ cbcv, long nm) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  double sum = 0.0;
+  for (int r = 0; r < rk; ++r) sum = sum + (c_rkc[r] + c_rkd[r]);
+  const double t = sc->t_total + sc->dt * sum;
+  const double amp = 1.0 + forcing * sin(6.283185307179586 * freq * t);
+  const double2 p = prof[m];
+  cbcv[m] = make_double2(amp * p.x, amp * p.y);
+}
+
+void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
+                   long nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_jet_bc, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, sc, rk, forcing, freq, prof,
+                     cbcv, nm);
+  LAUNCH_CHECK("k_jet_bc");
+}
+
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));

@@ -195,6 +195,21 @@ int chan_l2_norm(chan_ctx* ctx, double* l2, double* drag);
 int chan_mode_energy(chan_ctx* ctx, double* ke);
 int chan_plane_moments(chan_ctx* ctx, double* out);
 
+/* Wall jet-in-crossflow (configuration of the linearized/jet-control runs):
+ * a Gaussian wall-normal velocity v = +-A exp(-r^2/radius^2) around (xjet,
+ * zjet) on the nx x nz wall grid (periodic images; + at the bottom wall
+ * (wall 0), - at the top (wall 1), i.e. blowing into the channel), with its
+ * mean removed (a uniform suction over the wall, so no net mass flux), and
+ * amplitude 1 + forcing sin(2 pi frequency t) at each substep's end time.
+ * It overwrites the wall-normal wall velocity at every substep as
+ * pass2::rhs_solve's jet_profile call does (PFC/pass2.f90:38-43; the
+ * reference's jet is a tanh profile with random-walk forcing and suction in a
+ * buffer region, PFC/pass2.f90:423-673).  Combine with the feedback gains
+ * (chan_params ctrlu / ctrlw) for controlled runs.  amplitude = 0 removes the
+ * jet. */
+int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius, double xjet, double zjet,
+                       double forcing, double frequency);
+
 /* Launch-shape tuning of one context (no counterpart in the reference: the
  * reference's only tuning is FFTW_PATIENT planning, PFC/transform.f90:60-160).
  * Keys: 0 x-pass kernel (1: one wavefront per z-row, 0: workgroup per rows),

@@ -729,16 +729,49 @@ def rhs_solve(g: Grid, st: State, rk_step: int):
     return st
 
 
-def rk3_step(g: Grid, st: State, step: int, linear_base=None):
+def jet_profile_spec(g: Grid, wall, amplitude, radius, xjet, zjet):
+    """Gaussian wall jet (no counterpart in the reference, whose jet_profile,
+    PFC/pass2.f90:423-673, is tanh-shaped): v = +-A exp(-r^2/radius^2) about
+    (xjet, zjet) on the nx x nz wall grid with periodic images (+ bottom, -
+    top: blowing into the channel), small r2c (PFC/transform.f90:1071-1142),
+    mean mode removed (uniform suction, no net mass flux).  [nxh, nz]."""
+    x = np.arange(g.nx) * g.lx / g.nx - xjet
+    z = np.arange(g.nz) * g.lz / g.nz - zjet
+    x = x - g.lx * np.floor(x / g.lx + 0.5)
+    z = z - g.lz * np.floor(z / g.lz + 0.5)
+    sign = 1.0 if wall == 0 else -1.0
+    f = sign * amplitude * np.exp(-(z[:, None] ** 2 + x[None, :] ** 2) / (radius * radius))
+    C = small_r2c(g, f)
+    C[0, 0] = 0.0
+    return C
+
+
+def jet_amplitude(st: State, rk, forcing, freq):
+    """1 + forcing sin(2 pi f t) at the end time of substep rk,
+    t = t_total + dt sum_{r<=rk}(rkc + rkd) (PFC/pass2.f90:38-39)."""
+    s = 0.0
+    for r in range(rk):
+        s = s + (RKC[r] + RKD[r])
+    t = st.t_total + st.dt * s
+    return 1.0 + forcing * math.sin(6.283185307179586 * freq * t)
+
+
+def rk3_step(g: Grid, st: State, step: int, linear_base=None, jet=None):
     """One time step of the timestep loop (PFC/main_mpi.f90:70-87): three
     (nonlinear, rhs_solve) substeps, then t_total += dt.  boundary() with
-    zero gains just zeroes cbc (PFC/read_write.f90:1470-1471)."""
+    zero gains just zeroes cbc (PFC/read_write.f90:1470-1471).  jet = (wall,
+    spectral profile, forcing, frequency) sets the wall-normal wall velocity
+    of that wall before each rhs_solve, as jet_profile does (:42-43)."""
     boundary(g, st)
     for rk in (1, 2, 3):
         if linear_base is None:
             st.U, st.dt = nonlinear(g, st.U, step=step, rk_step=rk, dt=st.dt)
         else:
             st.U, st.dt = nonlinear_linearized(g, st.U, linear_base, step=step, rk_step=rk, dt=st.dt)
+        if jet is not None:
+            wall, prof, forcing, freq = jet
+            amp = jet_amplitude(st, rk, forcing, freq)
+            (st.cbc_bot if wall == 0 else st.cbc_top)[1] = amp * prof
         rhs_solve(g, st, rk)
     st.t_total = st.t_total + st.dt
     return st
