@@ -178,7 +178,7 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     XPassArgs a{};
     a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
     a.cflbits = cfl ? c->d_cfl : nullptr;
-    a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->xpass_chunk; a.rz = 0;
+    a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->tune.xpass_chunk; a.rz = 0;
     a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads; a.prefetch = c->tune.xpass_prefetch;
     a.tau = tau;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
@@ -344,7 +344,7 @@ void les_prepare_state(chan_ctx* c) { ensure_vel_in_vel(c); }
 
 // ---------------------------------------------------------------------------
 // Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
-// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma.
+// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk.
 Tuning tuning_defaults() {
   Tuning t;
   auto env = [](const char* n, int& v) {
@@ -360,6 +360,7 @@ Tuning tuning_defaults() {
   v = t.zfwd_kx; env("CHANFLOW_ZFWD_KX", v); tuning_set(t, 6, v);
   v = t.xpass_prefetch; env("CHANFLOW_XPASS_PREFETCH", v); tuning_set(t, 7, v);
   v = t.solve_dma; env("CHANFLOW_SOLVE_DMA", v); tuning_set(t, 8, v);
+  v = t.xpass_chunk; env("CHANFLOW_XPASS_CHUNK", v); tuning_set(t, 9, v);
   return t;
 }
 
@@ -374,6 +375,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 6: if (v != 4 && v != 8) return false; t.zfwd_kx = v; return true;
     case 7: if (v != 0 && v != 1) return false; t.xpass_prefetch = v; return true;
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
+    case 9: if (v < 2 || v > 64) return false; t.xpass_chunk = v; return true;
     default: return false;
   }
 }
@@ -389,6 +391,7 @@ int tuning_get(const Tuning& t, int key) {
     case 6: return t.zfwd_kx;
     case 7: return t.xpass_prefetch;
     case 8: return t.solve_dma;
+    case 9: return t.xpass_chunk;
     default: return -1;
   }
 }

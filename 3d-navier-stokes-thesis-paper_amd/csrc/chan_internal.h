@@ -101,6 +101,7 @@ struct Tuning {
   int zfwd_kx = 8;        // z-forward kx columns per workgroup (4, 8)
   int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
+  int xpass_chunk = 6;    // planes per x-pass workgroup (2..64)
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -150,7 +151,6 @@ struct chan_ctx {
   int jet_wall = 0;              // 0 bottom, 1 top
   double jet_forcing = 0.0, jet_freq = 0.0;
   // options
-  int xpass_chunk = 8;
   Tuning tune;
   int timing = 0;
   std::vector<hipEvent_t> ev;    // timing events (pairs)

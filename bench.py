@@ -118,6 +118,45 @@ def les_rate(cf, synth, c, steps):
     return steps / (t1 - t0), g / n
 
 
+def ensemble_rate(cf, synth, steps, n=48, seed=3):
+    """The batched ensemble: n independent channels drawn from a seed
+    (Nx, Nz in {32, 48, 64, 96}, Ny in {49, 65, 97}, Re_c uniform in
+    [2000, 3500], box 2pi x 2 x pi, dt 0.005, laminar + 1e-2 perturbation),
+    advanced together by one captured graph per step over 8 streams.
+    Returns (ensemble steps/s, grid-point steps/s summed over the members)."""
+    import math
+    pi = math.atan2(0.0, -1.0)
+    rng = np.random.default_rng(seed)
+    mem, pts = [], 0
+    for i in range(n):
+        nx = int(rng.choice([32, 48, 64, 96]))
+        nz = int(rng.choice([32, 48, 64, 96]))
+        ny = int(rng.choice([49, 65, 97]))
+        re = float(rng.uniform(2000.0, 3500.0))
+        ch = cf.Channel(nx, ny, nz, 3 * nx // 2, 3 * nz // 2, 2.0 * pi, pi, re, fix_mass=1, dt_flag=0)
+        y, yh = ch.build_operators(1.5)
+        U0 = synth.laminar_perturbed(nx, ny, nz, yh, y, seed=seed * 1000 + i, amp=1e-2, kmax=4.0)
+        ch.set_state(data1=U0)
+        ch.set_scalars(dt=0.005, t_total=0.0, p_grad=-2.0 / re)
+        ch.initialize_flow()
+        mem.append(ch)
+        pts += nx * ny * nz
+    ens = cf.Ensemble(mem, nstreams=8)
+    ens.step(3)
+    for ch in mem:
+        ch.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    for ch in mem:
+        ch.sync()
+    t1 = time.perf_counter()
+    ens.close()
+    for ch in mem:
+        ch.close()
+    sps = steps / (t1 - t0)
+    return sps, sps * pts, pts
+
+
 def les_eval_results():
     """Errors of the committed filtered-DNS evaluation (les_eval.py)."""
     path = os.path.join(ROOT, "profiles", "les_eval_64x65x64.md")
@@ -136,6 +175,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="steps of the 48-member batched ensemble timed (0: skip)")
     ap.add_argument("--les-steps", type=int, default=200,
                     help="RK3 steps of the learned-closure LES timed for the LES speed-up (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -228,6 +269,15 @@ def main():
                "error_source": None if ev is None else "profiles/les_eval_64x65x64.md (les_eval.py, "
                                                        "a posteriori against the filtered DNS)"}
 
+    ensemble = None
+    if args.ensemble_steps > 0:
+        e_sps, e_pts, npts = ensemble_rate(cf, synth, args.ensemble_steps)
+        ensemble = {"metric": "RK3 steps/s of a 48-member batched ensemble (one captured graph per step)",
+                    "config": "48 channels from seed 3: Nx, Nz in {32,48,64,96}, Ny in {49,65,97}, Re_c "
+                              "uniform in [2000,3500], box 2pi x 2 x pi, 3/2 dealiased, dt 0.005, 8 streams",
+                    "ensemble_steps_per_s": e_sps, "grid_point_steps_per_s": e_pts, "grid_points": npts,
+                    "steps": args.ensemble_steps}
+
     cpu = None
     if not args.no_cpu_baseline:
         sps, secs = cpu_baseline(c, U0, args.cpu_steps)
@@ -262,6 +312,7 @@ def main():
         "max_divergence": div,
         "cpu_baseline": cpu,
         "les": les,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
 
