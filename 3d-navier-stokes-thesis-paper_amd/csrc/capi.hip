@@ -169,24 +169,51 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     KTimer t(c, KT_OTHER);
     tau = les_tau_for_pass(c, rk);
   }
-  {
-    KTimer t(c, KT_ZINV);
-    launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream, c->tune.zinv_kx);
-  }
-  {
+  auto xpass = [&](int ka, int kb) {
     KTimer t(c, KT_XPASS);
     XPassArgs a{};
     a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
     a.cflbits = cfl ? c->d_cfl : nullptr;
-    a.ny = ny; a.ka = 2; a.kb = ny; a.chunk = c->tune.xpass_chunk; a.rz = 0;
+    a.ny = ny; a.ka = ka; a.kb = kb; a.chunk = c->tune.xpass_chunk; a.rz = 0;
     a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads; a.prefetch = c->tune.xpass_prefetch;
     a.tau = tau;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
-  }
-  {
-    KTimer t(c, KT_ZFWD);
-    launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * nyp, ny,
-                c->stream, c->tune.zfwd_v2, c->tune.zfwd_kx);
+  };
+  const int nch = std::max(1, std::min(c->tune.nl_chunks, ny - 1));
+  if (nch == 1) {
+    {
+      KTimer t(c, KT_ZINV);
+      launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * nyp, c->stream, c->tune.zinv_kx);
+    }
+    xpass(2, ny);
+    {
+      KTimer t(c, KT_ZFWD);
+      launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * nyp, ny,
+                  c->stream, c->tune.zfwd_v2, c->tune.zfwd_kx);
+    }
+  } else {
+    // plane ranges in turn, so that each range's intermediates are re-read
+    // soon after they are written; NL planes 1..ny-1 (0-based) are split,
+    // planes 0 and ny (zero) go with the first and last range
+    int done = -1;  // last z-inverse plane
+    for (int ci = 0; ci < nch; ++ci) {
+      const int a0 = 1 + (int)((long)(ny - 1) * ci / nch);
+      const int b0 = (int)((long)(ny - 1) * (ci + 1) / nch);
+      const int zlo = done + 1, zhi = b0 + 1;
+      if (zhi >= zlo) {
+        KTimer t(c, KT_ZINV);
+        launch_zinv(g, c->d_tw, c->d_vel, nmodes(c), c->d_Z, zps, 3 * (zhi - zlo + 1), c->stream,
+                    c->tune.zinv_kx, zhi - zlo + 1, nyp, zlo);
+      }
+      done = zhi;
+      xpass(a0 + 1, b0 + 1);
+      const int f0 = (ci == 0) ? 0 : a0, f1 = (ci == nch - 1) ? ny : b0;
+      {
+        KTimer t(c, KT_ZFWD);
+        launch_zfwd(g, c->d_tw, c->d_AB, c->d_AB + zps, 2 * zps, c->ops.kz, c->d_nl, nmodes(c), 3 * (f1 - f0 + 1),
+                    ny, c->stream, c->tune.zfwd_v2, c->tune.zfwd_kx, f1 - f0 + 1, nyp, f0);
+      }
+    }
   }
   if (cfl) launch_cfl_finalize(c->ops, c->prm, c->d_cfl, c->d_sc, step, c->stream);
 }
@@ -344,7 +371,7 @@ void les_prepare_state(chan_ctx* c) { ensure_vel_in_vel(c); }
 
 // ---------------------------------------------------------------------------
 // Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
-// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk.
+// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk, 10 nl_chunks.
 Tuning tuning_defaults() {
   Tuning t;
   auto env = [](const char* n, int& v) {
@@ -361,6 +388,7 @@ Tuning tuning_defaults() {
   v = t.xpass_prefetch; env("CHANFLOW_XPASS_PREFETCH", v); tuning_set(t, 7, v);
   v = t.solve_dma; env("CHANFLOW_SOLVE_DMA", v); tuning_set(t, 8, v);
   v = t.xpass_chunk; env("CHANFLOW_XPASS_CHUNK", v); tuning_set(t, 9, v);
+  v = t.nl_chunks; env("CHANFLOW_NL_CHUNKS", v); tuning_set(t, 10, v);
   return t;
 }
 
@@ -376,6 +404,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 7: if (v != 0 && v != 1) return false; t.xpass_prefetch = v; return true;
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
     case 9: if (v < 2 || v > 64) return false; t.xpass_chunk = v; return true;
+    case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
     default: return false;
   }
 }
@@ -392,6 +421,7 @@ int tuning_get(const Tuning& t, int key) {
     case 7: return t.xpass_prefetch;
     case 8: return t.solve_dma;
     case 9: return t.xpass_chunk;
+    case 10: return t.nl_chunks;
     default: return -1;
   }
 }

@@ -102,6 +102,7 @@ struct Tuning {
   int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
   int xpass_chunk = 6;    // planes per x-pass workgroup (2..64)
+  int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -167,15 +168,18 @@ FFTPlan make_plan(int n, int tw_offset);
 void xz_geom_init(XZGeom& g, int nx, int nz, int nx2, int nz2, int small, std::vector<double2>& roots);
 
 // launchers (k_transform.hip)
+// pcnt > 0: grid row y is plane poff + y % pcnt of field y / pcnt, fields
+// pstride planes apart (a range of planes of every field); 0: plane y
 void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
-                 long out_ps, int nplanes, hipStream_t s, int kx = 8);
+                 long out_ps, int nplanes, hipStream_t s, int kx = 8, int pcnt = 0, int pstride = 0,
+                 int poff = 0);
 void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps, double* out,
                  long out_ps, int nplanes, hipStream_t s);
 void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const double* fx,
                  const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s);
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s, int v2 = 1, int kx = 8);
+                 hipStream_t s, int v2 = 1, int kx = 8, int pcnt = 0, int pstride = 0, int poff = 0);
 // (k_nonlinear.hip)
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
                   hipStream_t s);

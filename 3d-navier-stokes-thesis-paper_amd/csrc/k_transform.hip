@@ -15,14 +15,23 @@
 
 #define ZT_KX 8
 
+// Plane of a workgroup: grid row y covers planes off .. off+cnt-1 of each
+// field, fields `stride` planes apart (identity: cnt = nplanes, stride = 0).
+struct PlaneMap {
+  int cnt, stride, off;
+};
+__device__ __forceinline__ long map_plane(const PlaneMap& pm) {
+  return (long)(blockIdx.y / pm.cnt) * pm.stride + pm.off + blockIdx.y % pm.cnt;
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
                                               const double2* __restrict__ in, long in_ps,
-                                              double2* __restrict__ out, long out_ps) {
+                                              double2* __restrict__ out, long out_ps, PlaneMap pm) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int kx0 = blockIdx.x * ZT_KX;
-  const long P = blockIdx.y;
+  const long P = map_plane(pm);
   const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   double2* rl = lds + ZT_KX * ss;
   load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
@@ -171,11 +180,11 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
                                               const double2* __restrict__ B, long ab_ps,
                                               const double* __restrict__ kzv,
                                               double2* __restrict__ out, long out_ps,
-                                              int nl_zero_ny) {
+                                              int nl_zero_ny, PlaneMap pm) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int kx0 = blockIdx.x * ZT_KX;
-  const long P = blockIdx.y;
+  const long P = map_plane(pm);
   const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   const int ncol = min(ZT_KX, g.nxh - kx0);
   double2* dst = out + P * out_ps;
@@ -246,12 +255,12 @@ __global__ __launch_bounds__(512) void k_zfwd(XZGeom g, const double2* __restric
 template <int NC, int KX>
 __global__ __launch_bounds__(32 * KX) void k_zinv2(XZGeom g, const double2* __restrict__ tw,
                                                    const double2* __restrict__ in, long in_ps,
-                                                   double2* __restrict__ out, long out_ps) {
+                                                   double2* __restrict__ out, long out_ps, PlaneMap pm) {
   extern __shared__ double2 lds[];
   constexpr int NT = 32 * KX, NZ2 = NC, SS = fft_seq_cells(NC);
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * KX;
-  const long P = blockIdx.y;
+  const long P = map_plane(pm);
   const int nz = g.nz;
   double2* rl = lds + KX * SS;
   load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);
@@ -292,14 +301,14 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
                                                const double2* __restrict__ B, long ab_ps,
                                                const double* __restrict__ kzv,
                                                double2* __restrict__ out, long out_ps,
-                                               int nl_zero_ny) {
+                                               int nl_zero_ny, PlaneMap pm) {
   extern __shared__ double2 lds[];
   constexpr int NZ2 = NC, SS = fft_seq_cells(NC);
   constexpr int NT = 32 * KX;
   constexpr int NLD = (NZ2 * KX + NT - 1) / NT;       // loads per thread per field
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * KX;
-  const long P = blockIdx.y;
+  const long P = map_plane(pm);
   const int nz = g.nz;
   const int ncol = min(KX, g.nxh - kx0);
   double2* dst = out + P * out_ps;
@@ -389,15 +398,16 @@ static void check_stage_fit(const FFTPlan& p, int nseq, int nthr) {
 }
 
 void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_ps, double2* out,
-                 long out_ps, int nplanes, hipStream_t s, int kx) {
+                 long out_ps, int nplanes, hipStream_t s, int kx, int pcnt, int pstride, int poff) {
+  const PlaneMap pm = pcnt > 0 ? PlaneMap{pcnt, pstride, poff} : PlaneMap{nplanes, 0, 0};
   const bool ct = fft2_split(g.nz2).n1 > 0;
   if (ct && (kx == 4 || kx == 16)) {
     dim3 gk((g.nxh + kx - 1) / kx, nplanes);
     const size_t lk = sizeof(double2) * (kx * fft_seq_cells(g.nz2) + g.nz2);
 #define ZINV2_CASE(N)                                                                                      \
   if constexpr (N > 0) {                                                                                   \
-    if (kx == 4) hipLaunchKernelGGL((k_zinv2<N, 4>), gk, dim3(128), lk, s, g, tw, in, in_ps, out, out_ps);  \
-    else hipLaunchKernelGGL((k_zinv2<N, 16>), gk, dim3(512), lk, s, g, tw, in, in_ps, out, out_ps);          \
+    if (kx == 4) hipLaunchKernelGGL((k_zinv2<N, 4>), gk, dim3(128), lk, s, g, tw, in, in_ps, out, out_ps, pm); \
+    else hipLaunchKernelGGL((k_zinv2<N, 16>), gk, dim3(512), lk, s, g, tw, in, in_ps, out, out_ps, pm);         \
   }
     FFT_DISPATCH(g.nz2, ZINV2_CASE)
 #undef ZINV2_CASE
@@ -409,7 +419,7 @@ void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_
   if (!ct) check_stage_fit(g.pz, ZT_KX, nthr);
   size_t lds = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
 #define ZINV_CASE(N) \
-  hipLaunchKernelGGL(k_zinv<N>, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps)
+  hipLaunchKernelGGL(k_zinv<N>, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps, pm)
   FFT_DISPATCH(g.nz2, ZINV_CASE)
 #undef ZINV_CASE
   LAUNCH_CHECK("k_zinv");
@@ -456,7 +466,8 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
 
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
-                 hipStream_t s, int v2, int kx) {
+                 hipStream_t s, int v2, int kx, int pcnt, int pstride, int poff) {
+  const PlaneMap pm = pcnt > 0 ? PlaneMap{pcnt, pstride, poff} : PlaneMap{nplanes, 0, 0};
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nf = B ? 2 : 1;
   const int nthr = 256 * nf;
@@ -468,9 +479,9 @@ void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const dou
 #define ZFWD2_CASE(N)                                                                                       \
   if constexpr (N > 0) {                                                                                    \
     if (k2 == 4) hipLaunchKernelGGL((k_zfwd2<N, 4>), g2, dim3(128), l2, s, g, tw, A, B, ab_ps, kz, out, out_ps, \
-                                    nl_zero_ny);                                                            \
+                                    nl_zero_ny, pm);                                                        \
     else hipLaunchKernelGGL((k_zfwd2<N, 8>), g2, dim3(256), l2, s, g, tw, A, B, ab_ps, kz, out, out_ps,       \
-                            nl_zero_ny);                                                                    \
+                            nl_zero_ny, pm);                                                                \
   }
     FFT_DISPATCH(g.nz2, ZFWD2_CASE)
 #undef ZFWD2_CASE
@@ -481,7 +492,7 @@ void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const dou
   size_t lds = sizeof(double2) * (nf * ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
 #define ZFWD_CASE(N)                                                                              \
   hipLaunchKernelGGL(k_zfwd<N>, grid, dim3(nthr), lds, s, g, tw, A, B, ab_ps, kz, out, out_ps, \
-                     nl_zero_ny)
+                     nl_zero_ny, pm)
   FFT_DISPATCH(g.nz2, ZFWD_CASE)
 #undef ZFWD_CASE
   LAUNCH_CHECK("k_zfwd");

@@ -107,6 +107,24 @@ def test_nonlinear_single(g0):
     ch.close()
 
 
+@pytest.mark.parametrize("nch", [2, 5])
+def test_nonlinear_plane_ranges(g0, nch):
+    """The nonlinear pass run over plane ranges in turn (tuning key nl_chunks)
+    computes the same term as the whole-plane kernels, bit for bit."""
+    U = init_cfg0(O, g0, seed=21, amp=1e-2)
+    out = []
+    for n in (1, nch):
+        ch = make_channel(g0)
+        ch.set_tuning("nl_chunks", n)
+        ch.set_state(data1=U)
+        ch.nonlinear(1, 1, dt=0.01)
+        out.append(ch.get_data1())
+        ch.close()
+    ref, _ = O.nonlinear(g0, U, step=1, rk_step=1, dt=0.01)
+    assert rel_l2(out[1], ref) < TOL1
+    assert np.array_equal(out[0], out[1])
+
+
 def test_nonlinear_dt_flag(g0):
     g = grid_cfg0(O, dt_flag=1)
     ch = make_channel(g)
