@@ -4,8 +4,8 @@
 ! pass1_linearized, pass2 and solve.
 !
 ! Every C entry point returns 0 on success; chanflow_check prints the
-! library's message and STOPs, as the reference STOPs on its own errors
-! (e.g. PFC/pass1.f90:243).
+! library's message and ERROR STOPs (a nonzero exit status), as the reference
+! STOPs on its own errors (e.g. PFC/pass1.f90:243).
 !
 ! State residency: after the first nonlinear call the newest data1, data2
 ! and data4 live on the GPU.  chanflow_upload_state copies the host arrays
@@ -27,7 +27,8 @@ MODULE chanflow_f
             chan_xz_fft_real_to_complex, chan_compute_divergence_of_fluxes, &
             chan_small_fft_complex_to_real, chan_small_fft_real_to_complex, &
             chan_solve_helmholtz, chan_solve_p, chan_linearized_matvec, c_string, &
-            chan_create, chan_destroy, chan_last_error
+            chan_create, chan_destroy, chan_last_error, chan_set_state, chan_get_state, &
+            chan_set_scalars, chan_build_operators, chan_initialize_flow, chan_boundary
 
   TYPE, BIND(C) :: chan_params
      INTEGER(c_int) :: nx, ny, nz, nx2, nz2
@@ -66,6 +67,21 @@ MODULE chanflow_f
        TYPE(c_ptr), VALUE               :: ctx
        TYPE(chan_operators), INTENT(IN) :: ops
      END FUNCTION chan_upload_operators
+     INTEGER(c_int) FUNCTION chan_build_operators(ctx, stretch, y, yh) BIND(C, name='chan_build_operators')
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE     :: ctx
+       REAL(c_double), VALUE  :: stretch
+       REAL(c_double)         :: y(*), yh(*)
+     END FUNCTION chan_build_operators
+     INTEGER(c_int) FUNCTION chan_initialize_flow(ctx, linearized) BIND(C, name='chan_initialize_flow')
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE    :: ctx
+       INTEGER(c_int), VALUE :: linearized
+     END FUNCTION chan_initialize_flow
+     INTEGER(c_int) FUNCTION chan_boundary(ctx) BIND(C, name='chan_boundary')
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: ctx
+     END FUNCTION chan_boundary
      INTEGER(c_int) FUNCTION chan_set_state(ctx, d1, d2, d4) BIND(C, name='chan_set_state')
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: ctx, d1, d2, d4
@@ -195,7 +211,7 @@ CONTAINS
     CHARACTER(LEN=*), INTENT(IN) :: where
     IF (rc /= 0) THEN
        WRITE(*,*) 'chanflow: ', where, ': ', c_string(chan_last_error(chanflow_ctx))
-       STOP 'CHANFLOW ERROR'
+       ERROR STOP 'CHANFLOW ERROR'
     END IF
   END SUBROUTINE chanflow_check
 
