@@ -226,6 +226,7 @@ void solve_pass(chan_ctx* c, int rk) {
   a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
   a.fix_mass = c->prm.fix_mass;
   a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl; a.dma = c->tune.solve_dma;
+  a.seg = c->tune.solve_seg;
   if (c->d_jet) {  // the jet overwrites the wall-normal velocity at its wall (PFC/pass2.f90:42-43)
     const long nm = nmodes(c);
     launch_jet_bc(c->d_sc, rk, c->jet_forcing, c->jet_freq, c->d_jet,
@@ -371,7 +372,8 @@ void les_prepare_state(chan_ctx* c) { ensure_vel_in_vel(c); }
 
 // ---------------------------------------------------------------------------
 // Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
-// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk, 10 nl_chunks.
+// 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk, 10 nl_chunks,
+// 11 solve_seg.
 Tuning tuning_defaults() {
   Tuning t;
   auto env = [](const char* n, int& v) {
@@ -389,6 +391,7 @@ Tuning tuning_defaults() {
   v = t.solve_dma; env("CHANFLOW_SOLVE_DMA", v); tuning_set(t, 8, v);
   v = t.xpass_chunk; env("CHANFLOW_XPASS_CHUNK", v); tuning_set(t, 9, v);
   v = t.nl_chunks; env("CHANFLOW_NL_CHUNKS", v); tuning_set(t, 10, v);
+  v = t.solve_seg; env("CHANFLOW_SOLVE_SEG", v); tuning_set(t, 11, v);
   return t;
 }
 
@@ -405,6 +408,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
     case 9: if (v < 2 || v > 64) return false; t.xpass_chunk = v; return true;
     case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
+    case 11: if (v != 0 && v != 1) return false; t.solve_seg = v; return true;
     default: return false;
   }
 }
@@ -422,6 +426,7 @@ int tuning_get(const Tuning& t, int key) {
     case 8: return t.solve_dma;
     case 9: return t.xpass_chunk;
     case 10: return t.nl_chunks;
+    case 11: return t.solve_seg;
     default: return -1;
   }
 }
@@ -1286,15 +1291,22 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
     };
     std::vector<Tuning> cand;
     for (int w : {1, 0})
-      for (int z : {1, 0})
+      for (int z : {1, 0}) {
+        if (solve_seg_rows(c->prm.ny)) {  // the segmented solve has no further shape
+          Tuning t = c->tune;
+          t.xpass_wave = w; t.zfwd_v2 = z; t.solve_seg = 1;
+          cand.push_back(t);
+        }
         for (int sl : {3, 1})
           for (int bl : {8, 16})
             for (int dm : {1, 0}) {
               if (dm && (sl != 3 || bl != 8)) continue;  // the ring variant has no block size
               Tuning t = c->tune;
-              t.xpass_wave = w; t.zfwd_v2 = z; t.solve_lanes = sl; t.solve_bl = bl; t.solve_dma = dm;
+              t.xpass_wave = w; t.zfwd_v2 = z; t.solve_seg = 0;
+              t.solve_lanes = sl; t.solve_bl = bl; t.solve_dma = dm;
               cand.push_back(t);
             }
+      }
     hipEvent_t a, b;
     CHAN_CHECK(hipEventCreate(&a));
     CHAN_CHECK(hipEventCreate(&b));

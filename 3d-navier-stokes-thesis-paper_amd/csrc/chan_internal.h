@@ -103,6 +103,7 @@ struct Tuning {
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
   int xpass_chunk = 6;    // planes per x-pass workgroup (2..64)
   int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
+  int solve_seg = 1;      // segmented column solve (columns in registers, no scratch)
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -199,8 +200,10 @@ struct SolveArgs {
   int fix_mass;
   int lanes, bl;      // kernel variant (Tuning::solve_lanes, solve_bl)
   int dma;            // 3-lane variant: rows streamed through an LDS ring (Tuning::solve_dma)
+  int seg;            // segmented column solve where the row count allows (Tuning::solve_seg)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
+int solve_seg_rows(int ny);  // modes per wavefront of the segmented solve, 0 if ny+1 rows do not fit
 void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
                    long nm, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,

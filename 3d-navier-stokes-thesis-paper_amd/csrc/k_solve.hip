@@ -439,6 +439,508 @@ __device__ __forceinline__ double2 shfl2(double2 v, int src) {
   return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
+// ---------------------------------------------------------------------------
+// Segmented column solve: the whole y column of every mode stays on chip.
+//
+// A wavefront owns MODES consecutive (kx,kz) modes; lane (s, q) = (lane /
+// MODES, lane % MODES) owns rows s*SEG .. s*SEG+SEG-1 of mode q, all three
+// velocity components and the pressure correction, in registers.  The last
+// segment holds the remaining 3..SEG rows; rows past ny are identity padding.
+// Each tridiagonal system (u and w with D2H, v with D2N, the Neumann Poisson
+// system) is solved by partition (seg_solve):
+//   1. every segment eliminates its interior rows top-down, carrying the
+//      coupling to the separator below it (the previous segment's last row)
+//      as a second right-hand side; the coupling to the separator above it
+//      (its own last row, except in the last segment) stays in c';
+//   2. with the interiors eliminated, the separators form a tridiagonal
+//      system of SEGS-1 unknowns per mode; its rows go through LDS and every
+//      lane of the mode solves it;
+//   3. every segment back-substitutes with both separators known.
+// The state is read once (NL, R and, at rk_step 1, P for the right-hand side;
+// P and NL again for the pressure update and the next RK register) and
+// written once (u, P, R): no scratch array.  The arithmetic per row is that
+// of k_solve; only the elimination order of the solves differs, so results
+// agree with it to rounding.
+// ---------------------------------------------------------------------------
+
+// Partitioned solve of NC right-hand sides X[c][0..SEG) that share one
+// matrix; coef(t, a, b, c) gives row t of this lane's segment; ns segments
+// are active.  red holds 64 * (3 + 2*NC) doubles of LDS.
+template <int MODES, int SEG, int NC, class CoefF>
+__device__ __forceinline__ void seg_solve(double2 (&X)[NC][SEG], CoefF coef, double* red, int lane, int ns) {
+  constexpr int W = 3 + 2 * NC;
+  const int s = lane / MODES, q = lane - (lane / MODES) * MODES;
+  const bool last = (s >= ns - 1);  // the last active segment (and idle lanes past it)
+  double cp[SEG], gp[SEG];
+  double aT = 0.0, bT = 1.0, cT = 0.0;
+  double2 fT[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) fT[c] = X[c][SEG - 1];
+  // 1. elimination; gp carries the coupling to the separator below
+#pragma unroll
+  for (int t = 0; t < SEG; ++t) {
+    double a_, b_, c_;
+    coef(t, a_, b_, c_);
+    if (t == SEG - 1) { aT = a_; bT = b_; cT = c_; }
+    const double bp = (t == 0) ? b_ : b_ - a_ * cp[t - 1];
+    const double inv = 1.0 / bp;
+    cp[t] = c_ * inv;
+    gp[t] = (t == 0) ? (-a_) * inv : (-(a_ * gp[t - 1])) * inv;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      X[c][t] = (t == 0) ? rmul(inv, X[c][t]) : rmul(inv, csub(X[c][t], rmul(a_, X[c][t - 1])));
+  }
+  // the separator row itself: x = 0 + 0 * xl - (-1) * xr
+  if (!last) {
+    cp[SEG - 1] = -1.0;
+    gp[SEG - 1] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) X[c][SEG - 1] = cz();
+  }
+  // row 0 of the segment in terms of the separators: x0 = Y + VL xl + VR xr
+  double2 Y[NC];
+  double VL = gp[SEG - 1], VR = -cp[SEG - 1];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) Y[c] = X[c][SEG - 1];
+#pragma unroll
+  for (int t = SEG - 2; t >= 0; --t) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) Y[c] = csub(X[c][t], rmul(cp[t], Y[c]));
+    VL = gp[t] - cp[t] * VL;
+    VR = -cp[t] * VR;
+  }
+  // 2. separator s (this segment's last row) couples rows SEG-2 here and 0 of segment s+1
+  const int nb = last ? lane : lane + MODES;
+  const double VLn = __shfl(VL, nb, 64), VRn = __shfl(VR, nb, 64);
+  double2 Yn[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) Yn[c] = shfl2(Y[c], nb);
+  if (!last) {
+    double* e = red + lane * W;
+    e[0] = aT * gp[SEG - 2];
+    e[1] = aT * (-cp[SEG - 2]) + bT + cT * VLn;
+    e[2] = cT * VRn;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double2 g = csub(csub(fT[c], rmul(aT, X[c][SEG - 2])), rmul(cT, Yn[c]));
+      e[3 + 2 * c] = g.x;
+      e[4 + 2 * c] = g.y;
+    }
+  }
+  __syncthreads();
+  // one lane per mode eliminates the separator system in place: slot 2 takes
+  // c', slots 3.. take f' and then the solution
+  if (s == 0) {
+    double rcp = 0.0;
+    double2 rfp[NC];
+    for (int k = 0; k < ns - 1; ++k) {
+      double* ek = red + (k * MODES + q) * W;
+      const double A = ek[0], B = ek[1], C = ek[2];
+      const double bp = (k == 0) ? B : B - A * rcp;
+      const double inv = 1.0 / bp;
+      rcp = C * inv;
+      ek[2] = rcp;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const double2 g = make_double2(ek[3 + 2 * c], ek[4 + 2 * c]);
+        rfp[c] = (k == 0) ? rmul(inv, g) : rmul(inv, csub(g, rmul(A, rfp[c])));
+        ek[3 + 2 * c] = rfp[c].x;
+        ek[4 + 2 * c] = rfp[c].y;
+      }
+    }
+    for (int k = ns - 3; k >= 0; --k) {
+      double* ek = red + (k * MODES + q) * W;
+      const double rck = ek[2];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const double2 f = make_double2(ek[3 + 2 * c], ek[4 + 2 * c]);
+        rfp[c] = csub(f, rmul(rck, rfp[c]));
+        ek[3 + 2 * c] = rfp[c].x;
+        ek[4 + 2 * c] = rfp[c].y;
+      }
+    }
+  }
+  __syncthreads();
+  double2 xl[NC], xr[NC];
+  {
+    const double* el = red + ((s > 0 && s < ns ? s - 1 : 0) * MODES + q) * W;
+    const double* er = red + ((last ? 0 : s) * MODES + q) * W;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      xl[c] = (s > 0 && s < ns) ? make_double2(el[3 + 2 * c], el[4 + 2 * c]) : cz();
+      xr[c] = last ? cz() : make_double2(er[3 + 2 * c], er[4 + 2 * c]);
+    }
+  }
+  __syncthreads();  // red is reused by the next system
+  // 3. back substitution
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double2 nx = xr[c];
+#pragma unroll
+    for (int t = SEG - 1; t >= 0; --t) {
+      X[c][t] = csub(cadd(X[c][t], rmul(gp[t], xl[c])), rmul(cp[t], nx));
+      nx = X[c][t];
+    }
+  }
+}
+
+// Memory fence for the scheduler every SEG_AHEAD rows of a streaming loop:
+// loads are not moved across it, so a loop keeps at most about SEG_AHEAD
+// rows of loads in flight instead of hoisting every row's loads.
+#ifndef SEG_AHEAD
+#define SEG_AHEAD 3
+#endif
+#define SEG_FENCE(t) do { if (((t) + 1) % SEG_AHEAD == 0) asm volatile("" ::: "memory"); } while (0)
+
+template <int MODES, int SEG>
+__global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
+  constexpr int SEGS = 64 / MODES;
+  __shared__ double red[64 * 7];
+  const int lane = threadIdx.x;
+  const int s = lane / MODES, q = lane - (lane / MODES) * MODES;
+  const long nm = (long)a.nxh * a.nz;
+  const int ny = a.ny, nyp = ny + 1;
+  const int ns = (nyp + SEG - 1) / SEG;   // active segments
+  const int rl = nyp - (ns - 1) * SEG;    // rows of the last one (3..SEG)
+  // Mode groups whose rows share 128-byte lines (MODES < 8) go to workgroups
+  // b and b+8, which land on the same XCD one after the other.
+  long grp = blockIdx.x;
+  if (MODES < 8) {
+    const long full = (long)(gridDim.x / 16) * 16;
+    if (grp < full) {
+      const long xcd = grp & 7, k = grp >> 3;
+      grp = 16 * (k >> 1) + 2 * xcd + (k & 1);
+    }
+  }
+  const long mg = grp * MODES + q;
+  const bool valid = (mg < nm) && (s < ns);
+  const long m = (mg < nm) ? mg : nm - 1;
+  const int ix = (int)(m / a.nz), iz = (int)(m - (long)ix * a.nz);
+  const int j0 = s * SEG;
+  const bool last = (s == ns - 1);
+  const double kxv = ops.kx[ix], kzv = ops.kz[iz];
+  const double k2 = kxv * kxv + kzv * kzv;
+  const bool mean = (k2 == 0.0);
+  const int r = a.rk - 1;
+  const double dt = a.sc->dt;
+  const double re = a.re;
+  const double* d2h = ops.d2h;
+  const double* d2n = ops.d2n;
+  const double* dn = ops.dn;
+  const double* dh = ops.dh;
+  const double* dm = ops.dm;
+  const double2* __restrict__ NL = a.nl;
+  double2* __restrict__ R = a.R;
+  double2* __restrict__ P = a.P;
+  double2* __restrict__ U = a.vel;
+  auto neg = [](double2 v) { return make_double2(-v.x, -v.y); };
+  auto sel = [](bool c, double2 x, double2 y) { return make_double2(c ? x.x : y.x, c ? x.y : y.y); };
+
+  const uint32_t rowB = (uint32_t)(nm * 16), cmpB = rowB * (uint32_t)nyp;
+  const uint32_t om0 = (uint32_t)(m * 16);
+  auto OM = [&](int j) { return om0 + (uint32_t)j * rowB; };
+  auto OC = [&](int c, int j) { return (uint32_t)c * cmpB + om0 + (uint32_t)j * rowB; };
+
+  const double vis_imp = c_rkd[r] * (dt / re);
+  const double rk_press = (c_rkc[r] + c_rkd[r]) * dt;
+  const double coef = c_rkb[r] * dt;
+  const double vis_exp1 = c_rkc[r] * (dt / re);
+  const double alpha = 1.0 + vis_imp * k2;
+
+  // mean-mode pressure gradient (compute_p_grad, :369-373); the mean mode is mode 0
+  double p_grad = a.sc->p_grad;
+  if (grp == 0 && a.fix_mass == 1) {
+    double part = 0.0;
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t;
+      const int jc = min(j, ny);
+      const double term = ops.dyh[jc] * (-ldo(NL, OC(0, jc)).x);
+      part = (j >= 1 && j <= ny - 1) ? part + term : part;
+    }
+    double tmp = 0.0;
+#pragma unroll
+    for (int k = 0; k < SEGS; ++k) tmp = (k < ns) ? tmp + __shfl(part, k * MODES + q, 64) : tmp;
+    if (mean) {
+      p_grad = -tmp + 0.5 * (a.dudy[ny - 1] - a.dudy[0]) / re;
+      if (lane == 0) a.sc->p_grad = p_grad;
+    }
+  }
+  const double2 msh0 = make_double2(a.cosb * rk_press * p_grad, 0.0);
+  const double2 msh2 = make_double2(a.sinb * rk_press * p_grad, 0.0);
+
+  const double2 top0 = a.cbc[(0L * 3 + 0) * nm + m], top1 = a.cbc[(0L * 3 + 1) * nm + m],
+                top2 = a.cbc[(0L * 3 + 2) * nm + m];
+  const double2 bot0 = a.cbc[(1L * 3 + 0) * nm + m], bot1 = a.cbc[(1L * 3 + 1) * nm + m],
+                bot2 = a.cbc[(1L * 3 + 2) * nm + m];
+
+  // ---------------- right-hand sides (rows j0 .. j0+SEG-1) ----------------
+  double2 UW[2][SEG], V[1][SEG];
+  auto rhs = [&](auto first_c) {
+    constexpr bool FIRST = decltype(first_c)::value;
+    const int jb = min(j0, ny), jm = max(min(j0 - 1, ny), 0);  // rows clamped for idle lanes
+    double2 Rm0 = cz(), Rm1 = cz(), Rm2 = cz(), Pc = cz();
+    if (FIRST) {
+      Rm0 = ldo(R, OC(0, jm)); Rm1 = ldo(R, OC(1, jm)); Rm2 = ldo(R, OC(2, jm));
+      Pc = ldo(P, OM(jb));
+    }
+    double2 Rc0 = ldo(R, OC(0, jb)), Rc1 = ldo(R, OC(1, jb)), Rc2 = ldo(R, OC(2, jb));
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t;
+      const int jc = min(j, ny), jn = min(j + 1, ny), jv = min(j, ny - 1);
+      const double2 n0 = ldo(NL, OC(0, jc)), n1 = ldo(NL, OC(1, jc)), n2 = ldo(NL, OC(2, jc));
+      double2 r0 = cz(), r1 = cz(), r2 = cz();
+      if (t + 1 < SEG || FIRST) { r0 = ldo(R, OC(0, jn)); r1 = ldo(R, OC(1, jn)); r2 = ldo(R, OC(2, jn)); }
+      double2 v0 = cadd(Rc0, rmul(coef, neg(n0)));
+      double2 v1 = cadd(Rc1, rmul(coef, neg(n1)));
+      double2 v2 = cadd(Rc2, rmul(coef, neg(n2)));
+      double2 pn = cz();
+      if (FIRST) {
+        // add_explicit_diffusive_terms(cv, cru) (:339-351)
+        pn = ldo(P, OM(jn));
+        const double a0 = d2h[3 * jc], a1 = d2h[3 * jc + 1], a2 = d2h[3 * jc + 2];
+        const double b0 = d2n[3 * jv], b1 = d2n[3 * jv + 1], b2 = d2n[3 * jv + 2];
+        const double dnj = dn[jv];
+        const double2 du = cadd(cadd(rmul(a0, Rm0), rmul(a1, Rc0)), rmul(a2, r0));
+        v0 = cadd(v0, rmul(vis_exp1, csub(du, rmul(k2, Rc0))));
+        const double2 dw = cadd(cadd(rmul(a0, Rm2), rmul(a1, Rc2)), rmul(a2, r2));
+        v2 = cadd(v2, rmul(vis_exp1, csub(dw, rmul(k2, Rc2))));
+        const double2 dv = cadd(cadd(rmul(b0, Rm1), rmul(b1, Rc1)), rmul(b2, r1));
+        v1 = sel(j <= ny - 2, cadd(v1, rmul(vis_exp1, csub(dv, rmul(k2, Rc1)))), v1);
+        // subtract_pressure_gradient(cv, cp, rk_press) (:54-77)
+        v0 = sel(mean, v0, csub(v0, imul(rk_press * kxv, Pc)));
+        v2 = sel(mean, v2, csub(v2, imul(rk_press * kzv, Pc)));
+        v1 = sel(!mean && j <= ny - 2, csub(v1, rmul(rk_press * dnj, csub(pn, Pc))), v1);
+      }
+      // mean pressure gradient (:220-221)
+      v0 = sel(mean, csub(v0, msh0), v0);
+      v2 = sel(mean, csub(v2, msh2), v2);
+      // wall rows carry the wall values (SOLVE_HELMHOLTZ, :174-213); v has no row ny
+      v0 = sel(j == 0, bot0, sel(j == ny, top0, sel(j > ny, cz(), v0)));
+      v2 = sel(j == 0, bot2, sel(j == ny, top2, sel(j > ny, cz(), v2)));
+      v1 = sel(j == 0, bot1, sel(j == ny - 1, top1, sel(j >= ny, cz(), v1)));
+      UW[0][t] = v0;
+      UW[1][t] = v2;
+      V[0][t] = v1;
+      Rm0 = Rc0; Rm1 = Rc1; Rm2 = Rc2;
+      Rc0 = r0; Rc1 = r1; Rc2 = r2;
+      Pc = pn;
+      SEG_FENCE(t);
+    }
+  };
+  if (r == 0) rhs(std::true_type{});
+  else rhs(std::false_type{});
+
+  // ---------------- Helmholtz solves ----------------
+  seg_solve<MODES, SEG, 2>(UW, [&](int t, double& a_, double& b_, double& c_) {
+    const int j = j0 + t, jc = min(j, ny);
+    const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
+    const bool wall = (j == 0 || j >= ny);
+    a_ = wall ? 0.0 : -vis_imp * h0;
+    b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
+    c_ = wall ? 0.0 : -vis_imp * h2;
+  }, red, lane, ns);
+  seg_solve<MODES, SEG, 1>(V, [&](int t, double& a_, double& b_, double& c_) {
+    const int j = j0 + t, jv = min(j, ny - 1);
+    const double h0 = d2n[3 * jv], h1 = d2n[3 * jv + 1], h2 = d2n[3 * jv + 2];
+    const bool wall = (j == 0 || j >= ny - 1);
+    a_ = wall ? 0.0 : -vis_imp * h0;
+    b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
+    c_ = wall ? 0.0 : -vis_imp * h2;
+  }, red, lane, ns);
+#pragma unroll
+  for (int t = 0; t < SEG; ++t)  // mean mode (:261-263)
+    V[0][t] = sel(mean, (j0 + t < ny) ? bot1 : cz(), V[0][t]);
+
+  // ---------------- Neumann Poisson for the pressure correction ----------------
+  double2 PH[1][SEG];
+  {
+    const int pv = (s == 0) ? lane : lane - MODES;
+    const double2 vprev = shfl2(V[0][SEG - 1], pv);
+    const double tmpc = 1.0 / rk_press;
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t, jc = min(j, ny);
+      const double2 vm = (t == 0) ? vprev : V[0][t - 1];
+      // strip_divergence rhs (:28-31)
+      const double2 dv = cadd(cadd(imul(kxv, UW[0][t]), rmul(dh[jc], csub(V[0][t], vm))), imul(kzv, UW[1][t]));
+      PH[0][t] = sel(j >= 1 && j <= ny - 1 && !mean, rmul(tmpc, dv), cz());
+    }
+  }
+  const double alp = mean ? -1.0 : -k2;  // the mean mode has no pressure correction
+  {
+    const double dn0 = dn[0], dn1 = dn[1], dh1 = dh[1];
+    const double dnt = dn[ny - 1], dnt2 = dn[ny - 2], dht = dh[ny - 1];
+    seg_solve<MODES, SEG, 1>(PH, [&](int t, double& a_, double& b_, double& c_) {
+      const int j = j0 + t, jc = min(j, ny);
+      const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
+      a_ = h0; b_ = h1 + alp; c_ = h2;
+      a_ = (j == 1) ? dh1 * dn0 : a_;
+      b_ = (j == 1) ? -dh1 * dn0 - dh1 * dn1 + alp : b_;
+      c_ = (j == 1) ? dh1 * dn1 : c_;
+      a_ = (j == ny - 1) ? dht * dnt2 : a_;
+      b_ = (j == ny - 1) ? -dht * dnt2 - dht * dnt + alp : b_;
+      c_ = (j == ny - 1) ? dht * dnt : c_;
+      a_ = (j == 0) ? 0.0 : (j == ny) ? -dnt : (j > ny) ? 0.0 : a_;
+      b_ = (j == 0) ? -dn0 : (j == ny) ? dnt : (j > ny) ? 1.0 : b_;
+      c_ = (j == 0) ? dn0 : (j >= ny) ? 0.0 : c_;
+    }, red, lane, ns);
+  }
+  // wall re-extrapolation (SOLVE_P :335-336, zero wall data)
+  {
+    const double2 pb = make_double2((0.0 - dm[1] * PH[0][1].x - dm[2] * PH[0][2].x) / dm[0],
+                                    (0.0 - dm[1] * PH[0][1].y - dm[2] * PH[0][2].y) / dm[0]);
+    PH[0][0] = sel(s == 0, pb, PH[0][0]);
+    double2 q1 = cz(), q2 = cz();
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      q1 = sel(t == rl - 3, PH[0][t], q1);
+      q2 = sel(t == rl - 2, PH[0][t], q2);
+    }
+    const double2 pt = make_double2((0.0 - dm[3] * q1.x - dm[4] * q2.x) / dm[5],
+                                    (0.0 - dm[3] * q1.y - dm[4] * q2.y) / dm[5]);
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) PH[0][t] = sel(mean, cz(), sel(last && t == rl - 1, pt, PH[0][t]));
+  }
+
+  // ---------------- projection and pressure update ----------------
+  const int nxt = (last || s + 1 >= SEGS) ? lane : lane + MODES, prv = (s == 0) ? lane : lane - MODES;
+  const double alpha2 = 1.0 + vis_imp * k2;
+  double2 PN[SEG];
+  {
+    const double2 phN = shfl2(PH[0][0], nxt);
+    const double2 phP = shfl2(PH[0][SEG - 1], prv);
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t, jc = min(j, ny), jv = min(j, ny - 1);
+      const double2 f0 = PH[0][t];
+      const double2 fn = (t + 1 < SEG) ? PH[0][t + 1] : phN;
+      double2 s0 = (t > 0) ? PH[0][t - 1] : phP, s1 = f0, s2 = fn;
+      if (t == 0) { s0 = sel(j == 0, PH[0][0], s0); s1 = sel(j == 0, PH[0][1], s1); s2 = sel(j == 0, PH[0][2], s2); }
+      if (t >= 2) { s0 = sel(j == ny, PH[0][t - 2], s0); s1 = sel(j == ny, PH[0][t - 1], s1); s2 = sel(j == ny, f0, s2); }
+      const double2 Pold = ldo(P, OM(jc));
+      const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
+      const double dnj = dn[jv];
+      const bool inner = !mean && j >= 1 && j <= ny - 1;
+      const double2 cu = sel(inner, csub(UW[0][t], imul(rk_press * kxv, f0)), UW[0][t]);
+      const double2 cw = sel(inner, csub(UW[1][t], imul(rk_press * kzv, f0)), UW[1][t]);
+      const double2 cv = sel(inner && j <= ny - 2, csub(V[0][t], rmul(rk_press * dnj, csub(fn, f0))), V[0][t]);
+      const double2 d2 = cadd(cadd(rmul(h0, s0), rmul(h1, s1)), rmul(h2, s2));
+      const double2 pn = sel(mean, cz(), csub(cadd(Pold, rmul(alpha2, f0)), rmul(vis_imp, d2)));
+      UW[0][t] = cu;
+      V[0][t] = cv;
+      UW[1][t] = cw;
+      PN[t] = pn;
+      SEG_FENCE(t);
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t;
+      if (j <= ny) {
+        sto(U, OC(0, j), UW[0][t]);
+        sto(U, OC(1, j), V[0][t]);
+        sto(U, OC(2, j), UW[1][t]);
+        sto(P, OM(j), PN[t]);
+      }
+    }
+  }
+
+  // ---------------- d_um_dy and the next RK register ----------------
+  const double2 uN = shfl2(UW[0][0], nxt), vN = shfl2(V[0][0], nxt), wN = shfl2(UW[1][0], nxt);
+  const double2 uP = shfl2(UW[0][SEG - 1], prv), vP = shfl2(V[0][SEG - 1], prv), wP = shfl2(UW[1][SEG - 1], prv);
+  const double2 pnN = shfl2(PN[0], nxt);
+  if (mean && valid) {  // d_um_dy update from the new mean profile (:84-89)
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t;
+      const double un = (t + 1 < SEG) ? UW[0][t + 1].x : uN.x;
+      if (j >= 1 && j <= ny - 2) a.dudy[j] = (dn[j] * (un - UW[0][t].x));
+      if (t == 0 && j == 0) a.dudy[0] = dm[0] * UW[0][0].x + dm[1] * UW[0][1].x + dm[2] * UW[0][2].x;
+      if (t >= 2 && j == ny)
+        a.dudy[ny - 1] = dm[3] * UW[0][t - 2].x + dm[4] * UW[0][t - 1].x + dm[5] * UW[0][t].x;
+    }
+  }
+  if (!valid) return;
+  if (a.rk == 3) {
+#pragma unroll
+    for (int t = 0; t < SEG; ++t) {
+      const int j = j0 + t;
+      if (j <= ny) {
+        sto(R, OC(0, j), UW[0][t]);
+        sto(R, OC(1, j), V[0][t]);
+        sto(R, OC(2, j), UW[1][t]);
+      }
+    }
+    return;
+  }
+  const double coef2 = dt * c_rka[r + 1];
+  const double vis_exp2 = (dt / re) * c_rkc[r + 1];
+  const double rk_press2 = (c_rkc[r + 1] + c_rkd[r + 1]) * dt;
+#pragma unroll
+  for (int t = 0; t < SEG; ++t) {
+    const int j = j0 + t, jc = min(j, ny), jv = min(j, ny - 1);
+    const double2 n0 = ldo(NL, OC(0, jc)), n1 = ldo(NL, OC(1, jc)), n2 = ldo(NL, OC(2, jc));
+    const double a0 = d2h[3 * jc], a1 = d2h[3 * jc + 1], a2 = d2h[3 * jc + 2];
+    const double b0 = d2n[3 * jv], b1 = d2n[3 * jv + 1], b2 = d2n[3 * jv + 2];
+    const double dnj = dn[jv];
+    // D2H / D2N stencil rows j-1, j, j+1; rows 0..2 at the bottom, the last three at the top
+    double2 u0 = (t > 0) ? UW[0][t - 1] : uP, u1 = UW[0][t], u2 = (t + 1 < SEG) ? UW[0][t + 1] : uN;
+    double2 w0 = (t > 0) ? UW[1][t - 1] : wP, w1 = UW[1][t], w2 = (t + 1 < SEG) ? UW[1][t + 1] : wN;
+    double2 x0 = (t > 0) ? V[0][t - 1] : vP, x1 = V[0][t], x2 = (t + 1 < SEG) ? V[0][t + 1] : vN;
+    if (t == 0) {
+      const bool b = (j == 0);
+      u0 = sel(b, UW[0][0], u0); u1 = sel(b, UW[0][1], u1); u2 = sel(b, UW[0][2], u2);
+      w0 = sel(b, UW[1][0], w0); w1 = sel(b, UW[1][1], w1); w2 = sel(b, UW[1][2], w2);
+      x0 = sel(b, V[0][0], x0); x1 = sel(b, V[0][1], x1); x2 = sel(b, V[0][2], x2);
+    }
+    if (t >= 2) {
+      const bool b = (j == ny);
+      u0 = sel(b, UW[0][t - 2], u0); u1 = sel(b, UW[0][t - 1], u1); u2 = sel(b, UW[0][t], u2);
+      w0 = sel(b, UW[1][t - 2], w0); w1 = sel(b, UW[1][t - 1], w1); w2 = sel(b, UW[1][t], w2);
+    }
+    if (t >= 1) {  // last v row: rows ny-3 .. ny-1
+      const bool b = (j == ny - 1);
+      x0 = sel(b, (t >= 2) ? V[0][t - 2] : vP, x0);
+      x1 = sel(b, V[0][t - 1], x1);
+      x2 = sel(b, V[0][t], x2);
+    }
+    const double2 cu = UW[0][t], cv = V[0][t], cw = UW[1][t];
+    double2 o0 = cadd(cu, rmul(coef2, neg(n0)));
+    double2 o1 = cadd(cv, rmul(coef2, neg(n1)));
+    double2 o2 = cadd(cw, rmul(coef2, neg(n2)));
+    const double2 du = cadd(cadd(rmul(a0, u0), rmul(a1, u1)), rmul(a2, u2));
+    const double2 dw = cadd(cadd(rmul(a0, w0), rmul(a1, w1)), rmul(a2, w2));
+    o0 = cadd(o0, rmul(vis_exp2, csub(du, rmul(k2, cu))));
+    o2 = cadd(o2, rmul(vis_exp2, csub(dw, rmul(k2, cw))));
+    const double2 dv = cadd(cadd(rmul(b0, x0), rmul(b1, x1)), rmul(b2, x2));
+    o1 = sel(j < ny, cadd(o1, rmul(vis_exp2, csub(dv, rmul(k2, cv)))), o1);
+    const double2 Pj = PN[t], Pj1 = (t + 1 < SEG) ? PN[t + 1] : pnN;
+    const bool inner = !mean && j >= 1 && j <= ny - 1;
+    o0 = sel(inner, csub(o0, imul(rk_press2 * kxv, Pj)), o0);
+    o2 = sel(inner, csub(o2, imul(rk_press2 * kzv, Pj)), o2);
+    o1 = sel(inner && j <= ny - 2, csub(o1, rmul(rk_press2 * dnj, csub(Pj1, Pj))), o1);
+    if (j <= ny) {
+      sto(R, OC(0, j), o0);
+      sto(R, OC(1, j), o1);
+      sto(R, OC(2, j), o2);
+    }
+    SEG_FENCE(t);
+  }
+}
+
+// Modes per wavefront of the segmented solve for ny+1 rows (segments of 9
+// rows; the last active one needs 3..9 rows), 0 if the column does not fit.
+int solve_seg_rows(int ny) {
+  const int nyp = ny + 1, ns = (nyp + 8) / 9, rl = nyp - (ns - 1) * 9;
+  if (ny < 8 || rl < 3) return 0;
+  if (ns <= 8) return 8;
+  if (ns <= 16) return 4;
+  return 0;
+}
+
 // ---- direct global-to-LDS row ring (the DMA variant of k_solve3) ----
 //
 // Each pass streams its rows through a ring of R slots in the wavefront's
@@ -905,8 +1407,18 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   long nm = (long)a.nxh * a.nz;
   dim3 grid((unsigned)((nm + 63) / 64));
   const int bl = a.bl, lanes = a.lanes;
-  // the component-parallel kernel needs the byte offsets of one array to fit 32 bits
-  if (lanes == 3 && 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9) {
+  // the segmented and component-parallel kernels need the byte offsets of one
+  // array to fit 32 bits
+  const bool off32 = 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9;
+  const int seg_modes = a.seg ? solve_seg_rows(a.ny) : 0;
+  if (seg_modes && off32) {
+    dim3 gs((unsigned)((nm + seg_modes - 1) / seg_modes));
+    if (seg_modes == 8) hipLaunchKernelGGL((k_solve_seg<8, 9>), gs, dim3(64), 0, s, ops, a);
+    else hipLaunchKernelGGL((k_solve_seg<4, 9>), gs, dim3(64), 0, s, ops, a);
+    LAUNCH_CHECK("k_solve_seg");
+    return;
+  }
+  if (lanes == 3 && off32) {
     dim3 g3((unsigned)((nm + 19) / 20));
     if (a.dma) hipLaunchKernelGGL((k_solve3<8, true>), g3, dim3(64), SOLVE_RING_BYTES, s, ops, a);
     else if (bl == 4) hipLaunchKernelGGL((k_solve3<4, false>), g3, dim3(64), 0, s, ops, a);

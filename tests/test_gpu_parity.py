@@ -161,10 +161,11 @@ def test_rhs_solve_single(g0, rk):
 
 @pytest.mark.parametrize("variant", [(3, 8, 1), (3, 8, 0), (3, 16, 0), (1, 8, 0)])
 def test_rhs_solve_variants(g0, variant):
-    """Every solve kernel variant (lanes per mode, row block, LDS ring) against
-    the oracle for the three substeps; the ring and the register-blocked loads
-    of the 3-lane kernel run the same recurrences (the compiler may contract
-    multiply-adds differently in the two) and agree to round-off."""
+    """Every sweep-based solve kernel variant (lanes per mode, row block, LDS
+    ring; the segmented solve switched off) against the oracle for the three
+    substeps; the ring and the register-blocked loads of the 3-lane kernel run
+    the same recurrences (the compiler may contract multiply-adds differently
+    in the two) and agree to round-off."""
     lanes, bl, dma = variant
     rng = np.random.default_rng(40)
     st = O.initial_state(g0, init_cfg0(O, g0, seed=9, amp=1e-2), dt=0.01)
@@ -177,6 +178,7 @@ def test_rhs_solve_variants(g0, variant):
         outs = []
         for d in ((dma, 1 - dma) if lanes == 3 else (dma,)):
             ch = make_channel(g0)
+            ch.set_tuning("solve_seg", 0)
             ch.set_tuning("solve_lanes", lanes)
             ch.set_tuning("solve_bl", bl)
             ch.set_tuning("solve_dma", d)
@@ -192,6 +194,50 @@ def test_rhs_solve_variants(g0, variant):
         assert rel_l2(d4, O.to_fortran_data4(ref.P)) < 1e-11, rk
         for x, y in zip(outs[0], outs[-1]):
             assert rel_l2(x, y) <= 1e-14, rk
+
+
+@pytest.mark.parametrize("ny", [33, 65, 97, 129])
+def test_rhs_solve_segmented(ny):
+    """The segmented column solve (partitioned tridiagonal solves, columns in
+    registers) against the oracle for the three substeps, with a perturbed RK
+    register and wall data on both walls, at row counts that use 8 modes per
+    wavefront (ny = 33 with idle segments, 65) and 4 (97, 129; 11 and 15
+    segments); and against the sweep-based 3-lane solve to 1e-12."""
+    g = O.make_grid(16, ny, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1, dt_flag=0)
+    rng = np.random.default_rng(ny)
+    s = __import__("helpers").synth()
+    U0 = s.laminar_perturbed(g.nx, g.ny, g.nz, g.yh, g.y, seed=ny, amp=1e-2, kmax=4.0)
+    st = O.initial_state(g, U0, dt=0.01)
+    cbc_top = 1e-3 * (rng.standard_normal((3, g.nxh, g.nz)) + 1j * rng.standard_normal((3, g.nxh, g.nz)))
+    cbc_bot = 1e-3 * (rng.standard_normal((3, g.nxh, g.nz)) + 1j * rng.standard_normal((3, g.nxh, g.nz)))
+    cbc_top[:, 0, 0] = cbc_top[:, 0, 0].real
+    cbc_bot[:, 0, 0] = cbc_bot[:, 0, 0].real
+    for rk in (1, 2, 3):
+        NL, _ = O.nonlinear(g, st.U, step=1, rk_step=rk, dt=0.01)
+        s0 = st.copy()
+        s0.U = NL
+        s0.R = s0.R + 1e-3 * (rng.standard_normal(s0.R.shape) + 1j * rng.standard_normal(s0.R.shape))
+        s0.R[1, g.ny] = 0.0
+        s0.cbc_top, s0.cbc_bot = cbc_top, cbc_bot
+        outs = []
+        for seg in (1, 0):
+            ch = make_channel(g)
+            ch.set_tuning("solve_seg", seg)
+            ch.set_state(data1=NL, data2=O.to_fortran_data2(s0.R), data4=O.to_fortran_data4(s0.P))
+            ch.set_scalars(dt=s0.dt, p_grad=s0.p_grad, d_um_dy=s0.d_um_dy)
+            ch.set_wall_bc(cbc_top, cbc_bot)
+            ch.rhs_solve(1, rk)
+            outs.append((ch.get_state(), ch.get_scalars()))
+            ch.close()
+        ref = O.rhs_solve(g, s0.copy(), rk)
+        (d1, d2, d4), sc = outs[0]
+        assert rel_l2(d1, ref.U) < TOL1, rk
+        assert rel_l2(d2, O.to_fortran_data2(ref.R)) < TOL1, rk
+        assert rel_l2(d4, O.to_fortran_data4(ref.P)) < 1e-11, rk
+        assert rel_l2(sc["d_um_dy"], ref.d_um_dy) < TOL1, rk
+        assert abs(sc["p_grad"] - ref.p_grad) <= 1e-12 * abs(ref.p_grad), rk
+        for x, y in zip(outs[0][0], outs[1][0]):
+            assert rel_l2(x, y) <= 1e-12, rk
 
 
 def test_initialize_flow(g0):
