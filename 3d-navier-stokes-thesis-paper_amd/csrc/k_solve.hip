@@ -447,155 +447,153 @@ __device__ __forceinline__ double2 shfl2(double2 v, int src) {
 // velocity components and the pressure correction, in registers.  The last
 // segment holds the remaining 3..SEG rows; rows past ny are identity padding.
 // Each tridiagonal system (u and w with D2H, v with D2N, the Neumann Poisson
-// system) is solved by partition (seg_solve):
+// system) is solved by partition (seg_eliminate, seg_separators, seg_backsub):
 //   1. every segment eliminates its interior rows top-down, carrying the
 //      coupling to the separator below it (the previous segment's last row)
 //      as a second right-hand side; the coupling to the separator above it
 //      (its own last row, except in the last segment) stays in c';
 //   2. with the interiors eliminated, the separators form a tridiagonal
-//      system of SEGS-1 unknowns per mode; its rows go through LDS and every
-//      lane of the mode solves it;
+//      system of (active segments - 1) unknowns per mode, solved by parallel
+//      cyclic reduction over lane shuffles;
 //   3. every segment back-substitutes with both separators known.
-// The state is read once (NL, R and, at rk_step 1, P for the right-hand side;
-// P and NL again for the pressure update and the next RK register) and
-// written once (u, P, R): no scratch array.  The arithmetic per row is that
+// The state is read once (NL, R and P; the NL and P rows are kept in LDS for
+// the pressure update and the next RK register) and written once (u, P, R):
+// no scratch array.  The arithmetic per row is that
 // of k_solve; only the elimination order of the solves differs, so results
 // agree with it to rounding.
 // ---------------------------------------------------------------------------
 
 // Partitioned solve of NC right-hand sides X[c][0..SEG) that share one
-// matrix; coef(t, a, b, c) gives row t of this lane's segment; ns segments
-// are active.  red holds 64 * (3 + 2*NC) doubles of LDS.
-template <int MODES, int SEG, int NC, class CoefF>
-__device__ __forceinline__ void seg_solve(double2 (&X)[NC][SEG], CoefF coef, double* red, int lane, int ns) {
-  constexpr int W = 3 + 2 * NC;
-  const int s = lane / MODES, q = lane - (lane / MODES) * MODES;
-  const bool last = (s >= ns - 1);  // the last active segment (and idle lanes past it)
-  double cp[SEG], gp[SEG];
-  double aT = 0.0, bT = 1.0, cT = 0.0;
+// matrix, in three calls so that independent systems can be interleaved:
+//   seg_eliminate  - step 1 (coef(t, a, b, c) gives row t of this lane's
+//                    segment) and the relation of the segment's row 0 to
+//                    its two separators;
+//   seg_separators - step 2: the separator equations, solved by parallel
+//                    cyclic reduction across the segment lanes of each mode
+//                    (log2 of the segment count steps of lane shuffles);
+//   seg_backsub    - step 3.
+// ns segments are active; lanes past them carry identity rows.
+template <int SEG, int NC>
+struct SegSys {
+  double cp[SEG], gp[SEG];  // c' and the coupling to the separator below
+  double aT, bT, cT;        // row of this segment's separator (its last row)
   double2 fT[NC];
+  double2 Y[NC];            // row 0: x = Y + VL xl + VR xr
+  double VL, VR;
+};
+
+template <int SEG, int NC, class CoefF>
+__device__ __forceinline__ void seg_eliminate(double2 (&X)[NC][SEG], SegSys<SEG, NC>& S, CoefF coef, bool last) {
+  S.aT = 0.0; S.bT = 1.0; S.cT = 0.0;
 #pragma unroll
-  for (int c = 0; c < NC; ++c) fT[c] = X[c][SEG - 1];
-  // 1. elimination; gp carries the coupling to the separator below
+  for (int c = 0; c < NC; ++c) S.fT[c] = X[c][SEG - 1];
 #pragma unroll
   for (int t = 0; t < SEG; ++t) {
     double a_, b_, c_;
     coef(t, a_, b_, c_);
-    if (t == SEG - 1) { aT = a_; bT = b_; cT = c_; }
-    const double bp = (t == 0) ? b_ : b_ - a_ * cp[t - 1];
+    if (t == SEG - 1) { S.aT = a_; S.bT = b_; S.cT = c_; }
+    const double bp = (t == 0) ? b_ : b_ - a_ * S.cp[t - 1];
     const double inv = 1.0 / bp;
-    cp[t] = c_ * inv;
-    gp[t] = (t == 0) ? (-a_) * inv : (-(a_ * gp[t - 1])) * inv;
+    S.cp[t] = c_ * inv;
+    S.gp[t] = (t == 0) ? (-a_) * inv : (-(a_ * S.gp[t - 1])) * inv;
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       X[c][t] = (t == 0) ? rmul(inv, X[c][t]) : rmul(inv, csub(X[c][t], rmul(a_, X[c][t - 1])));
   }
   // the separator row itself: x = 0 + 0 * xl - (-1) * xr
   if (!last) {
-    cp[SEG - 1] = -1.0;
-    gp[SEG - 1] = 0.0;
+    S.cp[SEG - 1] = -1.0;
+    S.gp[SEG - 1] = 0.0;
 #pragma unroll
     for (int c = 0; c < NC; ++c) X[c][SEG - 1] = cz();
   }
-  // row 0 of the segment in terms of the separators: x0 = Y + VL xl + VR xr
-  double2 Y[NC];
-  double VL = gp[SEG - 1], VR = -cp[SEG - 1];
+  S.VL = S.gp[SEG - 1];
+  S.VR = -S.cp[SEG - 1];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) Y[c] = X[c][SEG - 1];
+  for (int c = 0; c < NC; ++c) S.Y[c] = X[c][SEG - 1];
 #pragma unroll
   for (int t = SEG - 2; t >= 0; --t) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) Y[c] = csub(X[c][t], rmul(cp[t], Y[c]));
-    VL = gp[t] - cp[t] * VL;
-    VR = -cp[t] * VR;
+    for (int c = 0; c < NC; ++c) S.Y[c] = csub(X[c][t], rmul(S.cp[t], S.Y[c]));
+    S.VL = S.gp[t] - S.cp[t] * S.VL;
+    S.VR = -S.cp[t] * S.VR;
   }
-  // 2. separator s (this segment's last row) couples rows SEG-2 here and 0 of segment s+1
-  const int nb = last ? lane : lane + MODES;
-  const double VLn = __shfl(VL, nb, 64), VRn = __shfl(VR, nb, 64);
-  double2 Yn[NC];
+}
+
+template <int MODES, int SEG, int NC>
+__device__ __forceinline__ void seg_separators(const double2 (&X)[NC][SEG], const SegSys<SEG, NC>& S, int lane,
+                                               int ns, double2 (&xl)[NC], double2 (&xr)[NC]) {
+  constexpr int SEGS = 64 / MODES;
+  const int s = lane / MODES;
+  const bool eq = (s < ns - 1);  // this lane's segment ends in a separator
+  // separator s couples row SEG-2 of this segment and row 0 of segment s+1
+  const int nb = (s + 1 < SEGS) ? lane + MODES : lane;
+  const double VLn = __shfl(S.VL, nb, 64), VRn = __shfl(S.VR, nb, 64);
+  double A = eq ? S.aT * S.gp[SEG - 2] : 0.0;
+  double B = eq ? S.aT * (-S.cp[SEG - 2]) + S.bT + S.cT * VLn : 1.0;
+  double C = eq ? S.cT * VRn : 0.0;
+  double2 G[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) Yn[c] = shfl2(Y[c], nb);
-  if (!last) {
-    double* e = red + lane * W;
-    e[0] = aT * gp[SEG - 2];
-    e[1] = aT * (-cp[SEG - 2]) + bT + cT * VLn;
-    e[2] = cT * VRn;
+  for (int c = 0; c < NC; ++c) {
+    const double2 Yn = shfl2(S.Y[c], nb);
+    const double2 g = csub(csub(S.fT[c], rmul(S.aT, X[c][SEG - 2])), rmul(S.cT, Yn));
+    G[c] = eq ? g : cz();
+  }
+  // parallel cyclic reduction; rows outside 0..ns-2 are identity rows
+#pragma unroll
+  for (int d = 1; d < SEGS - 1; d *= 2) {
+    if (d >= ns - 1) continue;
+    const bool hasl = (s >= d), hash = (s + d < SEGS);
+    const int lo = hasl ? lane - d * MODES : lane, hi = hash ? lane + d * MODES : lane;
+    double Al = __shfl(A, lo, 64), Bl = __shfl(B, lo, 64), Cl = __shfl(C, lo, 64);
+    double Ah = __shfl(A, hi, 64), Bh = __shfl(B, hi, 64), Ch = __shfl(C, hi, 64);
+    double2 Gl[NC], Gh[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { Gl[c] = shfl2(G[c], lo); Gh[c] = shfl2(G[c], hi); }
+    if (!hasl) { Al = 0.0; Bl = 1.0; Cl = 0.0; }
+    if (!hash) { Ah = 0.0; Bh = 1.0; Ch = 0.0; }
+    const double k1 = hasl ? A / Bl : 0.0;
+    const double k2 = hash ? C / Bh : 0.0;
+    const double An = -(Al * k1), Cn = -(Ch * k2);
+    const double Bn = B - Cl * k1 - Ah * k2;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const double2 g = csub(csub(fT[c], rmul(aT, X[c][SEG - 2])), rmul(cT, Yn[c]));
-      e[3 + 2 * c] = g.x;
-      e[4 + 2 * c] = g.y;
+      const double2 gl = hasl ? Gl[c] : cz(), gh = hash ? Gh[c] : cz();
+      G[c] = csub(csub(G[c], rmul(k1, gl)), rmul(k2, gh));
     }
+    A = An; B = Bn; C = Cn;
   }
-  __syncthreads();
-  // one lane per mode eliminates the separator system in place: slot 2 takes
-  // c', slots 3.. take f' and then the solution
-  if (s == 0) {
-    double rcp = 0.0;
-    double2 rfp[NC];
-    for (int k = 0; k < ns - 1; ++k) {
-      double* ek = red + (k * MODES + q) * W;
-      const double A = ek[0], B = ek[1], C = ek[2];
-      const double bp = (k == 0) ? B : B - A * rcp;
-      const double inv = 1.0 / bp;
-      rcp = C * inv;
-      ek[2] = rcp;
+  const double rB = 1.0 / B;
+  const int pv = (s > 0) ? lane - MODES : lane;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const double2 g = make_double2(ek[3 + 2 * c], ek[4 + 2 * c]);
-        rfp[c] = (k == 0) ? rmul(inv, g) : rmul(inv, csub(g, rmul(A, rfp[c])));
-        ek[3 + 2 * c] = rfp[c].x;
-        ek[4 + 2 * c] = rfp[c].y;
-      }
-    }
-    for (int k = ns - 3; k >= 0; --k) {
-      double* ek = red + (k * MODES + q) * W;
-      const double rck = ek[2];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const double2 f = make_double2(ek[3 + 2 * c], ek[4 + 2 * c]);
-        rfp[c] = csub(f, rmul(rck, rfp[c]));
-        ek[3 + 2 * c] = rfp[c].x;
-        ek[4 + 2 * c] = rfp[c].y;
-      }
-    }
+  for (int c = 0; c < NC; ++c) {
+    const double2 x = eq ? rmul(rB, G[c]) : cz();
+    const double2 xp = shfl2(x, pv);
+    xr[c] = x;
+    xl[c] = (s > 0 && s < ns) ? xp : cz();
   }
-  __syncthreads();
-  double2 xl[NC], xr[NC];
-  {
-    const double* el = red + ((s > 0 && s < ns ? s - 1 : 0) * MODES + q) * W;
-    const double* er = red + ((last ? 0 : s) * MODES + q) * W;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      xl[c] = (s > 0 && s < ns) ? make_double2(el[3 + 2 * c], el[4 + 2 * c]) : cz();
-      xr[c] = last ? cz() : make_double2(er[3 + 2 * c], er[4 + 2 * c]);
-    }
-  }
-  __syncthreads();  // red is reused by the next system
-  // 3. back substitution
+}
+
+template <int SEG, int NC>
+__device__ __forceinline__ void seg_backsub(double2 (&X)[NC][SEG], const SegSys<SEG, NC>& S, const double2 (&xl)[NC],
+                                            const double2 (&xr)[NC]) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     double2 nx = xr[c];
 #pragma unroll
     for (int t = SEG - 1; t >= 0; --t) {
-      X[c][t] = csub(cadd(X[c][t], rmul(gp[t], xl[c])), rmul(cp[t], nx));
+      X[c][t] = csub(cadd(X[c][t], rmul(S.gp[t], xl[c])), rmul(S.cp[t], nx));
       nx = X[c][t];
     }
   }
 }
 
-// Memory fence for the scheduler every SEG_AHEAD rows of a streaming loop:
-// loads are not moved across it, so a loop keeps at most about SEG_AHEAD
-// rows of loads in flight instead of hoisting every row's loads.
-#ifndef SEG_AHEAD
-#define SEG_AHEAD 3
-#endif
-#define SEG_FENCE(t) do { if (((t) + 1) % SEG_AHEAD == 0) asm volatile("" ::: "memory"); } while (0)
-
 template <int MODES, int SEG>
 __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   constexpr int SEGS = 64 / MODES;
-  __shared__ double red[64 * 7];
+  // NL and P rows of this lane's segment, kept from the right-hand-side pass
+  // for the pressure update and the next RK register: [row][comp][lane]
+  __shared__ double2 stash[SEG][4][64];
   const int lane = threadIdx.x;
   const int s = lane / MODES, q = lane - (lane / MODES) * MODES;
   const long nm = (long)a.nxh * a.nz;
@@ -679,26 +677,27 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   auto rhs = [&](auto first_c) {
     constexpr bool FIRST = decltype(first_c)::value;
     const int jb = min(j0, ny), jm = max(min(j0 - 1, ny), 0);  // rows clamped for idle lanes
-    double2 Rm0 = cz(), Rm1 = cz(), Rm2 = cz(), Pc = cz();
-    if (FIRST) {
-      Rm0 = ldo(R, OC(0, jm)); Rm1 = ldo(R, OC(1, jm)); Rm2 = ldo(R, OC(2, jm));
-      Pc = ldo(P, OM(jb));
-    }
+    double2 Rm0 = cz(), Rm1 = cz(), Rm2 = cz();
+    double2 Pc = ldo(P, OM(jb));
+    if (FIRST) { Rm0 = ldo(R, OC(0, jm)); Rm1 = ldo(R, OC(1, jm)); Rm2 = ldo(R, OC(2, jm)); }
     double2 Rc0 = ldo(R, OC(0, jb)), Rc1 = ldo(R, OC(1, jb)), Rc2 = ldo(R, OC(2, jb));
 #pragma unroll
     for (int t = 0; t < SEG; ++t) {
       const int j = j0 + t;
       const int jc = min(j, ny), jn = min(j + 1, ny), jv = min(j, ny - 1);
       const double2 n0 = ldo(NL, OC(0, jc)), n1 = ldo(NL, OC(1, jc)), n2 = ldo(NL, OC(2, jc));
+      stash[t][0][lane] = n0;
+      stash[t][1][lane] = n1;
+      stash[t][2][lane] = n2;
+      stash[t][3][lane] = Pc;
       double2 r0 = cz(), r1 = cz(), r2 = cz();
       if (t + 1 < SEG || FIRST) { r0 = ldo(R, OC(0, jn)); r1 = ldo(R, OC(1, jn)); r2 = ldo(R, OC(2, jn)); }
       double2 v0 = cadd(Rc0, rmul(coef, neg(n0)));
       double2 v1 = cadd(Rc1, rmul(coef, neg(n1)));
       double2 v2 = cadd(Rc2, rmul(coef, neg(n2)));
-      double2 pn = cz();
+      const double2 pn = (t + 1 < SEG || FIRST) ? ldo(P, OM(jn)) : cz();
       if (FIRST) {
         // add_explicit_diffusive_terms(cv, cru) (:339-351)
-        pn = ldo(P, OM(jn));
         const double a0 = d2h[3 * jc], a1 = d2h[3 * jc + 1], a2 = d2h[3 * jc + 2];
         const double b0 = d2n[3 * jv], b1 = d2n[3 * jv + 1], b2 = d2n[3 * jv + 2];
         const double dnj = dn[jv];
@@ -726,29 +725,37 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       Rm0 = Rc0; Rm1 = Rc1; Rm2 = Rc2;
       Rc0 = r0; Rc1 = r1; Rc2 = r2;
       Pc = pn;
-      SEG_FENCE(t);
     }
   };
   if (r == 0) rhs(std::true_type{});
   else rhs(std::false_type{});
 
-  // ---------------- Helmholtz solves ----------------
-  seg_solve<MODES, SEG, 2>(UW, [&](int t, double& a_, double& b_, double& c_) {
-    const int j = j0 + t, jc = min(j, ny);
-    const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
-    const bool wall = (j == 0 || j >= ny);
-    a_ = wall ? 0.0 : -vis_imp * h0;
-    b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
-    c_ = wall ? 0.0 : -vis_imp * h2;
-  }, red, lane, ns);
-  seg_solve<MODES, SEG, 1>(V, [&](int t, double& a_, double& b_, double& c_) {
-    const int j = j0 + t, jv = min(j, ny - 1);
-    const double h0 = d2n[3 * jv], h1 = d2n[3 * jv + 1], h2 = d2n[3 * jv + 2];
-    const bool wall = (j == 0 || j >= ny - 1);
-    a_ = wall ? 0.0 : -vis_imp * h0;
-    b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
-    c_ = wall ? 0.0 : -vis_imp * h2;
-  }, red, lane, ns);
+  // ---------------- Helmholtz solves (u and w with D2H, v with D2N) ----------------
+  {
+    SegSys<SEG, 2> SH;
+    SegSys<SEG, 1> SN;
+    seg_eliminate(UW, SH, [&](int t, double& a_, double& b_, double& c_) {
+      const int j = j0 + t, jc = min(j, ny);
+      const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
+      const bool wall = (j == 0 || j >= ny);
+      a_ = wall ? 0.0 : -vis_imp * h0;
+      b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
+      c_ = wall ? 0.0 : -vis_imp * h2;
+    }, s >= ns - 1);
+    seg_eliminate(V, SN, [&](int t, double& a_, double& b_, double& c_) {
+      const int j = j0 + t, jv = min(j, ny - 1);
+      const double h0 = d2n[3 * jv], h1 = d2n[3 * jv + 1], h2 = d2n[3 * jv + 2];
+      const bool wall = (j == 0 || j >= ny - 1);
+      a_ = wall ? 0.0 : -vis_imp * h0;
+      b_ = wall ? 1.0 : -vis_imp * h1 + alpha;
+      c_ = wall ? 0.0 : -vis_imp * h2;
+    }, s >= ns - 1);
+    double2 xlH[2], xrH[2], xlN[1], xrN[1];
+    seg_separators<MODES, SEG, 2>(UW, SH, lane, ns, xlH, xrH);
+    seg_separators<MODES, SEG, 1>(V, SN, lane, ns, xlN, xrN);
+    seg_backsub(UW, SH, xlH, xrH);
+    seg_backsub(V, SN, xlN, xrN);
+  }
 #pragma unroll
   for (int t = 0; t < SEG; ++t)  // mean mode (:261-263)
     V[0][t] = sel(mean, (j0 + t < ny) ? bot1 : cz(), V[0][t]);
@@ -772,7 +779,8 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   {
     const double dn0 = dn[0], dn1 = dn[1], dh1 = dh[1];
     const double dnt = dn[ny - 1], dnt2 = dn[ny - 2], dht = dh[ny - 1];
-    seg_solve<MODES, SEG, 1>(PH, [&](int t, double& a_, double& b_, double& c_) {
+    SegSys<SEG, 1> SP;
+    seg_eliminate(PH, SP, [&](int t, double& a_, double& b_, double& c_) {
       const int j = j0 + t, jc = min(j, ny);
       const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
       a_ = h0; b_ = h1 + alp; c_ = h2;
@@ -785,7 +793,10 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       a_ = (j == 0) ? 0.0 : (j == ny) ? -dnt : (j > ny) ? 0.0 : a_;
       b_ = (j == 0) ? -dn0 : (j == ny) ? dnt : (j > ny) ? 1.0 : b_;
       c_ = (j == 0) ? dn0 : (j >= ny) ? 0.0 : c_;
-    }, red, lane, ns);
+    }, s >= ns - 1);
+    double2 xl[1], xr[1];
+    seg_separators<MODES, SEG, 1>(PH, SP, lane, ns, xl, xr);
+    seg_backsub(PH, SP, xl, xr);
   }
   // wall re-extrapolation (SOLVE_P :335-336, zero wall data)
   {
@@ -819,7 +830,7 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       double2 s0 = (t > 0) ? PH[0][t - 1] : phP, s1 = f0, s2 = fn;
       if (t == 0) { s0 = sel(j == 0, PH[0][0], s0); s1 = sel(j == 0, PH[0][1], s1); s2 = sel(j == 0, PH[0][2], s2); }
       if (t >= 2) { s0 = sel(j == ny, PH[0][t - 2], s0); s1 = sel(j == ny, PH[0][t - 1], s1); s2 = sel(j == ny, f0, s2); }
-      const double2 Pold = ldo(P, OM(jc));
+      const double2 Pold = stash[t][3][lane];
       const double h0 = d2h[3 * jc], h1 = d2h[3 * jc + 1], h2 = d2h[3 * jc + 2];
       const double dnj = dn[jv];
       const bool inner = !mean && j >= 1 && j <= ny - 1;
@@ -832,7 +843,6 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       V[0][t] = cv;
       UW[1][t] = cw;
       PN[t] = pn;
-      SEG_FENCE(t);
     }
   }
   if (valid) {
@@ -882,7 +892,7 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
 #pragma unroll
   for (int t = 0; t < SEG; ++t) {
     const int j = j0 + t, jc = min(j, ny), jv = min(j, ny - 1);
-    const double2 n0 = ldo(NL, OC(0, jc)), n1 = ldo(NL, OC(1, jc)), n2 = ldo(NL, OC(2, jc));
+    const double2 n0 = stash[t][0][lane], n1 = stash[t][1][lane], n2 = stash[t][2][lane];
     const double a0 = d2h[3 * jc], a1 = d2h[3 * jc + 1], a2 = d2h[3 * jc + 2];
     const double b0 = d2n[3 * jv], b1 = d2n[3 * jv + 1], b2 = d2n[3 * jv + 2];
     const double dnj = dn[jv];
@@ -927,7 +937,6 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       sto(R, OC(1, j), o1);
       sto(R, OC(2, j), o2);
     }
-    SEG_FENCE(t);
   }
 }
 
