@@ -67,22 +67,44 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ pac
   packed[blk * 64 + l] = v;
 }
 
-// GELU for the bf16 inference path: erf by Abramowitz & Stegun 7.1.26
-// (|error| <= 1.5e-7, with the hardware exp and reciprocal), far inside the
-// 2^-8 relative rounding of the bf16 result it feeds.
+// GELU for the bf16 inference path, without transcendental instructions:
+// GELU(x) = x/2 + h(x) with h(x) = x erf(x/sqrt 2)/2, an even function, taken
+// as a degree-10 polynomial in s = 2x^2/4.5^2 - 1 for |x| <= 4.5 (a
+// least-squares Chebyshev fit made for this kernel, converted to monomials in
+// s; error <= 2.4e-5 in float32) and GELU(x) = max(x, 0) beyond (error
+// <= 1.6e-5).  Both are far inside the 2^-9 relative rounding of the bf16
+// result the value feeds.
 __device__ __forceinline__ float gelu_bf(float x) {
-  const float z = fabsf(x) * 0.70710678118654752f;
-  const float t = __fdividef(1.0f, 1.0f + 0.3275911f * z);
-  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-  const float e = 1.0f - poly * __expf(-z * z);
-  return 0.5f * x * (1.0f + copysignf(e, x));
+  const float s = fminf(fmaf(x * x, 2.0f / 20.25f, -1.0f), 1.0f);
+  float h = -1.629356321e-02f;
+  h = fmaf(h, s, 3.052472295e-02f);
+  h = fmaf(h, s, -1.959422663e-02f);
+  h = fmaf(h, s, 3.656191228e-02f);
+  h = fmaf(h, s, -8.185058262e-02f);
+  h = fmaf(h, s, 1.049085769e-01f);
+  h = fmaf(h, s, -1.180014507e-01f);
+  h = fmaf(h, s, 1.458142942e-01f);
+  h = fmaf(h, s, -2.279429362e-01f);
+  h = fmaf(h, s, 8.071679794e-01f);
+  h = fmaf(h, s, 1.588666266e+00f);
+  const float g = fmaf(0.5f, x, h);
+  return (fabsf(x) > 4.5f) ? fmaxf(x, 0.0f) : g;
+}
+
+// Four float32 values rounded to bf16 (nearest even) by the hardware
+// conversion, as one MFMA operand.
+typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4 to_bf16x4(float a, float b, float c, float d) {
+  const bf16x2h lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
+  const u32x2 w = {__builtin_bit_cast(unsigned int, lo), __builtin_bit_cast(unsigned int, hi)};
+  return __builtin_bit_cast(bf16x4, w);
 }
 
 // Bias + GELU on a result tile, rounded to bf16 as the next B operand.
 __device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ bias, int ot, int lane) {
-  bf16x4 r;
-  for (int i = 0; i < 4; ++i) r[i] = (short)f2bf(gelu_bf(acc[i] + bias[ot * 16 + 4 * (lane >> 4) + i]));
-  return r;
+  const float* bb = bias + ot * 16 + 4 * (lane >> 4);
+  return to_bf16x4(gelu_bf(acc[0] + bb[0]), gelu_bf(acc[1] + bb[1]), gelu_bf(acc[2] + bb[2]), gelu_bf(acc[3] + bb[3]));
 }
 
 #define MLP_NT 2  // point tiles of 16 per wavefront
@@ -108,12 +130,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
 #pragma unroll
     for (int nt = 0; nt < MLP_NT; ++nt) {
       const long pt = base + 16 * nt + col;
-      bf16x4 b;
+      float v[4];
       for (int t = 0; t < 4; ++t) {
         const int f = 4 * q + t;
-        b[t] = (short)f2bf((pt < n && f < 10) ? x[pt * 10 + f] : 0.0f);
+        v[t] = (pt < n && f < 10) ? x[pt * 10 + f] : 0.0f;
       }
-      h[nt][0] = b;
+      h[nt][0] = to_bf16x4(v[0], v[1], v[2], v[3]);
     }
     f32x4 acc[MLP_NT][1];
     bf16x4 g[MLP_NT][8];
@@ -402,6 +424,22 @@ int mguard(chan_mlp* m, F&& f) {
 
 }  // namespace
 
+// Workgroups of the persistent inference kernel: as many as fit on the device
+// at once (its 75 KB of LDS weights allow two per CU), so that every
+// workgroup loads the weights once.
+static int mlp_blocks(long n) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = v;
+    else
+      cus = 256;
+  }
+  const long tiles = (n + 16 * MLP_NT - 1) / (16 * MLP_NT);
+  return (int)std::max<long>(1, std::min<long>(2L * cus, (tiles + 3) / 4));
+}
+
 void mlp_prepare_device(chan_mlp* m) {
   pack(m);
   mchk(hipStreamSynchronize(m->stream), "sync");
@@ -413,9 +451,7 @@ void mlp_infer_device(chan_mlp* m, long n, const float* x, float* y, hipStream_t
   if (!m->packed_ok) mlp_prepare_device(m);
   if (n <= 0) return;
   if (n > 0x7fffffffL) throw chan_error("too many points for one inference");
-  const long tiles = (n + 16 * MLP_NT - 1) / (16 * MLP_NT);
-  const int blocks = (int)std::max<long>(1, std::min<long>(1024, (tiles + 3) / 4));
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, m->d_packed, m->d_p, (int)n, x, y);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(256), 0, s, m->d_packed, m->d_p, (int)n, x, y);
   LAUNCH_CHECK("k_mlp_infer");
 }
 
@@ -487,9 +523,8 @@ int chan_mlp_infer(chan_mlp* m, int n, const float* x, float* y) {
     ensure_cap(m, n);
     pack(m);
     mchk(hipMemcpyAsync(m->d_x, x, sizeof(float) * 10 * n, hipMemcpyHostToDevice, m->stream), "copy");
-    const long tiles = ((long)n + 16 * MLP_NT - 1) / (16 * MLP_NT);
-    const int blocks = (int)std::max<long>(1, std::min<long>(1024, (tiles + 3) / 4));
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, m->stream, m->d_packed, m->d_p, n, m->d_x, m->d_y);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(256), 0, m->stream, m->d_packed, m->d_p, n, m->d_x,
+                       m->d_y);
     LAUNCH_CHECK("k_mlp_infer");
     mchk(hipMemcpyAsync(y, m->d_y, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, m->stream), "copy");
     mchk(hipStreamSynchronize(m->stream), "sync");
