@@ -16,9 +16,10 @@
 //   face difference dh(k) (v(k) - v(k-1)).
 // Models:
 //   1  dynamic Smagorinsky: tau_ij = -2 C(y) D^2 |S| S_ij, |S| = sqrt(2 S:S),
-//      D = (dx dz (y(k)-y(k-1)))^(1/3), C(y) from the Germano identity with a
-//      sharp test filter at half the resolved x and z wavenumbers (width ratio
-//      2), least squares over each plane (Lilly), clipped at zero; C(y) is
+//      D = (dx dz (y(k)-y(k-1)))^(1/3), C(y) from the Germano identity with
+//      the trapezoidal test filter (1/4, 1/2, 1/4) along x and z on the
+//      dealiased grid (width ratio 2 in the model), least squares over each
+//      plane (Lilly), clipped at zero; C(y) is
 //      evaluated at the first substep of each time step and held for the
 //      other two;
 //   2  learned: an MLP maps (G[0..8]/|G|, 1-|yh|) to tau_ij / (D^2 |G|^2);
@@ -74,19 +75,6 @@ __global__ void k_les_grad(DevOps ops, int ny, int nxh, int nz, const double2* _
   out(11, w);
 }
 
-// sharp test filter: keeps kx index < kxc and |kz index| < kzc; src -> dst
-__global__ void k_les_mask(const double2* __restrict__ src, double2* __restrict__ dst, int nxh, int nz, int kxc,
-                           int kzc) {
-  const long nm = (long)nxh * nz;
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nm) return;
-  const long o = (long)blockIdx.y * nm + m;
-  const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
-  const int kzi = iz <= nz / 2 ? iz : iz - nz;
-  const bool keep = ix < kxc && (kzi < 0 ? -kzi : kzi) < kzc;
-  dst[o] = keep ? src[o] : make_double2(0.0, 0.0);
-}
-
 struct Strain {
   double s[6];  // S_ij in the symmetric order
   double mag;   // |S| = sqrt(2 S:S)
@@ -109,51 +97,77 @@ __device__ __forceinline__ double contract(const double* a, const double* b) {
   return a[0] * b[0] + a[3] * b[3] + a[5] * b[5] + 2.0 * (a[1] * b[1] + a[2] * b[2] + a[4] * b[4]);
 }
 
-// products for the Germano identity: P[0..5] = u_i u_j, P[6..11] = |S| S_ij
-__global__ void k_les_prod(int ny, long ps, const double* __restrict__ Ph, double* __restrict__ P) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Dynamic procedure with the trapezoidal test filter (1/4, 1/2, 1/4) along x,
+// then along z, of the closure fields on the dealiased grid, fused: a
+// workgroup takes a 16 x 16 tile of one interior plane with a one-point
+// (periodic) halo in LDS, holding the 12 closure fields and the 12 products
+// u_i u_j, |S| S_ij at every halo point; every point of the tile filters
+// them, forms L_ij = T(u_i u_j) - T(u_i) T(u_j),
+// M_ij = 2 D^2 (T(|S| S_ij) - 4 |S^| S^_ij), and the workgroup sums L:M and
+// M:M (partials per tile, reduced per plane by k_les_cs2).
+#define DYN_T 16
+#define DYN_H (DYN_T + 2)
+__global__ __launch_bounds__(256) void k_les_dyn(int ny, int nz2, int nx2, const double* __restrict__ Ph,
+                                                 const double* __restrict__ delta2, double* __restrict__ part) {
+  __shared__ double F[24][DYN_H * DYN_H];
+  __shared__ double red[2][256];
   const int kk = blockIdx.y, nyp = ny + 1;
-  if (i >= ps) return;
-  auto at = [&](const double* b, int f) { return b[((long)f * nyp + kk) * ps + i]; };
-  double* o[12];
-  for (int f = 0; f < 12; ++f) o[f] = P + ((long)f * nyp + kk) * ps + i;
-  if (kk == 0 || kk == ny) {
-    for (int f = 0; f < 12; ++f) *o[f] = 0.0;
-    return;
-  }
-  double g[9], u[3];
-  for (int f = 0; f < 9; ++f) g[f] = at(Ph, f);
-  for (int c = 0; c < 3; ++c) u[c] = at(Ph, 9 + c);
-  const Strain S = strain(g);
-  for (int t = 0; t < 6; ++t) {
-    *o[t] = u[c_ti[t]] * u[c_tj[t]];
-    *o[6 + t] = S.mag * S.s[t];
-  }
-}
-
-// Germano identity per point, partial plane sums of L:M and M:M per block
-// (T = test-filtered fields: T[0..11] = T(P), T[12..23] = T(G))
-__global__ __launch_bounds__(256) void k_les_germano(int ny, long ps, const double* __restrict__ T,
-                                                     const double* __restrict__ delta2, double* __restrict__ part) {
-  const int kk = blockIdx.y, nyp = ny + 1;
+  const long ps = (long)nz2 * nx2;
+  const int ntx = (nx2 + DYN_T - 1) / DYN_T;
+  const int tz0 = (blockIdx.x / ntx) * DYN_T, tx0 = (blockIdx.x % ntx) * DYN_T;
+  const bool inner = (kk > 0 && kk < ny);
   double lm = 0.0, mm = 0.0;
-  if (kk > 0 && kk < ny) {
-    const double d2 = delta2[kk];
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ps; i += (long)gridDim.x * blockDim.x) {
-      auto at = [&](int f) { return T[((long)f * nyp + kk) * ps + i]; };
-      double g[9], u[3], L[6], M[6];
-      for (int f = 0; f < 9; ++f) g[f] = at(12 + f);
-      for (int c = 0; c < 3; ++c) u[c] = at(21 + c);
+  if (inner) {
+    for (int p = threadIdx.x; p < DYN_H * DYN_H; p += blockDim.x) {
+      const int hz = p / DYN_H, hx = p - hz * DYN_H;
+      int z = tz0 + hz - 1, x = tx0 + hx - 1;
+      z = (z < 0) ? z + nz2 : (z >= nz2 ? z - nz2 : z);
+      x = (x < 0) ? x + nx2 : (x >= nx2 ? x - nx2 : x);
+      const double* src = Ph + (long)kk * ps + (long)z * nx2 + x;
+      double g[9], u[3];
+#pragma unroll
+      for (int f = 0; f < 9; ++f) g[f] = src[(long)f * nyp * ps];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c] = src[(long)(9 + c) * nyp * ps];
       const Strain S = strain(g);
+#pragma unroll
+      for (int f = 0; f < 9; ++f) F[f][p] = g[f];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[9 + c][p] = u[c];
+#pragma unroll
       for (int t = 0; t < 6; ++t) {
-        L[t] = at(t) - u[c_ti[t]] * u[c_tj[t]];
-        M[t] = 2.0 * d2 * (at(6 + t) - 4.0 * S.mag * S.s[t]);
+        F[12 + t][p] = u[c_ti[t]] * u[c_tj[t]];
+        F[18 + t][p] = S.mag * S.s[t];
       }
-      lm += contract(L, M);
-      mm += contract(M, M);
     }
   }
-  __shared__ double red[2][256];
+  __syncthreads();
+  if (inner) {
+    const int lz = threadIdx.x / DYN_T, lx = threadIdx.x - lz * DYN_T;
+    if (tz0 + lz < nz2 && tx0 + lx < nx2) {
+      auto filt = [&](int f) {
+        const double* a = F[f] + lz * DYN_H + lx;
+        const double r0 = 0.25 * a[0] + 0.5 * a[1] + 0.25 * a[2];
+        const double r1 = 0.25 * a[DYN_H] + 0.5 * a[DYN_H + 1] + 0.25 * a[DYN_H + 2];
+        const double r2 = 0.25 * a[2 * DYN_H] + 0.5 * a[2 * DYN_H + 1] + 0.25 * a[2 * DYN_H + 2];
+        return 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
+      };
+      double tg[9], tu[3], L[6], M[6];
+#pragma unroll
+      for (int f = 0; f < 9; ++f) tg[f] = filt(f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tu[c] = filt(9 + c);
+      const Strain St = strain(tg);
+      const double d2 = delta2[kk];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        L[t] = filt(12 + t) - tu[c_ti[t]] * tu[c_tj[t]];
+        M[t] = 2.0 * d2 * (filt(18 + t) - 4.0 * St.mag * St.s[t]);
+      }
+      lm = contract(L, M);
+      mm = contract(M, M);
+    }
+  }
   red[0][threadIdx.x] = lm;
   red[1][threadIdx.x] = mm;
   __syncthreads();
@@ -274,15 +288,11 @@ struct LesState {
   int mode = 0;
   double beta = 1.0;
   chan_mlp* net = nullptr;
-  int kxc = 0, kzc = 0;        // test-filter cut-offs
   long ps = 0, nm = 0, npts = 0;
-  int nblk = 0;                // Germano partial sums per plane
-  double2* G = nullptr;        // [24][nyp][nm]: G, then its test-filtered copy
-  double2* Gt = nullptr;       // [12][nyp][nm]: transformed products
+  int nblk = 0;                // Germano partial sums per plane (tiles of k_les_dyn)
+  double2* G = nullptr;        // [12][nyp][nm]
   double2* Zl = nullptr;       // [12 nyp][nz2][kxpad] transform intermediate
   double* Ph = nullptr;        // [12][nyp][ps]
-  double* P = nullptr;         // [12][nyp][ps]
-  double* T = nullptr;         // [24][nyp][ps]
   double* tau = nullptr;       // [6][nyp][ps]
   double* cs2 = nullptr;       // [nyp]
   double* delta2 = nullptr;    // [nyp]
@@ -318,13 +328,6 @@ void les_c2r(chan_ctx* c, LesState* L, const double2* in, int np, double* out) {
   launch_xc2r(g, c->d_tw, L->Zl, zps, out, L->ps, np, c->stream);
 }
 
-void les_r2c(chan_ctx* c, LesState* L, const double* in, int np, double2* out) {
-  const XZGeom& g = c->gd;
-  const long zps = (long)g.nz2 * g.kxpad;
-  launch_xr2c(g, c->d_tw, in, nullptr, c->ops.kx, L->ps, L->Zl, zps, np, c->stream);
-  launch_zfwd(g, c->d_tw, L->Zl, nullptr, zps, c->ops.kz, out, L->nm, np, 0, c->stream);
-}
-
 // G, its physical fields Ph and (dynamic) the dynamic coefficient
 void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
   const int ny = c->prm.ny, nyp = c->nyp;
@@ -333,20 +336,10 @@ void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
   LAUNCH_CHECK("k_les_grad");
   les_c2r(c, L, L->G, LES_NF * nyp, L->Ph);
   if (!dynamic) return;
-  dim3 gp((unsigned)((L->ps + 255) / 256), nyp);
-  hipLaunchKernelGGL(k_les_prod, gp, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->P);
-  LAUNCH_CHECK("k_les_prod");
-  les_r2c(c, L, L->P, 12 * nyp, L->Gt);
-  dim3 gk((unsigned)((L->nm + 255) / 256), 12 * nyp);
-  double2* Gf = L->G + (long)LES_NF * nyp * L->nm;
-  hipLaunchKernelGGL(k_les_mask, gk, dim3(256), 0, c->stream, L->Gt, L->Gt, c->nxh, c->prm.nz, L->kxc, L->kzc);
-  hipLaunchKernelGGL(k_les_mask, gk, dim3(256), 0, c->stream, L->G, Gf, c->nxh, c->prm.nz, L->kxc, L->kzc);
-  LAUNCH_CHECK("k_les_mask");
-  les_c2r(c, L, L->Gt, 12 * nyp, L->T);
-  les_c2r(c, L, Gf, 12 * nyp, L->T + (long)12 * nyp * L->ps);
-  hipLaunchKernelGGL(k_les_germano, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, L->ps, L->T, L->delta2,
+  const XZGeom& g = c->gd;
+  hipLaunchKernelGGL(k_les_dyn, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, g.nz2, g.nx2, L->Ph, L->delta2,
                      L->part);
-  LAUNCH_CHECK("k_les_germano");
+  LAUNCH_CHECK("k_les_dyn");
   hipLaunchKernelGGL(k_les_cs2, dim3((nyp + 63) / 64), dim3(64), 0, c->stream, ny, L->nblk, L->part, L->cs2);
   LAUNCH_CHECK("k_les_cs2");
 }
@@ -387,7 +380,7 @@ LesState* les_need(chan_ctx* c) {
 
 void les_destroy(LesState* L) {
   if (!L) return;
-  lfree(L->G); lfree(L->Gt); lfree(L->Zl); lfree(L->Ph); lfree(L->P); lfree(L->T); lfree(L->tau);
+  lfree(L->G); lfree(L->Zl); lfree(L->Ph); lfree(L->tau);
   lfree(L->cs2); lfree(L->delta2); lfree(L->ydist); lfree(L->part); lfree(L->X); lfree(L->Y);
   lfree(L->scale); lfree(L->gated);
   delete L;
@@ -437,19 +430,14 @@ int chan_les_configure(chan_ctx* c, int mode, chan_mlp* net, double beta, int ev
     L->beta = beta;
     L->every = every_substep ? 1 : 0;
     L->net = net;
-    L->kxc = c->prm.nx / 4;
-    L->kzc = c->prm.nz / 4;
     L->ps = (long)g.nz2 * g.nx2;
     L->nm = (long)c->nxh * c->prm.nz;
     L->npts = (long)(ny - 1) * L->ps;
-    L->nblk = (int)std::min<long>(64, (L->ps + 255) / 256);
+    L->nblk = (int)(((g.nz2 + DYN_T - 1) / DYN_T) * ((g.nx2 + DYN_T - 1) / DYN_T));
     const long zps = (long)g.nz2 * g.kxpad;
-    lalloc(L->G, (size_t)2 * LES_NF * nyp * L->nm);
-    lalloc(L->Gt, (size_t)12 * nyp * L->nm);
+    lalloc(L->G, (size_t)LES_NF * nyp * L->nm);
     lalloc(L->Zl, (size_t)12 * nyp * zps);
     lalloc(L->Ph, (size_t)LES_NF * nyp * L->ps);
-    lalloc(L->P, (size_t)12 * nyp * L->ps);
-    lalloc(L->T, (size_t)24 * nyp * L->ps);
     lalloc(L->tau, (size_t)6 * nyp * L->ps);
     lalloc(L->cs2, nyp);
     lalloc(L->delta2, nyp);

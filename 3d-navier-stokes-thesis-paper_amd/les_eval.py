@@ -275,6 +275,9 @@ def main():
                          "(default: dynamic Smagorinsky's own a-priori error)")
     ap.add_argument("--out", default="les_eval.json")
     ap.add_argument("--params-out", default="")
+    ap.add_argument("--les-dts", default="",
+                    help="comma-separated LES time steps for the a-posteriori runs (default: the LES dt); "
+                         "the first is the reported one, all are kept under dt_sweep")
     args = ap.parse_args()
 
     cf = load_pkg()
@@ -362,7 +365,6 @@ def main():
     # ---- a posteriori: LES runs against the continued DNS (graph replay for both)
     nsamp = int(round(args.post_time / args.sample))
     k_dns = int(round(args.sample / DNS["dt"]))
-    k_les = int(round(args.sample / LES["dt"]))
     Ul0 = held[-1][0]
     dns.sync()
     acc_d = []
@@ -376,41 +378,70 @@ def main():
         acc_d.append(profiles(Uf))
     mean_d, rms_d, uv_d = [np.mean([a[j] for a in acc_d], axis=0) for j in range(3)]
     post = {"time_units": nsamp * args.sample, "dns_seconds": t_dns,
-            "dns_time_units_per_s": nsamp * args.sample / t_dns}
-    for mode in ("off", "dynamic", "learned"):
+            "dns_time_units_per_s": nsamp * args.sample / t_dns, "dt_sweep": {}}
+    dts = [float(v) for v in args.les_dts.split(",")] if args.les_dts else [LES["dt"]]
+    probe, _, _ = channel(cf, LES)  # CFL of a state: one nonlinear evaluation on a copy
+    def run_les(mode, dt_l):
+        """One a-posteriori LES run; profiles sampled every `sample` time units."""
+        k_les = int(round(args.sample / dt_l))
         les.les_configure(mode, net=net if mode == "learned" else None, beta=args.beta, every_substep=False)
         les.set_state(data1=Ul0)
-        les.set_scalars(dt=LES["dt"], t_total=0.0, p_grad=-2.0 / BOX["re"])
+        les.set_scalars(dt=dt_l, t_total=0.0, p_grad=-2.0 / BOX["re"])
         les.initialize_flow()
         lens = cf.Ensemble([les], nstreams=1)
-        lens.step(1)
-        les.sync()
-        acc_l = []
-        t_les = 0.0
-        for i in range(nsamp):
-            t0 = time.perf_counter()
-            lens.step(k_les if i else k_les - 1)
+        try:
+            lens.step(1)
             les.sync()
-            t_les += time.perf_counter() - t0
-            acc_l.append(profiles(les.get_data1()))
-        lens.close()
+            acc_l = []
+            t_les = 0.0
+            cfl_max = 0.0
+            for i in range(nsamp):
+                t0 = time.perf_counter()
+                lens.step(k_les if i else k_les - 1)
+                les.sync()
+                t_les += time.perf_counter() - t0
+                d1 = les.get_data1()
+                acc_l.append(profiles(d1))
+                probe.set_state(data1=d1)
+                try:
+                    _, cfl = probe.nonlinear(1, 1, dt=dt_l)
+                except cf.ChanError:
+                    cfl = float("inf")
+                cfl_max = max(cfl_max, cfl) if np.isfinite(cfl) else float("inf")
+        finally:
+            lens.close()
         mean_l, rms_l, uv_l = [np.mean([a[j] for a in acc_l], axis=0) for j in range(3)]
         k = slice(1, LES["ny"])
         ent = {
+            "dt": dt_l,
+            "cfl_max_sampled": cfl_max,
             "rel_err_mean_profile": rel(mean_l[k], mean_d[k]),
             "rel_err_rms_u": rel(rms_l[0, k], rms_d[0, k]),
             "rel_err_rms_v": rel(rms_l[1, k], rms_d[1, k]),
             "rel_err_rms_w": rel(rms_l[2, k], rms_d[2, k]),
             "rel_err_uv": rel(uv_l[k], uv_d[k]),
             "les_seconds": t_les,
-            "les_time_units_per_s": (nsamp * args.sample - LES["dt"]) / t_les,
+            "les_time_units_per_s": (nsamp * args.sample - dt_l) / t_les,
         }
         ent["speedup_vs_dns"] = ent["les_time_units_per_s"] / post["dns_time_units_per_s"]
         if mode == "learned":
             g, n = les.les_gated_points()
             ent["fallback_fraction_last"] = g / n
-        post[mode] = ent
-        log(f"a posteriori {mode}: {json.dumps(ent)}")
+        return ent
+
+    for di, dt_l in enumerate(dts):
+        sweep = {}
+        for mode in ("off", "dynamic", "learned"):
+            try:
+                ent = run_les(mode, dt_l)
+            except cf.ChanError as e:  # e.g. the reference's CFL abort at too large a time step
+                ent = {"dt": dt_l, "failed": str(e)}
+            sweep[mode] = ent
+            if di == 0:
+                post[mode] = ent
+            log(f"a posteriori dt={dt_l} {mode}: {json.dumps(ent)}")
+        post["dt_sweep"][str(dt_l)] = sweep
+    probe.close()
     res["aposteriori"] = post
     dens.close()
     with open(args.out, "w") as f:

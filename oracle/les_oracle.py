@@ -213,34 +213,30 @@ def les_ydist(g):
     return yd
 
 
-def test_filter_mask(g):
-    """Sharp test filter at half the resolved wavenumbers: kx index < nx/4,
-    |kz index| < nz/4.  Returns a boolean [nxh, nz]."""
-    ix = np.arange(g.nxh)[:, None]
-    iz = np.arange(g.nz)[None, :]
-    kzi = np.where(iz <= g.nz // 2, iz, iz - g.nz)
-    return (ix < g.nx // 4) & (np.abs(kzi) < g.nz // 4)
+def test_filter(f):
+    """Test filter of the dynamic procedure: the trapezoidal rule (1/4, 1/2,
+    1/4) along x, then along z, on the periodic dealiased grid (last two axes
+    z, x), in that order of operations."""
+    fx = 0.25 * np.roll(f, 1, axis=-1) + 0.5 * f + 0.25 * np.roll(f, -1, axis=-1)
+    return 0.25 * np.roll(fx, 1, axis=-2) + 0.5 * fx + 0.25 * np.roll(fx, -1, axis=-2)
 
 
 def dynamic_coefficient(g, U):
     """Dynamic Smagorinsky C(y): Germano identity L = C M with
-    L_ij = T(u_i u_j) - T(u_i) T(u_j), M_ij = 2 D^2 (T(|S| S_ij) - 4 |S^| S^_ij),
-    plane least squares <L:M>/<M:M> (Lilly 1992), clipped at zero."""
-    co = _co()
+    L_ij = T(u_i u_j) - T(u_i) T(u_j), M_ij = 2 D^2 (T(|S| S_ij) - 4 |S^| S^_ij)
+    (test-to-grid filter width ratio 2), T the trapezoidal test filter in x
+    and z of the closure fields on the dealiased grid, plane least squares
+    <L:M>/<M:M> (Lilly 1992), clipped at zero."""
     ny = g.ny
-    Gs = les_spectral_fields(g, U)
-    Ph = co.xz_c2r(g, Gs)
+    Ph = les_fields(g, U)
     s, mag = strain(Ph[:9])
     u = Ph[9:12]
     P = np.zeros((12,) + Ph.shape[1:])
     for t, (i, j) in enumerate(SYM):
         P[t] = u[i] * u[j]
         P[6 + t] = mag * s[t]
-    P[:, 0] = 0.0
-    P[:, ny] = 0.0
-    mask = test_filter_mask(g)
-    TP = co.xz_c2r(g, co.xz_r2c(g, P) * mask)
-    TG = co.xz_c2r(g, Gs * mask)
+    TP = test_filter(P)
+    TG = test_filter(Ph)
     st, magt = strain(TG[:9])
     ut = TG[9:12]
     d2 = les_delta2(g)

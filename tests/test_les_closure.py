@@ -144,3 +144,20 @@ def test_tau_divergence_second_order(comp):
         errs.append(np.abs(ph[ks, 0, :] - ex).max())
     assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0, errs
     assert errs[2] < 2e-3, errs
+
+
+def test_test_filter_transfer_function(g0):
+    """The trapezoidal test filter multiplies a Fourier mode exp(i(a x + b z))
+    of the dealiased grid by (1 + cos a h_x)/2 (1 + cos b h_z)/2 and keeps a
+    constant; a Nyquist mode in x is removed."""
+    x = np.arange(g0.nx2) * g0.lx / g0.nx2
+    z = np.arange(g0.nz2) * g0.lz / g0.nz2
+    hx, hz = g0.lx / g0.nx2, g0.lz / g0.nz2
+    for kx, kz in ((1, 0), (0, 2), (3, 5), (7, 1)):
+        a, b = 2 * math.pi * kx / g0.lx, 2 * math.pi * kz / g0.lz
+        f = np.cos(a * x[None, :] + b * z[:, None])[None]
+        ref = 0.25 * (1 + math.cos(a * hx)) * (1 + math.cos(b * hz)) * f
+        np.testing.assert_allclose(L.test_filter(f), ref, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(L.test_filter(np.full((1, g0.nz2, g0.nx2), 2.5)), 2.5, rtol=0, atol=1e-15)
+    nyq = np.cos(math.pi * np.arange(g0.nx2))[None, None, :] * np.ones((1, g0.nz2, 1))
+    assert np.abs(L.test_filter(nyq)).max() < 1e-15
