@@ -63,7 +63,7 @@ def load_pkg():
 
 PI = math.atan2(0.0, -1.0)
 DNS = dict(nx=192, ny=129, nz=192, nx2=288, nz2=288, dt=0.005)
-LES = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.01)
+LES = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.03)
 BOX = dict(lx=4.0 * PI, lz=4.0 * PI / 3.0, re=3300.0, stretch=2.75)
 SYM = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
 
