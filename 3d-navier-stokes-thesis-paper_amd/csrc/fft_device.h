@@ -372,10 +372,23 @@ __device__ __forceinline__ void load_roots(double2* dst, const double2* __restri
 template <int NC>
 __device__ __forceinline__ void load_tw2(double2* dst, const double2* __restrict__ roots, int tid, int nthr) {
   if constexpr (NC > 0) {
-    constexpr int N2 = fft2_split(NC).n2;
-    for (int e = tid; e < NC; e += nthr) {
-      const int k1 = e / N2, b = e - k1 * N2;
-      dst[e] = roots[b * k1];
+    // every load first (nthr >= 64), then the LDS writes: one memory round trip
+    constexpr int N2 = fft2_split(NC).n2, IT = (NC + 63) / 64;
+    double2 v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * nthr, ec = e < NC ? e : 0;
+      const int k1 = ec / N2, b = ec - k1 * N2;
+      v[i] = roots[b * k1];
+    }
+    // the values are inputs of an (empty) asm statement here, so the loads
+    // cannot be sunk into the conditional writes below
+#pragma unroll
+    for (int i = 0; i < IT; ++i) asm volatile("" ::"v"(v[i].x), "v"(v[i].y));
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * nthr;
+      if (e < NC) dst[e] = v[i];
     }
   }
 }

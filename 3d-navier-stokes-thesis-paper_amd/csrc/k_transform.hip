@@ -24,6 +24,35 @@ __device__ __forceinline__ long map_plane(const PlaneMap& pm) {
   return (long)(blockIdx.y / pm.cnt) * pm.stride + pm.off + blockIdx.y % pm.cnt;
 }
 
+// Source of padded row element t (0 <= t < nz2) of a spectral column of nz
+// stored kz modes, for a load with a valid address: index, scale and
+// conjugation (the Nyquist split: both halves get half of the Nyquist mode;
+// zero between the stored halves), and the x-Nyquist halving.
+struct PadSrc { int idx; double scale; bool conj; };
+__device__ __forceinline__ PadSrc zpad_src(int t, int nz, int nz2, bool pad, bool halve) {
+  PadSrc r{0, 1.0, false};
+  if (!pad) {
+    r.idx = t < nz ? t : nz - 1;
+    r.scale = t < nz ? 1.0 : 0.0;
+  } else {
+    const int h = nz / 2;
+    if (t < h) r.idx = t;
+    else if (t == h) { r.idx = h; r.scale = 0.5; }
+    else if (t == nz2 - h) { r.idx = h; r.scale = 0.5; r.conj = true; }
+    else if (t > nz2 - h) r.idx = t - (nz2 - nz);
+    else r.scale = 0.0;
+    if (halve) r.scale *= 0.5;
+  }
+  return r;
+}
+// the loaded value with its scale and conjugation; zero where the element is
+// padding or outside the workgroup's columns
+__device__ __forceinline__ double2 pad_apply(double2 v, const PadSrc& p, bool valid) {
+  if (p.conj) v = cconj(v);
+  v = cscale(p.scale, v);
+  return (valid && p.scale != 0.0) ? v : make_double2(0.0, 0.0);
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
                                               const double2* __restrict__ in, long in_ps,
@@ -34,27 +63,50 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
   const long P = map_plane(pm);
   const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   double2* rl = lds + ZT_KX * ss;
-  load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
+  if constexpr (NC == 0) load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
   const int ncol = min(ZT_KX, g.nxh - kx0);
   const double2* src = in + P * in_ps;
   const bool pad = (nz2 > nz) && !g.small;
   const int h = nz / 2;
-  for (int e = tid; e < ZT_KX * nz2; e += nthr) {
-    int s = e / nz2, t = e - s * nz2;
-    double2 v = make_double2(0.0, 0.0);
-    if (s < ncol) {
-      const double2* col = src + (long)(kx0 + s) * nz;
-      if (pad) {
-        if (t < h) v = col[t];
-        else if (t == h) v = cscale(0.5, col[h]);
-        else if (t == nz2 - h) v = cscale(0.5, cconj(col[h]));
-        else if (t > nz2 - h) v = col[t - (nz2 - nz)];
-        if (g.nx2 > g.nx && kx0 + s == g.nxh - 1) v = cscale(0.5, v);
-      } else {
-        v = col[t];
-      }
+  if constexpr (NC > 0) {
+    // all loads of the thread first (one memory round trip), then the LDS writes
+    constexpr int IT = (ZT_KX * NC + 255) / 256;
+    double2 v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * nthr;
+      const int s = e / NC, t = e - s * NC;
+      const PadSrc ps = zpad_src(t, nz, NC, pad, g.nx2 > g.nx && kx0 + s == g.nxh - 1);
+      const int sc = s < ncol ? s : ncol - 1;
+      v[i] = src[(long)(kx0 + sc) * nz + ps.idx];
+      v[i] = pad_apply(v[i], ps, e < ZT_KX * NC && s < ncol);
     }
-    lds[s * ss + fft_in_pos<NC>(t)] = v;
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the first write
+    load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);  // the twiddles' loads behind the data's
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * nthr;
+      const int s = e / NC, t = e - s * NC;
+      if (e < ZT_KX * NC) lds[s * ss + fft_in_pos<NC>(t)] = v[i];
+    }
+  } else {
+    for (int e = tid; e < ZT_KX * nz2; e += nthr) {
+      int s = e / nz2, t = e - s * nz2;
+      double2 v = make_double2(0.0, 0.0);
+      if (s < ncol) {
+        const double2* col = src + (long)(kx0 + s) * nz;
+        if (pad) {
+          if (t < h) v = col[t];
+          else if (t == h) v = cscale(0.5, col[h]);
+          else if (t == nz2 - h) v = cscale(0.5, cconj(col[h]));
+          else if (t > nz2 - h) v = col[t - (nz2 - nz)];
+          if (g.nx2 > g.nx && kx0 + s == g.nxh - 1) v = cscale(0.5, v);
+        } else {
+          v = col[t];
+        }
+      }
+      lds[s * ss + fft_in_pos<NC>(t)] = v;
+    }
   }
   __syncthreads();
   fft_run<NC, 1, FFT_MAXV>(lds, ss, ZT_KX, g.pz, tw, rl, tid, nthr);
@@ -263,28 +315,28 @@ __global__ __launch_bounds__(32 * KX) void k_zinv2(XZGeom g, const double2* __re
   const long P = map_plane(pm);
   const int nz = g.nz;
   double2* rl = lds + KX * SS;
-  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);
   const int ncol = min(KX, g.nxh - kx0);
   const double2* src = in + P * in_ps;
   const bool pad = (NZ2 > nz) && !g.small;
   const bool halve = g.nx2 > g.nx;
-  const int h = nz / 2;
-  for (int e = tid; e < KX * NZ2; e += NT) {
+  constexpr int IT = (KX * NZ2 + NT - 1) / NT;
+  double2 v[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NT;
     const int s = e / NZ2, t = e - s * NZ2;
-    double2 v = make_double2(0.0, 0.0);
-    if (s < ncol) {
-      const double2* col = src + (long)(kx0 + s) * nz;
-      if (pad) {
-        if (t < h) v = col[t];
-        else if (t == h) v = cscale(0.5, col[h]);
-        else if (t == NZ2 - h) v = cscale(0.5, cconj(col[h]));
-        else if (t > NZ2 - h) v = col[t - (NZ2 - nz)];
-        if (halve && kx0 + s == g.nxh - 1) v = cscale(0.5, v);
-      } else {
-        v = col[t];
-      }
-    }
-    lds[s * SS + fft2_in_pos<NC>(t)] = v;
+    const PadSrc ps = zpad_src(t, nz, NZ2, pad, halve && kx0 + s == g.nxh - 1);
+    const int sc = s < ncol ? s : ncol - 1;
+    v[i] = src[(long)(kx0 + sc) * nz + ps.idx];
+    v[i] = pad_apply(v[i], ps, e < KX * NZ2 && s < ncol);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the first write
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);  // the twiddles' loads behind the data's
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NT;
+    const int s = e / NZ2, t = e - s * NZ2;
+    if (e < KX * NZ2) lds[s * SS + fft2_in_pos<NC>(t)] = v[i];
   }
   __syncthreads();
   fft2p<NC, 1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
@@ -326,21 +378,27 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
     }
   }
   double2* rl = lds + KX * SS;
-  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);
   const double2* Ap = A + P * ab_ps;
   const double2* Bp = B + P * ab_ps;
-  double2 rb[NLD];
+  // all loads of the thread first (one memory round trip), then the LDS writes
+  double2 ra[NLD], rb[NLD];
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int e = tid + NT * k;
     const int t = e / KX, s = e - t * KX;
-    double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-    if (e < NZ2 * KX && s < ncol) {
-      va = Ap[(long)t * g.kxpad + kx0 + s];
-      vb = Bp[(long)t * g.kxpad + kx0 + s];
-    }
-    if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = va;
-    rb[k] = vb;
+    const bool ok = e < NZ2 * KX && s < ncol;
+    const long o = (long)(t < NZ2 ? t : NZ2 - 1) * g.kxpad + kx0 + (s < ncol ? s : ncol - 1);
+    const double2 a = Ap[o], b = Bp[o];
+    ra[k] = ok ? a : make_double2(0.0, 0.0);
+    rb[k] = ok ? b : make_double2(0.0, 0.0);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the first write
+  load_tw2<NC>(rl, tw + g.pz.tw, tid, NT);  // the twiddles' loads behind the data's
+#pragma unroll
+  for (int k = 0; k < NLD; ++k) {
+    const int e = tid + NT * k;
+    const int t = e / KX, s = e - t * KX;
+    if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = ra[k];
   }
   __syncthreads();
   fft2p<NC, -1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
