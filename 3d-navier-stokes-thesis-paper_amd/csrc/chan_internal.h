@@ -203,7 +203,7 @@ struct SolveArgs {
   int seg;            // segmented column solve where the row count allows (Tuning::solve_seg)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
-int solve_seg_rows(int ny);  // modes per wavefront of the segmented solve, 0 if ny+1 rows do not fit
+int solve_seg_rows(int ny);  // rows per segment of the segmented solve, 0 if ny+1 rows do not fit
 void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
                    long nm, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,

@@ -107,7 +107,7 @@ __device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ 
   return to_bf16x4(gelu_bf(acc[0] + bb[0]), gelu_bf(acc[1] + bb[1]), gelu_bf(acc[2] + bb[2]), gelu_bf(acc[3] + bb[3]));
 }
 
-#define MLP_NT 2  // point tiles of 16 per wavefront
+#define MLP_NT 3  // point tiles of 16 per wavefront (each weight operand read from LDS serves all of them)
 
 __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ packed, const float* __restrict__ p, int n,
                                                    const float* __restrict__ x, float* __restrict__ y) {

@@ -641,6 +641,17 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
       if (((nbf + nthr - 1) / nthr) * R > 12) throw chan_error("x-pass FFT staging bound");
     }
   int nplanes = a.kb - a.ka + 1;
+  if (a.wave && ct && M <= 192) {
+    // small grids: fewer planes per workgroup until there are two workgroups per CU
+    static int cus = 0;
+    if (!cus) {
+      int dev = 0, v = 0;
+      cus = (hipGetDevice(&dev) == hipSuccess &&
+             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+    }
+    const long zw = (g.nz2 + 3) / 4;
+    while (a.chunk > 2 && zw * ((nplanes + a.chunk - 1) / a.chunk) < 2L * cus) --a.chunk;
+  }
   int nch = (nplanes + a.chunk - 1) / a.chunk;
   if (a.tau && !(a.wave && ct && M <= 192))
     throw chan_error("the subgrid closure needs the wavefront x-pass (nx2/2 <= 192 of the form 2^a 3^b)");

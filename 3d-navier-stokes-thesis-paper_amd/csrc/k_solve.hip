@@ -940,13 +940,16 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   }
 }
 
-// Modes per wavefront of the segmented solve for ny+1 rows (segments of 9
-// rows; the last active one needs 3..9 rows), 0 if the column does not fit.
+// Rows per segment of the segmented solve for ny+1 rows: the shortest of
+// 5..9 with at most 16 segments and 3 or more rows in the last one (the wall
+// rows 0..2 and ny-2..ny each lie in one segment); 0 if none fits.
 int solve_seg_rows(int ny) {
-  const int nyp = ny + 1, ns = (nyp + 8) / 9, rl = nyp - (ns - 1) * 9;
-  if (ny < 8 || rl < 3) return 0;
-  if (ns <= 8) return 8;
-  if (ns <= 16) return 4;
+  const int nyp = ny + 1;
+  if (ny < 8) return 0;
+  for (int seg = 5; seg <= 9; ++seg) {
+    const int ns = (nyp + seg - 1) / seg, rl = nyp - (ns - 1) * seg;
+    if (ns <= 16 && rl >= 3) return seg;
+  }
   return 0;
 }
 
@@ -1419,11 +1422,16 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   // the segmented and component-parallel kernels need the byte offsets of one
   // array to fit 32 bits
   const bool off32 = 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9;
-  const int seg_modes = a.seg ? solve_seg_rows(a.ny) : 0;
-  if (seg_modes && off32) {
-    dim3 gs((unsigned)((nm + seg_modes - 1) / seg_modes));
-    if (seg_modes == 8) hipLaunchKernelGGL((k_solve_seg<8, 9>), gs, dim3(64), 0, s, ops, a);
-    else hipLaunchKernelGGL((k_solve_seg<4, 9>), gs, dim3(64), 0, s, ops, a);
+  const int seg = a.seg ? solve_seg_rows(a.ny) : 0;
+  if (seg && off32) {
+    dim3 gs((unsigned)((nm + 3) / 4));
+    switch (seg) {
+      case 5: hipLaunchKernelGGL((k_solve_seg<4, 5>), gs, dim3(64), 0, s, ops, a); break;
+      case 6: hipLaunchKernelGGL((k_solve_seg<4, 6>), gs, dim3(64), 0, s, ops, a); break;
+      case 7: hipLaunchKernelGGL((k_solve_seg<4, 7>), gs, dim3(64), 0, s, ops, a); break;
+      case 8: hipLaunchKernelGGL((k_solve_seg<4, 8>), gs, dim3(64), 0, s, ops, a); break;
+      default: hipLaunchKernelGGL((k_solve_seg<4, 9>), gs, dim3(64), 0, s, ops, a); break;
+    }
     LAUNCH_CHECK("k_solve_seg");
     return;
   }

@@ -181,6 +181,8 @@ def main():
                     help="RK3 steps of the learned-closure LES timed for the LES speed-up (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprof), print no JSON")
+    ap.add_argument("--les-profile-steps", type=int, default=0,
+                    help="only run this many steps of the learned-closure LES (for rocprof), print no JSON")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (a plain copy kernel "
                          "of known size, used to calibrate rocprof's FETCH_SIZE/WRITE_SIZE)")
@@ -203,6 +205,10 @@ def main():
     cf = load_pkg()
     import chanflow.synth as synth
     c = HEADLINE
+    if args.les_profile_steps:
+        sps, fb = les_rate(cf, synth, c, args.les_profile_steps)
+        print(f"LES {args.les_profile_steps} steps, {sps:.1f} steps/s, fallback fraction {fb:.3f}", flush=True)
+        return
     ch, y, yh, lx, lz = make_channel(cf, c)
     U0 = synth.turbulent_like(c["nx"], c["ny"], c["nz"], yh, y, seed=c["seed"], rms=c["rms"], re_tau=c["re_tau"])
     ch.set_state(data1=U0)
