@@ -200,9 +200,10 @@ def test_rhs_solve_variants(g0, variant):
 def test_rhs_solve_segmented(ny):
     """The segmented column solve (partitioned tridiagonal solves, columns in
     registers) against the oracle for the three substeps, with a perturbed RK
-    register and wall data on both walls, at row counts that use 8 modes per
-    wavefront (ny = 33 with idle segments, 65) and 4 (97, 129; 11 and 15
-    segments); and against the sweep-based 3-lane solve to 1e-12."""
+    register and wall data on both walls, at row counts that pick segments of
+    5 rows (ny = 33: 7 segments, idle lanes past them), 6 (65: 11), 7 (97: 14)
+    and 9 (129: 15), 4 modes per wavefront; and against the sweep-based
+    3-lane solve to 1e-12."""
     g = O.make_grid(16, ny, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1, dt_flag=0)
     rng = np.random.default_rng(ny)
     s = __import__("helpers").synth()
