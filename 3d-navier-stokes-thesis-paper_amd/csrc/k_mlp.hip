@@ -3,7 +3,8 @@
 //
 // Inference (k_mlp_infer) runs on the matrix cores in bfloat16 with float32
 // accumulation: every layer is H_out^T = W H_in^T + b with the 16 points of a
-// tile as the N dimension of v_mfma_f32_16x16x16_bf16.  In that orientation
+// tile as the N dimension of v_mfma_f32_16x16x16_bf16 (layer 1, K = 16) and
+// v_mfma_f32_16x16x32_bf16 (layers 2-4, K = 32).  In that orientation
 // the 16x16 float32 result tile (lane l: rows 4(l/16)..+3, column l%16) is
 // already the B operand of the next layer for the matching 16-feature
 // K-block (lane l: k = 4(l/16)..+3, column l%16), so activations never leave
@@ -17,13 +18,14 @@
 #include <cmath>
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  unsigned int u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (unsigned short)(u >> 16);
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+// two 4-value operands side by side as one 8-value operand
+__device__ __forceinline__ bf16x8 cat8(bf16x4 a, bf16x4 b) {
+  const s16x8 r = __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, r);
 }
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float gelu_f(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad_f(float x) {
@@ -50,21 +52,48 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
 #define MLP_B4 35200
 #define MLP_NPARAM 35206
 
+// Four float32 values rounded to bf16 (nearest even) by the hardware
+// conversion, as one MFMA operand.
+typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4 to_bf16x4(float a, float b, float c, float d) {
+  const bf16x2h lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
+  const u32x2 w = {__builtin_bit_cast(unsigned int, lo), __builtin_bit_cast(unsigned int, hi)};
+  return __builtin_bit_cast(bf16x4, w);
+}
+
+// Layer 1 (K = 16, v_mfma_f32_16x16x16_bf16): block ot holds lane l's A
+// operand (row l%16 of output tile ot, k = 4(l/16)..+3).  Layers 2-4
+// (K = 32, v_mfma_f32_16x16x32_bf16): the two 512-byte blocks of K-block kb of
+// output tile ot hold lane l's 8 values (row l%16, logical k = 8(l/16)..+7) at
+// entries 2l, 2l+1.  Logical k = 8q + j stands for input feature
+// 32 kb + 16 (j/4) + 4q + j%4, so that a lane's B operand is its two result
+// tiles of the previous layer side by side (features 4q..4q+3 of tiles 2kb
+// and 2kb+1).
 __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ packed) {
   const int blk = blockIdx.x, l = threadIdx.x;  // 64 threads
-  int wofs, nin, nout, ot, kb;
-  if (blk < MLP_BLK2) { wofs = MLP_W1; nin = 10; nout = 128; ot = blk; kb = 0; }
-  else if (blk < MLP_BLK3) { wofs = MLP_W2; nin = 128; nout = 128; ot = (blk - MLP_BLK2) / 8; kb = (blk - MLP_BLK2) % 8; }
-  else if (blk < MLP_BLK4) { wofs = MLP_W3; nin = 128; nout = 128; ot = (blk - MLP_BLK3) / 8; kb = (blk - MLP_BLK3) % 8; }
-  else { wofs = MLP_W4; nin = 128; nout = 6; ot = 0; kb = blk - MLP_BLK4; }
-  const int row = ot * 16 + (l & 15);
-  bf16x4 v;
-  for (int t = 0; t < 4; ++t) {
-    const int k = kb * 16 + 4 * (l >> 4) + t;
-    const float w = (row < nout && k < nin) ? p[wofs + row * nin + k] : 0.0f;
-    v[t] = (short)f2bf(w);
+  int wofs, nin, nout, row, f0;
+  if (blk < MLP_BLK2) {
+    wofs = MLP_W1; nin = 10; nout = 128;
+    row = blk * 16 + (l & 15);
+    f0 = 4 * (l >> 4);
+  } else {
+    const int b0 = blk < MLP_BLK3 ? MLP_BLK2 : (blk < MLP_BLK4 ? MLP_BLK3 : MLP_BLK4);
+    wofs = blk < MLP_BLK3 ? MLP_W2 : (blk < MLP_BLK4 ? MLP_W3 : MLP_W4);
+    nin = 128;
+    nout = blk < MLP_BLK4 ? 128 : 6;
+    const int pi = (blk - b0) / 2, hb = (blk - b0) % 2;
+    const int ot = pi / 4, kb = pi % 4;
+    const int L = 32 * hb + l / 2, part = l % 2;   // lane of the operand, half of its 8 values
+    row = ot * 16 + (L & 15);
+    f0 = 32 * kb + 16 * part + 4 * (L >> 4);
   }
-  packed[blk * 64 + l] = v;
+  float w[4];
+  for (int t = 0; t < 4; ++t) {
+    const int k = f0 + t;
+    w[t] = (row < nout && k < nin) ? p[wofs + row * nin + k] : 0.0f;
+  }
+  packed[blk * 64 + l] = to_bf16x4(w[0], w[1], w[2], w[3]);
 }
 
 // GELU for the bf16 inference path, without transcendental instructions:
@@ -91,15 +120,6 @@ __device__ __forceinline__ float gelu_bf(float x) {
   return (fabsf(x) > 4.5f) ? fmaxf(x, 0.0f) : g;
 }
 
-// Four float32 values rounded to bf16 (nearest even) by the hardware
-// conversion, as one MFMA operand.
-typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bf16x4 to_bf16x4(float a, float b, float c, float d) {
-  const bf16x2h lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
-  const u32x2 w = {__builtin_bit_cast(unsigned int, lo), __builtin_bit_cast(unsigned int, hi)};
-  return __builtin_bit_cast(bf16x4, w);
-}
 
 // Bias + GELU on a result tile, rounded to bf16 as the next B operand.
 __device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ bias, int ot, int lane) {
@@ -111,7 +131,8 @@ __device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ 
 
 __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ packed, const float* __restrict__ p, int n,
                                                    const float* __restrict__ x, float* __restrict__ y) {
-  __shared__ bf16x4 W[MLP_NBLK * 64];
+  __shared__ __align__(16) bf16x4 W[MLP_NBLK * 64];
+  const bf16x8* W8 = reinterpret_cast<const bf16x8*>(W);
   __shared__ float bias[128 * 3 + 16];
   for (int i = threadIdx.x; i < MLP_NBLK * 64; i += blockDim.x) W[i] = packed[i];
   for (int i = threadIdx.x; i < 128; i += blockDim.x) {
@@ -163,11 +184,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
 #pragma unroll
         for (int nt = 0; nt < MLP_NT; ++nt) acc[nt][0] = f32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb) {
-          const bf16x4 a = W[(blk0 + ot * 8 + kb) * 64 + lane];
+        for (int kb = 0; kb < 4; ++kb) {
+          const bf16x8 a = W8[(blk0 + (ot * 4 + kb) * 2) * 32 + lane];
 #pragma unroll
           for (int nt = 0; nt < MLP_NT; ++nt)
-            acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[nt][kb], acc[nt][0], 0, 0, 0);
+            acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cat8(h[nt][2 * kb], h[nt][2 * kb + 1]),
+                                                                 acc[nt][0], 0, 0, 0);
         }
 #pragma unroll
         for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[nt][0], bl, ot, lane);
@@ -183,8 +205,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
     for (int nt = 0; nt < MLP_NT; ++nt) {
       f32x4 o = f32x4{0, 0, 0, 0};
 #pragma unroll
-      for (int kb = 0; kb < 8; ++kb)
-        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[(MLP_BLK4 + kb) * 64 + lane], h[nt][kb], o, 0, 0, 0);
+      for (int kb = 0; kb < 4; ++kb)
+        o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W8[(MLP_BLK4 + kb * 2) * 32 + lane],
+                                                    cat8(h[nt][2 * kb], h[nt][2 * kb + 1]), o, 0, 0, 0);
       const long pt = base + 16 * nt + col;
       if (pt < n)
         for (int i = 0; i < 4; ++i) {
