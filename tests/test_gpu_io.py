@@ -1,19 +1,16 @@
 """Device checkpoints through the C-ABI: SAVE_VELOCITY of the device state
 and read_velocity into a fresh context (PFC/read_write.f90:21-292)."""
-import os
 
 import numpy as np
 import pytest
 
-from helpers import ROOT, grid_cfg0, init_cfg0, make_channel
+from helpers import grid_cfg0, init_cfg0, make_channel
 from oracle import chan_oracle as O
 
 pytestmark = pytest.mark.gpu
-WORK = os.path.join(ROOT, "gpurun_out", "test_gpu_io")
 
 
-def test_save_and_read_velocity():
-    os.makedirs(WORK, exist_ok=True)
+def test_save_and_read_velocity(tmp_path):
     g = grid_cfg0(O)
     a = make_channel(g)
     U0 = init_cfg0(O, g, seed=2, amp=1e-2)
@@ -21,7 +18,7 @@ def test_save_and_read_velocity():
     a.set_scalars(dt=0.01, t_total=0.0, p_grad=-2.0 / g.re)
     a.initialize_flow()
     a.step(3, step0=1)
-    path = os.path.join(WORK, "newvel.out")
+    path = str(tmp_path / "newvel.out")
     a.save_velocity(path)
     b = make_channel(g)
     b.read_velocity(path)
@@ -36,14 +33,13 @@ def test_save_and_read_velocity():
         ch.close()
 
 
-def test_write_vtk_from_device():
+def test_write_vtk_from_device(tmp_path):
     import struct
-    os.makedirs(WORK, exist_ok=True)
     g = O.make_grid(16, 17, 12, 24, 18, 2.0, 2.0, 1000.0, stretch=1.5)
     ch = make_channel(g)
     U = O.remove_initial_divergence(g, init_cfg0(O, g, seed=4, amp=1e-1))
     ch.set_state(data1=U)
-    path = os.path.join(WORK, "t_000.vtk")
+    path = str(tmp_path / "t_000.vtk")
     ch.write_vtk(path, g.yh)
     raw = open(path, "rb").read()
     i = raw.index(b"VECTORS flowvec float\n") + len(b"VECTORS flowvec float\n")
