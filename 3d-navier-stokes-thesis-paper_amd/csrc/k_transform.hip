@@ -53,6 +53,19 @@ __device__ __forceinline__ double2 pad_apply(double2 v, const PadSrc& p, bool va
   return (valid && p.scale != 0.0) ? v : make_double2(0.0, 0.0);
 }
 
+// Row t and column s of element e of an 8-column tile, with each 8-lane group
+// on 4 columns x 2 rows (e in whole groups of 16 covers 2 rows x 8 columns).
+// The group's 16-byte LDS writes to s*ss + pos(t) then fall in distinct banks
+// for ss = 2 (mod 8) (ss = 306 at nz2 = 288), where the 8 columns of one row
+// meet in pairs (measured: LDS bank-conflict cycles of the z-forward 3.6e7 ->
+// 1.6e7 per step); every 16 lanes still read two whole 128-byte rows of the
+// [z][kx] array.  Needs an even row count.
+__device__ __forceinline__ void tile8_ts(int e, int& t, int& s) {
+  const int r = e & 15;
+  t = 2 * (e >> 4) + ((r >> 2) & 1);
+  s = (r & 3) | ((r >> 3) << 2);
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
                                               const double2* __restrict__ in, long in_ps,
@@ -385,7 +398,8 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int e = tid + NT * k;
-    const int t = e / KX, s = e - t * KX;
+    int t = e / KX, s = e - t * KX;
+    if constexpr (KX == 8) tile8_ts(e, t, s);
     const bool ok = e < NZ2 * KX && s < ncol;
     const long o = (long)(t < NZ2 ? t : NZ2 - 1) * g.kxpad + kx0 + (s < ncol ? s : ncol - 1);
     const double2 a = Ap[o], b = Bp[o];
@@ -397,7 +411,8 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int e = tid + NT * k;
-    const int t = e / KX, s = e - t * KX;
+    int t = e / KX, s = e - t * KX;
+    if constexpr (KX == 8) tile8_ts(e, t, s);
     if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = ra[k];
   }
   __syncthreads();
@@ -425,7 +440,8 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int e = tid + NT * k;
-    const int t = e / KX, s = e - t * KX;
+    int t = e / KX, s = e - t * KX;
+    if constexpr (KX == 8) tile8_ts(e, t, s);
     if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = rb[k];
   }
   __syncthreads();
