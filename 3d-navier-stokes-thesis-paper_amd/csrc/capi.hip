@@ -406,7 +406,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 6: if (v != 4 && v != 8) return false; t.zfwd_kx = v; return true;
     case 7: if (v != 0 && v != 1) return false; t.xpass_prefetch = v; return true;
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
-    case 9: if (v < 2 || v > 64) return false; t.xpass_chunk = v; return true;
+    case 9: if (v != 0 && (v < 2 || v > 64)) return false; t.xpass_chunk = v; return true;
     case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
     case 11: if (v != 0 && v != 1) return false; t.solve_seg = v; return true;
     default: return false;

@@ -101,7 +101,7 @@ struct Tuning {
   int zfwd_kx = 8;        // z-forward kx columns per workgroup (4, 8)
   int xpass_prefetch = 1; // wave x-pass streams the next plane's rows into LDS
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
-  int xpass_chunk = 6;    // planes per x-pass workgroup (2..64)
+  int xpass_chunk = 0;    // planes per x-pass workgroup (2..64; 0: from the resident workgroups)
   int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
   int solve_seg = 1;      // segmented column solve (columns in registers, no scratch)
 };

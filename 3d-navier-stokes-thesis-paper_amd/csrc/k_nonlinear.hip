@@ -619,6 +619,18 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   }
 }
 
+using XPassFn = void (*)(XZGeom, const double2*, DevOps, XPassArgs);
+
+// Planes per workgroup of the wavefront x-pass when no chunk is set: the
+// plane-rows of the pass spread evenly over the `res` workgroups resident at
+// a time, so that one wave of workgroups with full ranges covers nearly all
+// of them and the short last ranges fill the remaining slots; at least two
+// planes per range.
+static int xpass_chunk_auto(long zw, int np, long res) {
+  const long c = (zw * np + res - 1) / res;
+  return (int)std::max(2L, std::min<long>(c, std::max(2, np)));
+}
+
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a0,
                   hipStream_t s) {
   XPassArgs a = a0;
@@ -640,49 +652,48 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
       long nbf = (long)8 * rz * (M / R);
       if (((nbf + nthr - 1) / nthr) * R > 12) throw chan_error("x-pass FFT staging bound");
     }
-  int nplanes = a.kb - a.ka + 1;
-  if (a.wave && ct && M <= 192) {
-    // small grids: fewer planes per workgroup until there are two workgroups per CU
-    static int cus = 0;
-    if (!cus) {
-      int dev = 0, v = 0;
-      cus = (hipGetDevice(&dev) == hipSuccess &&
-             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
-    }
-    const long zw = (g.nz2 + 3) / 4;
-    while (a.chunk > 2 && zw * ((nplanes + a.chunk - 1) / a.chunk) < 2L * cus) --a.chunk;
-  }
-  int nch = (nplanes + a.chunk - 1) / a.chunk;
+  const int nplanes = a.kb - a.ka + 1;
   if (a.tau && !(a.wave && ct && M <= 192))
     throw chan_error("the subgrid closure needs the wavefront x-pass (nx2/2 <= 192 of the form 2^a 3^b)");
   if (a.wave && ct && M <= 192) {
-    dim3 gw((g.nz2 + 3) / 4, nch);
     const bool pf = a.prefetch && M <= 144;
     const int rawn = ((M + 1 + 63) / 64) * 64;
     size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
-#define XPASSW_CASE(N)                                                                                  \
-  if constexpr (N > 0 && N <= 192) {                                                                    \
-    if (a.tau) {                                                                                        \
-      if (a.base) throw chan_error("the subgrid closure is not defined for the linearized pass");       \
-      if constexpr (N <= 144) {                                                                         \
-        if (pf) hipLaunchKernelGGL((k_xpass_w<false, N, true, true>), gw, dim3(256), ldw, s, g, tw, ops, a); \
-        else hipLaunchKernelGGL((k_xpass_w<false, N, false, true>), gw, dim3(256), ldw, s, g, tw, ops, a);   \
-      } else {                                                                                          \
-        hipLaunchKernelGGL((k_xpass_w<false, N, false, true>), gw, dim3(256), ldw, s, g, tw, ops, a);     \
-      }                                                                                                 \
-    } else if (pf) {                                                                                    \
-      if constexpr (N <= 144) {                                                                         \
-        if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a); \
-        else hipLaunchKernelGGL((k_xpass_w<false, N, true>), gw, dim3(256), ldw, s, g, tw, ops, a);       \
-      }                                                                                                 \
-    } else if (a.base) hipLaunchKernelGGL((k_xpass_w<true, N, false>), gw, dim3(256), ldw, s, g, tw, ops, a); \
-    else hipLaunchKernelGGL((k_xpass_w<false, N, false>), gw, dim3(256), ldw, s, g, tw, ops, a);        \
+    XPassFn fn = nullptr;
+#define XPASSW_CASE(N)                                                                  \
+  if constexpr (N > 0 && N <= 192) {                                                    \
+    if (a.tau) {                                                                        \
+      if (a.base) throw chan_error("the subgrid closure is not defined for the linearized pass"); \
+      if constexpr (N <= 144) fn = pf ? k_xpass_w<false, N, true, true> : k_xpass_w<false, N, false, true>; \
+      else fn = k_xpass_w<false, N, false, true>;                                       \
+    } else if (pf) {                                                                    \
+      if constexpr (N <= 144) fn = a.base ? k_xpass_w<true, N, true> : k_xpass_w<false, N, true>; \
+    } else {                                                                            \
+      fn = a.base ? k_xpass_w<true, N, false> : k_xpass_w<false, N, false>;             \
+    }                                                                                   \
   }
     FFT_DISPATCH(M, XPASSW_CASE)
 #undef XPASSW_CASE
+    if (!fn) throw chan_error("x-pass: no wavefront kernel for this length");
+    const long zw = (g.nz2 + 3) / 4;
+    if (a.chunk <= 0) {
+      static int cus = 0;
+      if (!cus) {
+        int dev = 0, v = 0;
+        cus = (hipGetDevice(&dev) == hipSuccess &&
+               hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+      }
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, ldw) != hipSuccess || per_cu < 1) per_cu = 1;
+      a.chunk = xpass_chunk_auto(zw, nplanes, (long)per_cu * cus);
+    }
+    const int nch = (nplanes + a.chunk - 1) / a.chunk;
+    hipLaunchKernelGGL(fn, dim3((unsigned)zw, nch), dim3(256), ldw, s, g, tw, ops, a);
     LAUNCH_CHECK("k_xpass_w");
     return;
   }
+  if (a.chunk <= 0) a.chunk = 6;
+  const int nch = (nplanes + a.chunk - 1) / a.chunk;
   dim3 grid(g.nz2 / rz, nch);
   size_t lds = sizeof(double2) * (8 * rz * fft_seq_cells(M) + (ct ? M : 0));
 #define XPASS_CASE(N)                                                                 \

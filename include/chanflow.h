@@ -219,10 +219,11 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * workgroup (4, 8 or 16), 6 z-forward kx columns per workgroup (4 or 8),
  * 7 x-pass prefetch of the next plane's rows into LDS (1 on, 0 off),
  * 8 solve rows streamed through an LDS ring (1 on, 0 off; 3-lane solve only),
- * 9 planes per x-pass workgroup (2..64), 10 plane ranges the nonlinear pass
- * runs in turn (1..64; 1: every kernel over all planes), 11 segmented column
- * solve (1 on, 0 off) wherever ny+1 rows fit 16 segments of 9 rows (keys 3,
- * 4 and 8 then do not apply).
+ * 9 planes per x-pass workgroup (2..64; 0, the default: the plane-rows of the
+ * pass spread evenly over the workgroups resident at a time), 10 plane
+ * ranges the nonlinear pass runs in turn (1..64; 1: every kernel over all
+ * planes), 11 segmented column solve (1 on, 0 off) wherever ny+1 rows fit 16
+ * segments of 5 to 9 rows (keys 3, 4 and 8 then do not apply).
  * chan_autotune times `steps`
  * timesteps of every candidate, keeps the fastest and restores the state. */
 int chan_set_tuning(chan_ctx* ctx, int key, int value);

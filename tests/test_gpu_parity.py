@@ -125,6 +125,27 @@ def test_nonlinear_plane_ranges(g0, nch):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("chunk", [2, 3, 7, 64])
+def test_nonlinear_xpass_chunks(g0, chunk):
+    """The x-pass with its planes split into ranges of at most `chunk` planes
+    (tuning key xpass_chunk; 0, the default, picks the ranges from the
+    occupancy) computes the same term, bit for bit, CFL included."""
+    U = init_cfg0(O, g0, seed=22, amp=1e-2)
+    out = []
+    for c in (0, chunk):
+        ch = make_channel(g0)
+        ch.set_tuning("xpass_chunk", c)
+        assert ch.get_tuning()["xpass_chunk"] == c
+        ch.set_state(data1=U)
+        dt, cfl = ch.nonlinear(1, 1, dt=0.01)
+        out.append((ch.get_data1(), cfl))
+        ch.close()
+    ref, _, cflr = O.nonlinear(g0, U, step=1, rk_step=1, dt=0.01, return_cfl=True)
+    assert rel_l2(out[1][0], ref) < TOL1
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and abs(out[1][1] - cflr) <= 1e-14 * abs(cflr)
+
+
 def test_nonlinear_dt_flag(g0):
     g = grid_cfg0(O, dt_flag=1)
     ch = make_channel(g)
