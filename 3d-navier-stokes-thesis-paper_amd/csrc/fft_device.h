@@ -310,6 +310,47 @@ __device__ __forceinline__ void fft2p(double2* lds, int sstride, int B, const do
   __syncthreads();
 }
 
+// fft2p without the twiddle table in LDS: a pass-1 task b reads w = rg[b]
+// from the length-N root table (exp(-2 pi i b/N)) and forms the inter-pass
+// twiddles as w^k1 = w^(k1-1) w in registers (w^k1 carries k1 - 1 roundings
+// beyond the table's), which leaves the workgroup's LDS to the data alone.
+template <int N, int SGN>
+__device__ __forceinline__ void fft2_pass1_rec(double2* seq, int b, const double2* __restrict__ rg) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2, RS = N2 + 1;
+  double2* p = seq + b;
+  const double2 wt = rg[b];
+  double2 v[N1];
+#pragma unroll
+  for (int a = 0; a < N1; ++a) v[a] = p[a * RS];
+  rdft<N1, SGN, N2>(v, rg);
+  const double2 w1 = SGN > 0 ? cconj(wt) : wt;
+  double2 w = w1;
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) {
+    if (k1 > 1) w = cmul(w, w1);
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) p[k1 * RS] = v[k1];
+}
+
+template <int N, int SGN>
+__device__ __forceinline__ void fft2p_rec(double2* lds, int sstride, int B, const double2* __restrict__ rg,
+                                          int tid, int nthr) {
+  constexpr int N1 = fft2_split(N).n1, N2 = fft2_split(N).n2;
+  static_assert(N1 * N2 == N && N1 > 0, "unsupported two-pass length");
+  for (int g = tid; g < B * N2; g += nthr) {
+    const int s = g / N2, b = g - s * N2;
+    fft2_pass1_rec<N, SGN>(lds + s * sstride, b, rg);
+  }
+  __syncthreads();
+  for (int g = tid; g < B * N1; g += nthr) {
+    const int s = g / N1, k1 = g - s * N1;
+    fft2_pass2<N, SGN>(lds + s * sstride, k1, rg);
+  }
+  __syncthreads();
+}
+
 // Orders the LDS accesses of one wavefront: everything a lane wrote before
 // is visible to every lane of the same wavefront after it.
 __device__ __forceinline__ void wave_sync() {

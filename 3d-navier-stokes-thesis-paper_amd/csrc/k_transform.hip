@@ -94,8 +94,9 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
       v[i] = src[(long)(kx0 + sc) * nz + ps.idx];
       v[i] = pad_apply(v[i], ps, e < ZT_KX * NC && s < ncol);
     }
+    // the inter-pass twiddles are formed in registers (fft2p_rec): with no
+    // twiddle table, the workgroup's 38 KB of LDS let 4 workgroups share a CU
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the first write
-    load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);  // the twiddles' loads behind the data's
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       const int e = tid + i * nthr;
@@ -122,7 +123,8 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
     }
   }
   __syncthreads();
-  fft_run<NC, 1, FFT_MAXV>(lds, ss, ZT_KX, g.pz, tw, rl, tid, nthr);
+  if constexpr (NC > 0) fft2p_rec<NC, 1>(lds, ss, ZT_KX, tw + g.pz.tw, tid, nthr);
+  else fft_run<NC, 1, FFT_MAXV>(lds, ss, ZT_KX, g.pz, tw, rl, tid, nthr);
   double2* dst = out + P * out_ps;
   for (int e = tid; e < nz2 * ZT_KX; e += nthr) {
     int t = e / ZT_KX, s = e - t * ZT_KX;
@@ -491,7 +493,7 @@ void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_
   dim3 grid((g.nxh + ZT_KX - 1) / ZT_KX, nplanes);
   const int nthr = 256;
   if (!ct) check_stage_fit(g.pz, ZT_KX, nthr);
-  size_t lds = sizeof(double2) * (ZT_KX * fft_seq_cells(g.nz2) + (ct ? g.nz2 : 0));
+  size_t lds = sizeof(double2) * ZT_KX * fft_seq_cells(g.nz2);  // no twiddle table (k_zinv)
 #define ZINV_CASE(N) \
   hipLaunchKernelGGL(k_zinv<N>, grid, dim3(nthr), lds, s, g, tw, in, in_ps, out, out_ps, pm)
   FFT_DISPATCH(g.nz2, ZINV_CASE)
