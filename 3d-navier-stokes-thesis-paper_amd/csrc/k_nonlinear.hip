@@ -677,12 +677,11 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     if (!fn) throw chan_error("x-pass: no wavefront kernel for this length");
     const long zw = (g.nz2 + 3) / 4;
     if (a.chunk <= 0) {
-      static int cus = 0;
-      if (!cus) {
-        int dev = 0, v = 0;
-        cus = (hipGetDevice(&dev) == hipSuccess &&
-               hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
-      }
+      // resident workgroups of this kernel on the current device
+      int dev = 0, cus = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
       int per_cu = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, ldw) != hipSuccess || per_cu < 1) per_cu = 1;
       a.chunk = xpass_chunk_auto(zw, nplanes, (long)per_cu * cus);
