@@ -451,14 +451,10 @@ int mguard(chan_mlp* m, F&& f) {
 // at once (its 75 KB of LDS weights allow two per CU), so that every
 // workgroup loads the weights once.
 static int mlp_blocks(long n) {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-      cus = v;
-    else
-      cus = 256;
-  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
   const long tiles = (n + 16 * MLP_NT - 1) / (16 * MLP_NT);
   return (int)std::max<long>(1, std::min<long>(2L * cus, (tiles + 3) / 4));
 }
