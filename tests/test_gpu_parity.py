@@ -299,6 +299,30 @@ def test_ten_steps_config0(g0):
     ch.close()
 
 
+def test_ten_steps_reference_defaults(g0):
+    """Config 0 at the reference's own defaults rather than the patched ones:
+    grid stretching 2.75 (PFC/setup.f90:545) and the initial dt rule
+    0.25 lx/nx (PFC/setup.f90:1377, dt 0.154 here) held fixed (dt_flag 0);
+    10 RK3 steps, rel L2 <= 1e-10, divergence <= 1e-12."""
+    g = O.make_grid(g0.nx, g0.ny, g0.nz, g0.nx2, g0.nz2, g0.lx, g0.lz, g0.re, stretch=2.75, fix_mass=1)
+    dt = O.default_dt(g)
+    ch = make_channel(g)
+    U0 = init_cfg0(O, g, seed=5)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=dt, p_grad=-2.0 / g.re)
+    ch.initialize_flow()
+    st = O.initial_state(g, U0, dt=dt)
+    ch.step(10, step0=1)
+    for s in range(1, 11):
+        O.rk3_step(g, st, s)
+    d1 = ch.get_data1()
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
+    assert ch.divergence_max() <= 1e-12
+    assert abs(ch.get_scalars()["t_total"] - st.t_total) <= 1e-13
+    ch.close()
+
+
 def test_step_api_matches_phase_calls(g0):
     """chan_step == the driver's boundary/nonlinear/rhs_solve call sequence."""
     U0 = init_cfg0(O, g0, seed=8, amp=1e-2)
