@@ -729,6 +729,16 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   };
   if (r == 0) rhs(std::true_type{});
   else rhs(std::false_type{});
+  // In the first substep the right-hand side reads rows of the neighbouring
+  // segments (R at j0-1, R and P at j0+SEG), which those segments' lanes
+  // overwrite later in this kernel.  The lanes share one instruction stream,
+  // so it suffices that no store is issued before every load has returned:
+  // the memory clobbers keep the compiler from moving a store above this
+  // point (a single-lane view sees no dependence between another lane's row
+  // and its own stores), and the wait holds the stores until the loads are in.
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  asm volatile("" ::: "memory");
 
   // ---------------- Helmholtz solves (u and w with D2H, v with D2N) ----------------
   {
