@@ -65,3 +65,40 @@ def test_null_context_is_rejected():
     lib = cf.load_library()
     assert lib.chan_sync(None) != 0
     assert lib.chan_destroy(None) == 0
+
+
+C_CLIENT = r"""
+#include <stdio.h>
+#include <string.h>
+#include "chanflow.h"
+int main(void) {
+  chan_params p = {32, 65, 31, 48, 48, 6.28, 3.14, 2500.0, 0.0, 1, 0, 0.95, 0.0, 0.0, 0};  /* nz odd */
+  chan_ctx* ctx = NULL;
+  if (strncmp(chan_version(), "chanflow", 8) != 0) return 2;
+  if (chan_create(&p, &ctx) == 0 || ctx != NULL) return 3;
+  printf("%s\n", chan_last_error(NULL));
+  return chan_destroy(NULL);
+}
+"""
+
+
+def test_plain_c_client(tmp_path):
+    """A C99 program includes the header, links libchanflow.so and calls
+    the ABI: chan_create rejects an odd nz with the library's message, with no
+    device touched."""
+    import shutil
+    import subprocess
+    cf = chanflow()
+    lib_path = cf.load_library()._name
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    exe = tmp_path / "client"
+    lib_dir = os.path.dirname(lib_path)
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", lib_dir, "-lchanflow", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "even" in r.stdout
