@@ -407,6 +407,32 @@ def test_nonlinear_x_lengths(dims):
     ch.close()
 
 
+@pytest.mark.parametrize("dims", [(16, 17, 16, 24, 24), (32, 17, 16, 48, 24), (64, 97, 64, 96, 96),
+                                  (96, 17, 16, 144, 24), (192, 9, 16, 288, 24)])
+def test_nonlinear_linearized_x_lengths(dims):
+    """One linearized nonlinear pass (PFC/pass1_linearized.f90:135-509,
+    products U.u' + u'.U about the laminar base flow, CFL with the base flow
+    included) through each x-pass variant's linearized instance: the workgroup
+    kernel with the runtime-planned transform (nx2/2 = 12) and the
+    wave-per-row kernel with one, two and three points per lane (nx2/2 = 24,
+    48, 72, 144; 64x97x64 is configuration 2's shape, nx2/2 = 144 the
+    headline's x length)."""
+    nx, ny, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, ny, nz, nx2, nz2, 2.0, 1.0, 2800.0, stretch=2.75, fix_mass=0)
+    ch = make_channel(g)
+    base = O.laminar_baseflow(g)
+    ch.set_baseflow(base)
+    s = __import__("helpers").synth()
+    U = O.remove_initial_divergence(g, s.laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=12, amp=1e-3, kmax=6.0))
+    ch.set_state(data1=U)
+    dt, cfl = ch.nonlinear(1, 1, dt=0.002, linearized=True)
+    ref, dtr, cflr = O.nonlinear_linearized(g, U, base, step=1, rk_step=1, dt=0.002, return_cfl=True)
+    assert rel_l2(ch.get_data1(), ref) < TOL1
+    assert dt == dtr
+    assert abs(cfl - cflr) <= 1e-14 * abs(cflr)
+    ch.close()
+
+
 @pytest.fixture(scope="module")
 def g1():
     """Headline shape: Re_c 3300, box 4pi x 2 x 4pi/3, 192x129x192, 288x288, stretch 2.75."""
