@@ -146,6 +146,35 @@ def test_nonlinear_xpass_chunks(g0, chunk):
     assert out[0][1] == out[1][1] and abs(out[1][1] - cflr) <= 1e-14 * abs(cflr)
 
 
+@pytest.mark.parametrize("key,value", [("xpass_wave", 0), ("xpass_prefetch", 0), ("xpass_threads", 256),
+                                       ("zfwd_v2", 0), ("zinv_kx", 4), ("zinv_kx", 16), ("zfwd_kx", 4)])
+@pytest.mark.parametrize("dims", [(32, 65, 32, 48, 48), (192, 9, 32, 288, 48)])
+def test_nonlinear_tuning_variants(dims, key, value):
+    """Every launch-shape variant that chan_set_tuning / chan_autotune can
+    select for the nonlinear pass (the workgroup x-pass, the x-pass without
+    its LDS row prefetch, a smaller x-pass workgroup, the z-forward with A and
+    B together, z-inverse and z-forward workgroups of 4 or 16 columns), at
+    nx2/2 = 24 and at the headline's 144: the same term as the default
+    kernels to 1e-12 and the oracle's to 1e-12, CFL included."""
+    nx, ny, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, ny, nz, nx2, nz2, 2.0, 1.0, 2500.0, stretch=1.5, fix_mass=1)
+    s = __import__("helpers").synth()
+    U = O.remove_initial_divergence(g, s.laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=13, amp=5e-2, kmax=6.0))
+    out = []
+    for v in (None, value):
+        ch = make_channel(g)
+        if v is not None:
+            ch.set_tuning(key, v)
+        ch.set_state(data1=U)
+        dt, cfl = ch.nonlinear(1, 1, dt=0.002)
+        out.append((ch.get_data1(), cfl))
+        ch.close()
+    ref, _, cflr = O.nonlinear(g, U, step=1, rk_step=1, dt=0.002, return_cfl=True)
+    assert rel_l2(out[1][0], out[0][0]) <= 1e-12
+    assert rel_l2(out[1][0], ref) < TOL1
+    assert abs(out[1][1] - cflr) <= 1e-14 * abs(cflr)
+
+
 def test_nonlinear_dt_flag(g0):
     g = grid_cfg0(O, dt_flag=1)
     ch = make_channel(g)
