@@ -21,7 +21,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libchanflow.so")
+# CHANFLOW_LIB names another build of the library (e.g. one compiled with a
+# different instruction scheduler) to load instead of the in-tree one.
+LIBPATH = os.environ.get("CHANFLOW_LIB") or os.path.join(_HERE, "libchanflow.so")
 
 _dp = C.POINTER(C.c_double)
 
@@ -57,10 +59,12 @@ ENTRY_POINTS = (
     "chan_vtk_write", "chan_write_vtk",
     "chan_les_configure", "chan_les_update", "chan_les_set_tau", "chan_les_get_tau", "chan_les_get_coefficient",
     "chan_les_get_fields", "chan_les_features", "chan_les_gated_points", "chan_jet_configure",
+    "chan_strip_divergence_modes", "chan_subtract_pressure_gradient_modes",
 )
 
 _lib = None
 _fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
 
 
 def load_library(path: str = LIBPATH):
@@ -103,6 +107,8 @@ def load_library(path: str = LIBPATH):
                                            C.c_int, C.c_int]),
         "chan_solve_p": (C.c_int, [vp, C.c_int, _dp, _dp]),
         "chan_strip_divergence": (C.c_int, [vp, _dp, _dp, C.c_double]),
+        "chan_strip_divergence_modes": (C.c_int, [vp, C.c_int, _dp, _dp, C.c_double, _ip, _ip]),
+        "chan_subtract_pressure_gradient_modes": (C.c_int, [vp, C.c_int, _dp, _dp, C.c_double, _ip, _ip]),
         "chan_linearized_matvec": (C.c_int, [vp, _dp, _dp, C.c_int, _dp]),
         "chan_divergence_max": (C.c_int, [vp, _dp]),
         "chan_mass_flux": (C.c_int, [vp, _dp]),
@@ -474,6 +480,30 @@ class Channel:
         cp = np.empty((self.nxh, self.nz, self.nyp), np.complex128)
         self._chk(self.lib.chan_strip_divergence(self.h, _ptr(cu), _ptr(cp), coef))
         return cu, cp
+
+    def strip_divergence_modes(self, cu, coef, p, q):
+        """strip_divergence per (kx,kz) column (PFC/solve.f90:14-50): cu
+        complex [nsys][3][ny+1] (Fortran cu(ny+1, 3, nsys)), p, q the
+        reference's 1-based x- and z-mode indices.  Returns (cu, cp)."""
+        cu = _c128(cu).copy()
+        nsys = cu.shape[0]
+        cp = np.empty((nsys, self.nyp), np.complex128)
+        pi = np.ascontiguousarray(p, dtype=np.int32)
+        qi = np.ascontiguousarray(q, dtype=np.int32)
+        self._chk(self.lib.chan_strip_divergence_modes(self.h, nsys, _ptr(cu), _ptr(cp), coef,
+                                                       pi.ctypes.data_as(_ip), qi.ctypes.data_as(_ip)))
+        return cu, cp
+
+    def subtract_pressure_gradient_modes(self, crhs, cpress, coef, p, q):
+        """subtract_pressure_gradient per column (PFC/solve.f90:54-77)."""
+        crhs = _c128(crhs).copy()
+        cpress = _c128(cpress)
+        pi = np.ascontiguousarray(p, dtype=np.int32)
+        qi = np.ascontiguousarray(q, dtype=np.int32)
+        self._chk(self.lib.chan_subtract_pressure_gradient_modes(self.h, crhs.shape[0], _ptr(crhs), _ptr(cpress),
+                                                                 coef, pi.ctypes.data_as(_ip),
+                                                                 qi.ctypes.data_as(_ip)))
+        return crhs
 
     def linearized_matvec(self, v, tend):
         v = _f64(v)

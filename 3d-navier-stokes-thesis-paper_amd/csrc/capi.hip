@@ -578,6 +578,8 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     sc.dt = 0.25 * p.lx / (double)p.nx;       // PFC/setup.f90:1377
     sc.p_grad = -2.0 / p.re;                  // PFC/setup.f90:1384
     h2d(c, c->d_sc, &sc, sizeof(sc));
+    c->h_dt = sc.dt;
+    c->h_dt_known = true;
     CHAN_CHECK(hipDeviceSynchronize());
   } catch (const chan_error& e) {
     g_create_error = e.msg;
@@ -766,6 +768,8 @@ int chan_set_scalars(chan_ctx* c, const double* dt, const double* t_total, const
     if (t_total) sc.t_total = *t_total;
     if (p_grad) sc.p_grad = *p_grad;
     h2d(c, c->d_sc, &sc, sizeof(sc));
+    c->h_dt = sc.dt;
+    c->h_dt_known = true;
     if (d_um_dy) h2d(c, c->d_dudy, d_um_dy, sizeof(double) * c->prm.ny);
     CHAN_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -791,24 +795,41 @@ int chan_set_wall_bc(chan_ctx* c, const double* top, const double* bot) {
   });
 }
 
+// The caller's dt goes to the device only when it differs from the value the
+// host last knew there.  Only rk_step 1 can change dt (the CFL check and the
+// adaptation, PFC/pass1.f90:106-124), so only then does the call wait for
+// the kernels and read dt and the CFL number back; at rk_step 2 and 3 it
+// returns as soon as the pass is queued, with dt unchanged.
 static int nonlinear_common(chan_ctx* c, int step, int rk, double* dt_inout, double* cfl_out, bool lin) {
   return guarded(c, [&] {
     check_ops(c);
-    if (dt_inout) {
-      DevScalars sc;
-      d2h(c, &sc, c->d_sc, sizeof(sc));
-      sc.dt = *dt_inout;
-      h2d(c, c->d_sc, &sc, sizeof(sc));
+    if (dt_inout && !(c->h_dt_known && c->h_dt == *dt_inout)) {
+      h2d(c, &c->d_sc->dt, dt_inout, sizeof(double));
+      c->h_dt = *dt_inout;
+      c->h_dt_known = true;
     }
     ensure_vel_in_vel(c);
     nl_pass(c, step, rk, lin, rk == 1);
     c->data1_is_nl = true;
-    CHAN_CHECK(hipStreamSynchronize(c->stream));
-    if (rk == 1) check_err(c);
-    DevScalars sc;
-    d2h(c, &sc, c->d_sc, sizeof(sc));
-    if (dt_inout) *dt_inout = sc.dt;
-    if (cfl_out) *cfl_out = (rk == 1) ? sc.cfl : 0.0;
+    if (rk == 1) {
+      CHAN_CHECK(hipStreamSynchronize(c->stream));
+      check_err(c);
+      DevScalars sc;
+      d2h(c, &sc, c->d_sc, sizeof(sc));
+      c->h_dt = sc.dt;
+      c->h_dt_known = true;
+      if (dt_inout) *dt_inout = sc.dt;
+      if (cfl_out) *cfl_out = sc.cfl;
+    } else {
+      if (dt_inout && !c->h_dt_known) {
+        DevScalars sc;
+        d2h(c, &sc, c->d_sc, sizeof(sc));
+        c->h_dt = sc.dt;
+        c->h_dt_known = true;
+      }
+      if (dt_inout) *dt_inout = c->h_dt;
+      if (cfl_out) *cfl_out = 0.0;
+    }
   });
 }
 
@@ -887,6 +908,7 @@ int chan_step(chan_ctx* c, int step0, int nsteps, int lin) {
   return guarded(c, [&] {
     check_ops(c);
     ensure_vel_in_vel(c);
+    c->h_dt_known = false;
     for (int s = 0; s < nsteps; ++s) step_body(c, step0 + s, lin != 0);
     c->data1_is_nl = false;
     CHAN_CHECK(hipStreamSynchronize(c->stream));
@@ -1056,12 +1078,74 @@ int chan_strip_divergence(chan_ctx* c, double* cu, double* cp, double coef) {
   });
 }
 
+// 1-based reference mode indices -> 0-based pairs on the device (after cu in the staging buffer)
+static int* stage_cols(chan_ctx* c, char* at, int nsys, const int* p, const int* q, bool reject_mean) {
+  if (!p || !q) throw chan_error("mode indices p, q are required");
+  std::vector<int> cols(2 * (size_t)nsys);
+  for (int s = 0; s < nsys; ++s) {
+    const int ix = p[s] - 1, iz = q[s] - 1;
+    if (ix < 0 || ix >= c->nxh || iz < 0 || iz >= c->prm.nz)
+      throw chan_error("mode index out of range (p = " + std::to_string(p[s]) + ", q = " + std::to_string(q[s]) + ")");
+    if (reject_mean && ix == 0 && iz == 0) throw chan_error("strip_divergence: the mean mode has no projection");
+    cols[2 * s] = ix;
+    cols[2 * s + 1] = iz;
+  }
+  h2d(c, at, cols.data(), sizeof(int) * cols.size());
+  return (int*)at;
+}
+
+int chan_strip_divergence_modes(chan_ctx* c, int nsys, double* cu, double* cp, double coef, const int* p,
+                                const int* q) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (nsys < 0) throw chan_error("nsys must be >= 0");
+    if (nsys == 0) return;
+    const size_t nyp = (size_t)c->nyp;
+    const size_t vb = sizeof(double2) * 3 * nyp * nsys, pb = sizeof(double2) * nyp * nsys;
+    const size_t sb = sizeof(double2) * nyp * nsys + sizeof(double) * nyp * nsys, ib = sizeof(int) * 2 * nsys;
+    char* cur = (char*)stage(c, vb + pb + sb + ib + 4 * 256);
+    char* dv = carve(cur, vb);
+    char* dp = carve(cur, pb);
+    char* ds = carve(cur, sb);
+    char* di = carve(cur, ib);
+    const int* cols = stage_cols(c, di, nsys, p, q, true);
+    h2d(c, dv, cu, vb);
+    launch_project(c->ops, c->prm.ny, c->nxh, c->prm.nz, (const double2*)dv, (double2*)dv, (double2*)dp,
+                   (double2*)ds, coef, 2, nullptr, c->prm.re, c->stream, cols, nsys);
+    d2h(c, cu, dv, vb);
+    if (cp) d2h(c, cp, dp, pb);
+  });
+}
+
+int chan_subtract_pressure_gradient_modes(chan_ctx* c, int nsys, double* crhs, const double* cpress,
+                                          double coef, const int* p, const int* q) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (nsys < 0) throw chan_error("nsys must be >= 0");
+    if (nsys == 0) return;
+    if (!cpress) throw chan_error("subtract_pressure_gradient: cpress is required");
+    const size_t nyp = (size_t)c->nyp;
+    const size_t vb = sizeof(double2) * 3 * nyp * nsys, pb = sizeof(double2) * nyp * nsys;
+    const size_t ib = sizeof(int) * 2 * nsys;
+    char* cur = (char*)stage(c, vb + pb + ib + 3 * 256);
+    char* dv = carve(cur, vb);
+    char* dp = carve(cur, pb);
+    char* di = carve(cur, ib);
+    const int* cols = stage_cols(c, di, nsys, p, q, false);
+    h2d(c, dv, crhs, vb);
+    h2d(c, dp, cpress, pb);
+    launch_subtract_grad_cols(c->ops, c->prm.ny, (double2*)dv, (const double2*)dp, coef, cols, nsys, c->stream);
+    d2h(c, crhs, dv, vb);
+  });
+}
+
 int chan_linearized_matvec(chan_ctx* c, const double* v, double* w, int tend, double* t_out) {
   return guarded(c, [&] {
     check_ops(c);
     if (!c->d_base) throw chan_error("base flow not set (chan_set_baseflow)");
     if (c->prm.fix_mass != 0) throw chan_error("the linearized matvec needs fix_mass = 0");
     if (tend < 1) throw chan_error("tend must be >= 1");
+    c->h_dt_known = false;
     const XZGeom& g = c->gs;
     const int nyp = c->nyp, ny = c->prm.ny;
     const long plane_r = (long)g.nx2 * g.nz2;          // nx * nz reals
@@ -1278,6 +1362,7 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
   return guarded(c, [&] {
     if (steps < 1) throw chan_error("steps must be >= 1");
     prepare_capture(c);
+    c->h_dt_known = false;
     const size_t vb = vel_bytes(c), pb = p_bytes(c), db = sizeof(double) * c->prm.ny;
     char* bk = (char*)dalloc(2 * vb + pb + sizeof(DevScalars) + db + 256);
     auto save = [&](bool back) {
@@ -1323,10 +1408,15 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
         CHAN_CHECK(hipEventSynchronize(b));
         float ms = 0.f;
         CHAN_CHECK(hipEventElapsedTime(&ms, a, b));
+        // a candidate whose trial aborted on the CFL check or blew up is not
+        // eligible; its state is discarded with the others' by the restore
+        DevScalars tsc;
+        d2h(c, &tsc, c->d_sc, sizeof(tsc));
+        const bool ok = tsc.err == 0 && std::isfinite(tsc.cfl);
         save(true);
-        check_err(c);
-        if (ms / steps < best) { best = ms / steps; bestt = t; }
+        if (ok && ms / steps < best) { best = ms / steps; bestt = t; }
       }
+      if (!(best < 1e300)) throw chan_error("autotune: every candidate's trial steps failed the CFL check");
     } catch (...) {
       c->tune = bestt;
       (void)hipEventDestroy(a);
@@ -1383,6 +1473,7 @@ int chan_ensemble_step(chan_ensemble* e, int nsteps) {
     for (chan_ctx* c : e->m) {
       check_ops(c);
       ensure_vel_in_vel(c);
+      c->h_dt_known = false;
       CHAN_CHECK(hipStreamSynchronize(c->stream));
     }
     if (!e->exec) ensemble_capture(e);

@@ -148,6 +148,10 @@ struct chan_ctx {
   void* h_pin = nullptr;         // pinned host staging (hipHostMalloc)
   size_t pin_bytes = 0;
   bool data1_is_nl = false;      // Fortran's data1 currently holds the NL term
+  // The device dt as last set or read by the host; h_dt_known is cleared by
+  // every entry point that may change dt on the device without reading it back.
+  double h_dt = 0.0;
+  bool h_dt_known = false;
   LesState* les = nullptr;       // subgrid closure (chan_les_configure)
   double2* d_jet = nullptr;      // wall jet: spectral v profile [nxh][nz] (chan_jet_configure)
   int jet_wall = 0;              // 0 bottom, 1 top
@@ -212,9 +216,13 @@ void launch_l2_final(const double* part, int n, double npts, const double* dudy,
 void launch_mode_energy(const DevOps& ops, int ny, int nxh, int nz, const double2* vel, double* ke, hipStream_t s);
 void launch_plane_moments(const double* phys, long ps, long cs, int nyp, double* out, hipStream_t s);
 void launch_vtk_field(int ny, long nm, const double2* vel, double2* out, hipStream_t s);
+// cols == nullptr: every (kx, kz) of a [3][ny+1][nxh][nz] field; else ncols
+// columns complex(ny+1, 3, ncols) with 0-based mode indices cols[2m], cols[2m+1]
 void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
                     double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
-                    double re, hipStream_t s);
+                    double re, hipStream_t s, const int* cols = nullptr, int ncols = 0);
+void launch_subtract_grad_cols(const DevOps& ops, int ny, double2* crhs, const double2* cpress, double coef,
+                               const int* cols, int ncols, hipStream_t s);
 void launch_cfl_finalize(const DevOps& ops, chan_params prm, unsigned long long* cfl, DevScalars* sc,
                          int step, hipStream_t s);
 void launch_fill_zero(void* p, size_t bytes, hipStream_t s);

@@ -20,13 +20,15 @@ MODULE chanflow_f
   PRIVATE
   PUBLIC :: chan_params, chan_operators, chanflow_ctx, chanflow_check, &
             chanflow_create, chanflow_upload_operators, chanflow_upload_state, &
-            chanflow_download_state, chanflow_download_planes, chanflow_on_device, &
+            chanflow_download_state, chanflow_download_data1, chanflow_download_planes, &
+            chanflow_on_device, &
             chanflow_set_scalars, chanflow_get_scalars, chanflow_destroy, &
             chan_version, chan_nonlinear, chan_nonlinear_linearized, chan_set_baseflow, &
             chan_rhs_solve, chan_set_wall_bc, chan_xz_fft_complex_to_real, &
             chan_xz_fft_real_to_complex, chan_compute_divergence_of_fluxes, &
             chan_small_fft_complex_to_real, chan_small_fft_real_to_complex, &
             chan_solve_helmholtz, chan_solve_p, chan_linearized_matvec, c_string, &
+            chan_strip_divergence_modes, chan_subtract_pressure_gradient_modes, &
             chan_create, chan_destroy, chan_last_error, chan_set_state, chan_get_state, &
             chan_set_scalars, chan_build_operators, chan_initialize_flow, chan_boundary
 
@@ -126,9 +128,9 @@ MODULE chanflow_f
      END FUNCTION chan_set_baseflow
      INTEGER(c_int) FUNCTION chan_rhs_solve(ctx, step, rk_step, p_grad) BIND(C, name='chan_rhs_solve')
        IMPORT :: c_int, c_ptr, c_double
-       TYPE(c_ptr), VALUE    :: ctx
-       INTEGER(c_int), VALUE :: step, rk_step
-       REAL(c_double)        :: p_grad
+       TYPE(c_ptr), VALUE       :: ctx
+       INTEGER(c_int), VALUE    :: step, rk_step
+       REAL(c_double), OPTIONAL :: p_grad
      END FUNCTION chan_rhs_solve
      INTEGER(c_int) FUNCTION chan_set_wall_bc(ctx, cbc_top, cbc_bot) BIND(C, name='chan_set_wall_bc')
        IMPORT :: c_int, c_ptr, c_double_complex
@@ -192,6 +194,26 @@ MODULE chanflow_f
        COMPLEX(c_double_complex), INTENT(INOUT) :: p(*)
        REAL(c_double), INTENT(IN)               :: alpha(*)
      END FUNCTION chan_solve_p
+     INTEGER(c_int) FUNCTION chan_strip_divergence_modes(ctx, nsys, cu, cp, coef, p, q) &
+          BIND(C, name='chan_strip_divergence_modes')
+       IMPORT :: c_int, c_ptr, c_double, c_double_complex
+       TYPE(c_ptr), VALUE                       :: ctx
+       INTEGER(c_int), VALUE                    :: nsys
+       COMPLEX(c_double_complex), INTENT(INOUT) :: cu(*)
+       COMPLEX(c_double_complex), INTENT(OUT)   :: cp(*)
+       REAL(c_double), VALUE                    :: coef
+       INTEGER(c_int), INTENT(IN)               :: p(*), q(*)
+     END FUNCTION chan_strip_divergence_modes
+     INTEGER(c_int) FUNCTION chan_subtract_pressure_gradient_modes(ctx, nsys, crhs, cpress, coef, p, q) &
+          BIND(C, name='chan_subtract_pressure_gradient_modes')
+       IMPORT :: c_int, c_ptr, c_double, c_double_complex
+       TYPE(c_ptr), VALUE                       :: ctx
+       INTEGER(c_int), VALUE                    :: nsys
+       COMPLEX(c_double_complex), INTENT(INOUT) :: crhs(*)
+       COMPLEX(c_double_complex), INTENT(IN)    :: cpress(*)
+       REAL(c_double), VALUE                    :: coef
+       INTEGER(c_int), INTENT(IN)               :: p(*), q(*)
+     END FUNCTION chan_subtract_pressure_gradient_modes
      INTEGER(c_int) FUNCTION chan_linearized_matvec(ctx, v, w, tend, t_total) &
           BIND(C, name='chan_linearized_matvec')
        IMPORT :: c_int, c_ptr, c_double
@@ -275,6 +297,13 @@ CONTAINS
          'chan_set_state')
     chanflow_on_device = .TRUE.
   END SUBROUTINE chanflow_upload_state
+
+  ! data1 alone (the NL term between nonlinear and rhs_solve, else the velocity)
+  SUBROUTINE chanflow_download_data1(data1)
+    COMPLEX(c_double_complex), INTENT(INOUT), TARGET :: data1(*)
+    CALL chanflow_check(chan_get_state(chanflow_ctx, C_LOC(data1), c_null_ptr, c_null_ptr), &
+         'chan_get_state')
+  END SUBROUTINE chanflow_download_data1
 
   SUBROUTINE chanflow_download_state(data1, data2, data4)
     COMPLEX(c_double_complex), INTENT(INOUT), TARGET :: data1(*), data2(*), data4(*)

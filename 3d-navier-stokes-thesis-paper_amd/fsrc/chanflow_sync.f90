@@ -11,16 +11,16 @@ MODULE chanflow_sync
   USE chanflow_f
   IMPLICIT NONE
   PRIVATE
-  PUBLIC :: chanflow_ensure_device, chanflow_sync_host, chanflow_host_reads_step
+  PUBLIC :: chanflow_ensure_ops, chanflow_ensure_device, chanflow_sync_host, chanflow_host_reads_step
 
   LOGICAL, SAVE :: ops_uploaded = .FALSE.
 
 CONTAINS
 
-  ! First call of a run (or after the host arrays were declared newer by
-  ! setting chanflow_on_device = .FALSE.): operators, state and scalars go
-  ! to the device.
-  SUBROUTINE chanflow_ensure_device()
+  ! The context and the operators, for the routines that work on host arrays
+  ! passed to them (transform, solve): they neither read nor change the
+  ! device copy of the module state.
+  SUBROUTINE chanflow_ensure_ops()
     IF (.NOT. C_ASSOCIATED(chanflow_ctx)) &
          CALL chanflow_create(nx, ny, nz, nx2, nz2, lx, lz, re, beta, fix_mass, dt_flag, &
          cfl_max, ctrlu, ctrlw, bl_grid)
@@ -31,6 +31,14 @@ CONTAINS
             buffer, target_inflow)
        ops_uploaded = .TRUE.
     END IF
+  END SUBROUTINE chanflow_ensure_ops
+
+  ! Before a routine that works on the module state (nonlinear, rhs_solve):
+  ! the first such call of a run, or the first after the host arrays were
+  ! declared newer by setting chanflow_on_device = .FALSE., sends the
+  ! operators, the state and the scalars to the device.
+  SUBROUTINE chanflow_ensure_device()
+    CALL chanflow_ensure_ops()
     IF (.NOT. chanflow_on_device) THEN
        CALL chanflow_upload_state(data1, data2, data4)
        CALL chanflow_set_scalars(dt, t_total, p_grad, d_um_dy)

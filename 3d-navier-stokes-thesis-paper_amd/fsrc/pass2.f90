@@ -18,14 +18,21 @@ MODULE pass2
 
 CONTAINS
 
-  ! PFC/pass2.f90:29-180
+  ! PFC/pass2.f90:29-180.  With crossflow = 0 the wall data change only
+  ! through the feedback law of read_write::boundary, so with both gains zero
+  ! cbc_top/cbc_bot (zeroed by boundary every step) go to the device once.
+  ! p_grad stays on the device between substeps and comes back with the
+  ! state whenever the driver reads the host arrays.
   SUBROUTINE rhs_solve
     INTEGER :: c
+    LOGICAL, SAVE :: wall_data_sent = .FALSE.
     IF (crossflow == 1) STOP 'chanflow: jets in crossflow are not provided (crossflow must be 0)'
     CALL chanflow_ensure_device()
-    IF (rk_step == 1) &
-         CALL chanflow_check(chan_set_wall_bc(chanflow_ctx, cbc_top, cbc_bot), 'rhs_solve wall data')
-    CALL chanflow_check(chan_rhs_solve(chanflow_ctx, step, rk_step, p_grad), 'rhs_solve')
+    IF (rk_step == 1 .AND. (ctrlu /= 0.d0 .OR. ctrlw /= 0.d0 .OR. .NOT. wall_data_sent)) THEN
+       CALL chanflow_check(chan_set_wall_bc(chanflow_ctx, cbc_top, cbc_bot), 'rhs_solve wall data')
+       wall_data_sent = .TRUE.
+    END IF
+    CALL chanflow_check(chan_rhs_solve(chanflow_ctx, step, rk_step), 'rhs_solve')
     IF (rk_step /= rk_last) RETURN
     IF (chanflow_host_reads_step(step)) THEN
        CALL chanflow_sync_host()

@@ -10,7 +10,6 @@ MODULE solve
   USE, INTRINSIC :: iso_c_binding
   USE runparms
   USE gridparms
-  USE diff_int
   USE chanflow_f
   USE chanflow_sync
   IMPLICIT NONE
@@ -20,37 +19,33 @@ MODULE solve
 
 CONTAINS
 
-  ! PFC/solve.f90:14-50
+  ! PFC/solve.f90:14-50: one (kx,kz) column, projected on the GPU
+  ! (chan_strip_divergence_modes); p, q are the x- and z-mode indices.
   SUBROUTINE strip_divergence(cu, cp, coef, p, q)
     COMPLEX(KIND=prec), INTENT(INOUT) :: cu(1:ny+1,3)
     COMPLEX(KIND=prec), INTENT(OUT)   :: cp(1:ny+1)
     REAL(KIND=prec), INTENT(IN)       :: coef
     INTEGER, INTENT(IN)               :: p, q
-    COMPLEX(KIND=prec) :: dv(1:ny+1), topbc, botbc
-    REAL(KIND=prec)    :: tmp, alpha
-    tmp = 1.d0/coef
-    CALL d_dy(ny, cu(1,2), ny+1, dv, dh)
-    cp(1)    = 0.d0
-    cp(2:ny) = tmp*(ikx_me(p)*cu(2:ny,1) + dv(2:ny) + ikz(q)*cu(2:ny,3))
-    cp(ny+1) = 0.d0
-    topbc = 0.d0
-    botbc = 0.d0
-    alpha = -k2_me(q,p)
-    CALL solve_p(cp, alpha, topbc, botbc)
-    CALL subtract_pressure_gradient(cu, cp, coef, p, q)
+    INTEGER(c_int) :: ip(1), iq(1)
+    CALL chanflow_ensure_ops()
+    ip(1) = p
+    iq(1) = q
+    CALL chanflow_check(chan_strip_divergence_modes(chanflow_ctx, 1, cu, cp, coef, ip, iq), 'strip_divergence')
   END SUBROUTINE strip_divergence
 
-  ! PFC/solve.f90:54-77
+  ! PFC/solve.f90:54-77: one column, on the GPU
+  ! (chan_subtract_pressure_gradient_modes).
   SUBROUTINE subtract_pressure_gradient(crhs, cpress, coef, p, q)
     COMPLEX(KIND=prec), DIMENSION(1:ny+1,1:3), INTENT(INOUT) :: crhs
     COMPLEX(KIND=prec), DIMENSION(1:ny+1), INTENT(IN)        :: cpress
     REAL(KIND=prec), INTENT(IN)                              :: coef
     INTEGER, INTENT(IN) :: p, q
-    COMPLEX(KIND=prec), DIMENSION(1:ny) :: dp
-    CALL d_dy(ny+1, cpress, ny, dp, dn)
-    crhs(2:ny,1)   = crhs(2:ny,1)   - coef*ikx_me(p)*cpress(2:ny)
-    crhs(2:ny-1,2) = crhs(2:ny-1,2) - coef*dp(2:ny-1)
-    crhs(2:ny,3)   = crhs(2:ny,3)   - coef*ikz(q)*cpress(2:ny)
+    INTEGER(c_int) :: ip(1), iq(1)
+    CALL chanflow_ensure_ops()
+    ip(1) = p
+    iq(1) = q
+    CALL chanflow_check(chan_subtract_pressure_gradient_modes(chanflow_ctx, 1, crhs, cpress, coef, ip, iq), &
+         'subtract_pressure_gradient')
   END SUBROUTINE subtract_pressure_gradient
 
   ! PFC/solve.f90:154-264 (Dirichlet rows; the channel never sets the
@@ -61,7 +56,7 @@ CONTAINS
     REAL(KIND=prec), INTENT(IN)       :: ALPHA, DIFF, D2(3,N)
     COMPLEX(KIND=prec), INTENT(IN)    :: TOPBC(NRHS), BOTBC(NRHS)
     REAL(KIND=prec) :: al(NRHS)
-    CALL chanflow_ensure_device()
+    CALL chanflow_ensure_ops()
     al = ALPHA
     CALL chanflow_check(chan_solve_helmholtz(chanflow_ctx, N, NRHS, F, al, DIFF, D2, TOPBC, BOTBC, &
          TOP_NEUMANN, BOT_NEUMANN), 'solve_helmholtz')
@@ -90,7 +85,7 @@ CONTAINS
     REAL(KIND=prec), INTENT(IN)       :: ALPHA
     COMPLEX(KIND=prec), INTENT(IN)    :: TOPBC, BOTBC
     REAL(KIND=prec) :: al(1)
-    CALL chanflow_ensure_device()
+    CALL chanflow_ensure_ops()
     al(1) = ALPHA
     CALL chanflow_check(chan_solve_p(chanflow_ctx, 1, P, al), 'solve_p')
     IF (BOTBC /= (0.d0, 0.d0)) P(1) = P(1) + BOTBC/DM(1,1)

@@ -14,6 +14,7 @@ MODULE transform
   USE runparms
   USE gridparms
   USE chanflow_f
+  USE chanflow_sync
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: initialize_fft,             finalize_fft,                  &
@@ -42,6 +43,7 @@ CONTAINS
   SUBROUTINE xz_fft_real_to_complex(data, cdata)
     REAL(KIND=prec), INTENT(IN)     :: data(1:nx2,1:local_nz)
     COMPLEX(KIND=prec), INTENT(OUT) :: cdata(1:nz,1:local_nx)
+    CALL chanflow_ensure_ops()
     CALL chanflow_check(chan_xz_fft_real_to_complex(chanflow_ctx, data, cdata, 1), &
          'xz_fft_real_to_complex')
   END SUBROUTINE xz_fft_real_to_complex
@@ -50,6 +52,7 @@ CONTAINS
   SUBROUTINE xz_fft_complex_to_real(cdata, data)
     COMPLEX(KIND=prec), INTENT(IN) :: cdata(1:nz,1:local_nx)
     REAL(KIND=prec), INTENT(OUT)   :: data(1:nx2,1:local_nz)
+    CALL chanflow_ensure_ops()
     CALL chanflow_check(chan_xz_fft_complex_to_real(chanflow_ctx, cdata, data, 1), &
          'xz_fft_complex_to_real')
   END SUBROUTINE xz_fft_complex_to_real
@@ -129,6 +132,7 @@ CONTAINS
   SUBROUTINE small_fft_complex_to_real(cdata, data)
     COMPLEX(KIND=prec), INTENT(IN) :: cdata(1:nz,1:local_nx)
     REAL(KIND=prec), INTENT(OUT)   :: data(1:nx,1:local_nz_small)
+    CALL chanflow_ensure_ops()
     CALL chanflow_check(chan_small_fft_complex_to_real(chanflow_ctx, cdata, data, 1), &
          'small_fft_complex_to_real')
   END SUBROUTINE small_fft_complex_to_real
@@ -146,6 +150,7 @@ CONTAINS
   SUBROUTINE small_fft_real_to_complex(data, cdata)
     REAL(KIND=prec), INTENT(IN)     :: data(1:nx,1:local_nz_small)
     COMPLEX(KIND=prec), INTENT(OUT) :: cdata(1:nz,1:local_nx)
+    CALL chanflow_ensure_ops()
     CALL chanflow_check(chan_small_fft_real_to_complex(chanflow_ctx, data, cdata, 1), &
          'small_fft_real_to_complex')
   END SUBROUTINE small_fft_real_to_complex
@@ -154,6 +159,7 @@ CONTAINS
   SUBROUTINE compute_divergence_of_fluxes(datax, datay, dataz, cdata)
     REAL(KIND=prec), INTENT(IN), DIMENSION(1:nx2,1:local_nz) :: datax, datay, dataz
     COMPLEX(KIND=prec), INTENT(OUT) :: cdata(1:nz,1:local_nx)
+    CALL chanflow_ensure_ops()
     CALL chanflow_check(chan_compute_divergence_of_fluxes(chanflow_ctx, datax, datay, dataz, cdata, 1), &
          'compute_divergence_of_fluxes')
   END SUBROUTINE compute_divergence_of_fluxes

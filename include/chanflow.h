@@ -160,6 +160,20 @@ int chan_solve_p(chan_ctx* ctx, int nsys, double* p, const double* alpha);
  * (data1 layout) with scale coef; cp complex(ny+1, nz, nx/2+1) receives the
  * pressure correction (may be NULL). */
 int chan_strip_divergence(chan_ctx* ctx, double* cu, double* cp, double coef);
+/* The per-mode forms of the same routines, for nsys independent (kx,kz)
+ * columns at once.  p(nsys), q(nsys) are the reference's 1-based x- and
+ * z-mode indices (x = 1..nx/2+1, z = 1..nz), so that a column is the
+ * reference's cu(1:ny+1, 1:3) of mode (q, p).
+ * strip_divergence (PFC/solve.f90:14-50): cu complex(ny+1, 3, nsys) in/out,
+ *   cp complex(ny+1, nsys) out (may be NULL).  The mean mode (k^2 = 0) is
+ *   rejected, as the reference's callers never pass it (its Neumann system
+ *   is singular).
+ * subtract_pressure_gradient (PFC/solve.f90:54-77): crhs complex(ny+1, 3,
+ *   nsys) in/out, cpress complex(ny+1, nsys). */
+int chan_strip_divergence_modes(chan_ctx* ctx, int nsys, double* cu, double* cp, double coef, const int* p,
+                                const int* q);
+int chan_subtract_pressure_gradient_modes(chan_ctx* ctx, int nsys, double* crhs, const double* cpress,
+                                          double coef, const int* p, const int* q);
 
 /* arnoldi::compute_time_deriv (PFC/arnoldi.f90:485-567): w = W P_T W^-1 v
  * with v, w real(nx, nz, ny+1, 3), P_T = tend linearized RK3 steps.  Needs

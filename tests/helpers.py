@@ -61,3 +61,29 @@ def make_channel(g, **kw):
                     dt_flag=g.dt_flag, cfl_max=g.cfl_max, ctrlu=g.ctrlu, ctrlw=g.ctrlw, **kw)
     ch.upload_grid(g)
     return ch
+
+
+def write_flow_dat(path, re, lx_over_pi, aspect, nx, ny, nz, nx2, nz2, tend, output_step, save_step, l2_step,
+                   read_vel=1, save_vel=1, fix_mass=1, dt_flag=0, ctrlu=0.0, ctrlw=0.0, beta=0.0, which_test=1):
+    """The run-parameter file the reference's read_parms reads line by line
+    (PFC/setup.f90:56-160): one value per line in that order, free text on the
+    lines it skips.  Channel runs: no scalars, no subgrid model, no jets, no
+    fringe, channel walls."""
+    rows = [
+        f"{re!r} re", "-", "-", "-", f"{ctrlu!r} ctrlu", f"{ctrlw!r} ctrlw", f"{lx_over_pi!r} lx",
+        f"{aspect!r} lx/lz", "1 scale_by_pi", f"{nx} nx", f"{ny} ny", f"{nz} nz", f"{nx2} nx2", f"{nz2} nz2",
+        "0 nscalar", "", "", "0 split_scalar", "0 subgrid", "3 filter_dim", "0 lagr_avg", "0 dynamic_scalar",
+        f"{tend} tend", f"{output_step} output_step", f"{save_step} save_step", f"{l2_step} l2_step",
+        f"{read_vel} read_vel", f"{save_vel} save_vel", "0 vel_out", "-", "0 snap_out", "-",
+        f"{which_test} which_test", "1.d-3 disturb_ampl"] + ["-"] * 10 + [
+        "0 test_run", f"{fix_mass} fix_mass", f"{dt_flag} dt_flag", "0 divg_check", f"{beta!r} beta", "-", "-",
+        "- output", "1 xskip", "1 yskip", "1 zskip", "0 filter_output",
+        "- grid", "0 bl_grid", "1.0 ly", "0.5 ly_half", "0.33 delta_bl", "5 nbl",
+        "- jet", "0 crossflow", "0 filter_jet_profile", "-", "1.0 jet_diam", "0.0 forcing_phase", "-", "-",
+        "0.125 xjet_top", "0.5 zjet_top", "0.0 jetmag_top", "0.0 forcing_top", "0.2 for_freq_top", "0.0 swirl_top",
+        "-", "0.125 xjet_bot", "0.5 zjet_bot", "0.0 jetmag_bot", "0.0 forcing_bot", "0.19 for_freq_bot",
+        "0.0 swirl_bot",
+        "- fringe", "10.0 fringe_gain", "-", "0 xfringe", "0.25 frac_xbuff", "0.1 buff_xrise", "0.05 buff_xfall",
+        "-", "0 zfringe", "0.1 frac_zbuff", "0.05 buff_zrise", "0.05 buff_zfall"]
+    with open(path, "w") as f:
+        f.write("\n".join(rows) + "\n")
