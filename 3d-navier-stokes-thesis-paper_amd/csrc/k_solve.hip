@@ -676,11 +676,31 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   double2 UW[2][SEG], V[1][SEG];
   auto rhs = [&](auto first_c) {
     constexpr bool FIRST = decltype(first_c)::value;
-    const int jb = min(j0, ny), jm = max(min(j0 - 1, ny), 0);  // rows clamped for idle lanes
+    const int jb = min(j0, ny);  // rows clamped for idle lanes
     double2 Rm0 = cz(), Rm1 = cz(), Rm2 = cz();
     double2 Pc = ldo(P, OM(jb));
-    if (FIRST) { Rm0 = ldo(R, OC(0, jm)); Rm1 = ldo(R, OC(1, jm)); Rm2 = ldo(R, OC(2, jm)); }
     double2 Rc0 = ldo(R, OC(0, jb)), Rc1 = ldo(R, OC(1, jb)), Rc2 = ldo(R, OC(2, jb));
+    // The first substep's stencils reach one row into each neighbouring
+    // segment: R at j0-1 (the explicit diffusion of row j0) and R, P at
+    // j0+SEG (those of the last row).  Every lane reads only its own rows from
+    // memory, the rows it overwrites later in this kernel; the neighbours'
+    // rows come from the lanes that own them, through register shuffles.
+    // Wall rows take the wall values, so what the segments at the ends
+    // receive from beyond the walls (their own rows) is never used.
+    double2 Rx0 = cz(), Rx1 = cz(), Rx2 = cz(), Px = cz();
+    if (FIRST) {
+      const int jl = min(j0 + SEG - 1, ny);
+      const double2 L0 = ldo(R, OC(0, jl)), L1 = ldo(R, OC(1, jl)), L2 = ldo(R, OC(2, jl));
+      const int prv = (s == 0) ? lane : lane - MODES;
+      const int nxt = (s + 1 >= SEGS) ? lane : lane + MODES;
+      Rm0 = sel(s == 0, Rc0, shfl2(L0, prv));
+      Rm1 = sel(s == 0, Rc1, shfl2(L1, prv));
+      Rm2 = sel(s == 0, Rc2, shfl2(L2, prv));
+      Rx0 = shfl2(Rc0, nxt);
+      Rx1 = shfl2(Rc1, nxt);
+      Rx2 = shfl2(Rc2, nxt);
+      Px = shfl2(Pc, nxt);
+    }
 #pragma unroll
     for (int t = 0; t < SEG; ++t) {
       const int j = j0 + t;
@@ -690,12 +710,17 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       stash[t][1][lane] = n1;
       stash[t][2][lane] = n2;
       stash[t][3][lane] = Pc;
-      double2 r0 = cz(), r1 = cz(), r2 = cz();
-      if (t + 1 < SEG || FIRST) { r0 = ldo(R, OC(0, jn)); r1 = ldo(R, OC(1, jn)); r2 = ldo(R, OC(2, jn)); }
+      double2 r0 = cz(), r1 = cz(), r2 = cz(), pn = cz();
+      if (t + 1 < SEG) {
+        r0 = ldo(R, OC(0, jn)); r1 = ldo(R, OC(1, jn)); r2 = ldo(R, OC(2, jn));
+        pn = ldo(P, OM(jn));
+      } else if (FIRST) {
+        r0 = Rx0; r1 = Rx1; r2 = Rx2;
+        pn = Px;
+      }
       double2 v0 = cadd(Rc0, rmul(coef, neg(n0)));
       double2 v1 = cadd(Rc1, rmul(coef, neg(n1)));
       double2 v2 = cadd(Rc2, rmul(coef, neg(n2)));
-      const double2 pn = (t + 1 < SEG || FIRST) ? ldo(P, OM(jn)) : cz();
       if (FIRST) {
         // add_explicit_diffusive_terms(cv, cru) (:339-351)
         const double a0 = d2h[3 * jc], a1 = d2h[3 * jc + 1], a2 = d2h[3 * jc + 2];
@@ -729,16 +754,6 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   };
   if (r == 0) rhs(std::true_type{});
   else rhs(std::false_type{});
-  // In the first substep the right-hand side reads rows of the neighbouring
-  // segments (R at j0-1, R and P at j0+SEG), which those segments' lanes
-  // overwrite later in this kernel.  The lanes share one instruction stream,
-  // so it suffices that no store is issued before every load has returned:
-  // the memory clobbers keep the compiler from moving a store above this
-  // point (a single-lane view sees no dependence between another lane's row
-  // and its own stores), and the wait holds the stores until the loads are in.
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-  asm volatile("" ::: "memory");
 
   // ---------------- Helmholtz solves (u and w with D2H, v with D2N) ----------------
   {
@@ -1469,25 +1484,39 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
 __global__ __launch_bounds__(64) void k_project(DevOps ops, int ny, int nxh, int nz,
                                                 const double2* __restrict__ in, double2* out_vel,
                                                 double2* out_p, double2* scr, double coef, int mode,
-                                                const double2* __restrict__ vel, double re) {
-  const long nm = (long)nxh * nz;
+                                                const double2* __restrict__ vel, double re,
+                                                const int* __restrict__ cols, int ncols) {
+  // Two layouts: the whole field (cols == nullptr; system m = (kx, kz) of
+  // [3][ny+1][nxh][nz], pressure [ny+1][nxh][nz]) or ncols independent
+  // columns (system m has mode indices cols[2m], cols[2m+1]; velocity
+  // complex(ny+1, 3, ncols) and pressure complex(ny+1, ncols), Fortran order).
+  const long nm = cols ? (long)ncols : (long)nxh * nz;
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nm) return;
-  const int ix = (int)(m / nz), iz = (int)(m - (long)ix * nz);
   const int nyp = ny + 1;
-  const long MS = nm;
+  int ix, iz;
+  long vb, vcs, vrs, pb, prs;
+  if (cols) {
+    ix = cols[2 * m]; iz = cols[2 * m + 1];
+    vb = m * 3 * nyp; vcs = nyp; vrs = 1; pb = m * nyp; prs = 1;
+  } else {
+    ix = (int)(m / nz); iz = (int)(m - (long)ix * nz);
+    vb = m; vcs = (long)nyp * nm; vrs = nm; pb = m; prs = nm;
+  }
   const double kxv = ops.kx[ix], kzv = ops.kz[iz];
   const double k2 = kxv * kxv + kzv * kzv;
   const bool mean = (k2 == 0.0);
   const double* d2h = ops.d2h; const double* d2n = ops.d2n; const double* dh = ops.dh;
   const double* dn = ops.dn; const double* dm = ops.dm;
+  auto VI = [&](int c, int j) -> long { return vb + c * vcs + j * vrs; };
+  auto PI = [&](int j) -> long { return pb + j * prs; };
   auto X = [&](int c, int j) -> double2 {
-    if (mode != 1) return in[((long)c * nyp + j) * MS + m];
+    if (mode != 1) return in[VI(c, j)];
     // mode 1: cru = -NL + (1/Re)(D2 u - k2 u)  (:488-501)
-    double2 t = in[((long)c * nyp + j) * MS + m];
+    double2 t = in[VI(c, j)];
     double2 v = make_double2(-t.x, -t.y);
     const double visc = 1.0 / re;
-    auto uu = [&](int cc, int jj) { return vel[((long)cc * nyp + jj) * MS + m]; };
+    auto uu = [&](int cc, int jj) { return vel[VI(cc, jj)]; };
     if (c != 1) {
       int j0 = min(max(j - 1, 0), nyp - 3);
       double2 du = cadd(cadd(rmul(d2h[3 * j], uu(c, j0)), rmul(d2h[3 * j + 1], uu(c, j0 + 1))),
@@ -1502,23 +1531,23 @@ __global__ __launch_bounds__(64) void k_project(DevOps ops, int ny, int nxh, int
     return v;
   };
   double2* F = scr;                                        // f' then phi
-  double* C = reinterpret_cast<double*>(scr + (long)nyp * MS);  // c'
+  double* C = reinterpret_cast<double*>(scr + (long)nyp * nm);  // c'
   if (mean) {
     if (mode == 0) {
       // set the mean wall-normal velocity to its bottom-wall value (:526-528)
-      double2 v0 = in[((long)1 * nyp + 0) * MS + m];
+      double2 v0 = in[VI(1, 0)];
       for (int c = 0; c < 3; ++c)
         for (int j = 0; j <= ny; ++j) {
-          double2 v = in[((long)c * nyp + j) * MS + m];
+          double2 v = in[VI(c, j)];
           if (c == 1 && j >= 1 && j <= ny - 1) v = v0;
-          out_vel[((long)c * nyp + j) * MS + m] = v;
+          out_vel[VI(c, j)] = v;
         }
     } else if (mode == 2 && out_vel && out_vel != in) {
       for (int c = 0; c < 3; ++c)
         for (int j = 0; j <= ny; ++j)
-          out_vel[((long)c * nyp + j) * MS + m] = in[((long)c * nyp + j) * MS + m];
+          out_vel[VI(c, j)] = in[VI(c, j)];
     }
-    if (out_p) for (int j = 0; j <= ny; ++j) out_p[(long)j * MS + m] = cz();
+    if (out_p) for (int j = 0; j <= ny; ++j) out_p[PI(j)] = cz();
     return;
   }
   const double tmp = 1.0 / coef, alp = -k2;
@@ -1543,54 +1572,82 @@ __global__ __launch_bounds__(64) void k_project(DevOps ops, int ny, int nxh, int
     double rb = 1.0 / bp;
     cprev = c_ * rb;
     fprev = rmul(rb, g);
-    F[(long)i * MS + m] = fprev;
-    C[(long)i * MS + m] = cprev;
+    F[PI(i)] = fprev;
+    C[PI(i)] = cprev;
   }
   // back substitution (descending)
-  double2 x1 = F[(long)ny * MS + m];
+  double2 x1 = F[PI(ny)];
   double2 p1 = cz(), p2 = cz(), pm1 = cz(), pm2 = cz();
   for (int i = ny - 1; i >= 0; --i) {
-    double2 x = csub(F[(long)i * MS + m], rmul(C[(long)i * MS + m], x1));
-    F[(long)i * MS + m] = x;
+    double2 x = csub(F[PI(i)], rmul(C[PI(i)], x1));
+    F[PI(i)] = x;
     if (i == 1) p1 = x;
     if (i == 2) p2 = x;
     if (i == ny - 2) pm2 = x;
     if (i == ny - 1) pm1 = x;
     x1 = x;
   }
-  double2 pb = make_double2((0.0 - dm[1] * p1.x - dm[2] * p2.x) / dm[0], (0.0 - dm[1] * p1.y - dm[2] * p2.y) / dm[0]);
-  double2 pt = make_double2((0.0 - dm[3] * pm2.x - dm[4] * pm1.x) / dm[5], (0.0 - dm[3] * pm2.y - dm[4] * pm1.y) / dm[5]);
-  F[0 * MS + m] = pb;
-  F[(long)ny * MS + m] = pt;
-  if (out_p) for (int j = 0; j <= ny; ++j) out_p[(long)j * MS + m] = F[(long)j * MS + m];
+  double2 pbot = make_double2((0.0 - dm[1] * p1.x - dm[2] * p2.x) / dm[0], (0.0 - dm[1] * p1.y - dm[2] * p2.y) / dm[0]);
+  double2 ptop = make_double2((0.0 - dm[3] * pm2.x - dm[4] * pm1.x) / dm[5], (0.0 - dm[3] * pm2.y - dm[4] * pm1.y) / dm[5]);
+  F[PI(0)] = pbot;
+  F[PI(ny)] = ptop;
+  if (out_p) for (int j = 0; j <= ny; ++j) out_p[PI(j)] = F[PI(j)];
   if ((mode == 0 || mode == 2) && out_vel) {
     // subtract_pressure_gradient(cu, cp, coef) (:72-75)
     for (int j = 0; j <= ny; ++j) {
       double2 u = X(0, j), v = X(1, j), w = X(2, j);
-      double2 f0 = F[(long)j * MS + m];
+      double2 f0 = F[PI(j)];
       if (j >= 1 && j <= ny - 1) {
         u = csub(u, imul(coef * kxv, f0));
         w = csub(w, imul(coef * kzv, f0));
       }
-      if (j >= 1 && j <= ny - 2) v = csub(v, rmul(coef * dn[j], csub(F[(long)(j + 1) * MS + m], f0)));
-      out_vel[((long)0 * nyp + j) * MS + m] = u;
-      out_vel[((long)1 * nyp + j) * MS + m] = v;
-      out_vel[((long)2 * nyp + j) * MS + m] = w;
+      if (j >= 1 && j <= ny - 2) v = csub(v, rmul(coef * dn[j], csub(F[PI(j + 1)], f0)));
+      out_vel[VI(0, j)] = u;
+      out_vel[VI(1, j)] = v;
+      out_vel[VI(2, j)] = w;
     }
   }
 }
 
+// subtract_pressure_gradient (PFC/solve.f90:54-77) for ncols independent
+// columns with a given pressure: crhs complex(ny+1, 3, ncols) in place,
+// cpress complex(ny+1, ncols); mode indices cols[2m], cols[2m+1].
+__global__ __launch_bounds__(64) void k_subtract_grad_cols(DevOps ops, int ny, double2* crhs,
+                                                           const double2* __restrict__ cpress, double coef,
+                                                           const int* __restrict__ cols, int ncols) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= ncols) return;
+  const int nyp = ny + 1;
+  const double kxv = ops.kx[cols[2 * m]], kzv = ops.kz[cols[2 * m + 1]];
+  double2* u = crhs + m * 3 * nyp;
+  const double2* p = cpress + m * nyp;
+  for (int j = 1; j <= ny - 1; ++j) {
+    u[j] = csub(u[j], imul(coef * kxv, p[j]));
+    u[2 * nyp + j] = csub(u[2 * nyp + j], imul(coef * kzv, p[j]));
+  }
+  for (int j = 1; j <= ny - 2; ++j) u[nyp + j] = csub(u[nyp + j], rmul(coef * ops.dn[j], csub(p[j + 1], p[j])));
+}
+
 void launch_project(const DevOps& ops, int ny, int nxh, int nz, const double2* in, double2* out_vel,
                     double2* out_p, double2* scr, double coef, int mode, double2* vel_for_visc,
-                    double re, hipStream_t s) {
-  long nm = (long)nxh * nz;
+                    double re, hipStream_t s, const int* cols, int ncols) {
+  long nm = cols ? (long)ncols : (long)nxh * nz;
   if (mode < 0 || mode > 2) throw chan_error("k_project: bad mode");
   if (mode == 1 && !vel_for_visc) throw chan_error("k_project: mode 1 needs the velocity");
   if (mode != 1 && !out_vel) throw chan_error("k_project: projection needs an output field");
+  if (nm <= 0) return;
   dim3 grid((unsigned)((nm + 63) / 64));
   hipLaunchKernelGGL(k_project, grid, dim3(64), 0, s, ops, ny, nxh, nz, in, out_vel, out_p, scr, coef,
-                     mode, vel_for_visc, re);
+                     mode, vel_for_visc, re, cols, ncols);
   LAUNCH_CHECK("k_project");
+}
+
+void launch_subtract_grad_cols(const DevOps& ops, int ny, double2* crhs, const double2* cpress, double coef,
+                               const int* cols, int ncols, hipStream_t s) {
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL(k_subtract_grad_cols, dim3((ncols + 63) / 64), dim3(64), 0, s, ops, ny, crhs, cpress, coef,
+                     cols, ncols);
+  LAUNCH_CHECK("k_subtract_grad_cols");
 }
 
 // ---------------------------------------------------------------------------
