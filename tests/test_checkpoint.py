@@ -17,7 +17,6 @@ import pytest
 
 from helpers import ROOT, chanflow
 
-WORK = os.path.join(ROOT, "_build", "test_checkpoint")
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
 NX, NY, NZ = 8, 5, 6
 
@@ -53,9 +52,8 @@ def _reference_field(nx, ny, nz):
 
 
 @pytest.fixture(scope="module")
-def work():
-    os.makedirs(WORK, exist_ok=True)
-    return WORK
+def work(tmp_path_factory):
+    return str(tmp_path_factory.mktemp("checkpoint"))
 
 
 def test_roundtrip_and_byte_layout(work):

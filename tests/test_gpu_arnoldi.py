@@ -38,3 +38,67 @@ def test_orr_sommerfeld_eigenvalues_match_oracle():
     cr0, ci0 = ref["cr"], ref["ci"]
     assert np.abs(cr - cr0).max() <= 1e-8 and np.abs(ci - ci0).max() <= 1e-8
     ch.close()
+
+
+def test_linearized_stability_workload_64x97x64():
+    """The linearized-stability configuration at its own size and settings:
+    ARPACK (scipy's, as the reference's pdnaupd reverse-communication loop,
+    PFC/arnoldi.f90:244-288, which = 'LM') with Krylov dimension 60, 20
+    eigenvalues and tol 1e-10 over the device matvec
+    (chan_linearized_matvec) of the 64x97x64 channel about the laminar
+    profile at Re = 2800 (box 2 pi x 2 x pi, 96x96 dealiased, tend = 51 steps
+    of dt = 0.00194).
+
+    tests/golden/arnoldi_64x97x64.npz holds the eigenvalues of every Fourier
+    block with kx index <= 6 and |kz index| <= 4, computed with the oracle
+    block by block (the base flow couples no two blocks).  Every converged
+    Ritz value, mapped to (growth rate, frequency) as PFC/arnoldi.f90:805-806
+    does, matches one of them to 1e-8.  The leading eigenvalues come in exact
+    pairs (cos/sin of each x-independent mode; u and w of the mean mode),
+    of which a Krylov space started from one vector sees one copy, so the
+    check is on distinct values: the ten largest distinct ones are all found."""
+    spec = importlib.util.spec_from_file_location("mk_arn", os.path.join(HERE, "golden", "make_arnoldi_fixture.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    from scipy.sparse.linalg import LinearOperator, eigs
+    G = mk.big_grid()
+    ch = make_channel(G)
+    ch.set_baseflow(O.laminar_baseflow(G))
+    dt = O.linearized_dt(G)
+    tend = O.linearized_tend(dt)
+    ref = np.load(os.path.join(HERE, "golden", "arnoldi_64x97x64.npz"))
+    assert ref["tend"] == tend and ref["dt"] == dt
+    ch.set_scalars(dt=dt)
+    shape = (3, G.ny + 1, G.nz, G.nx)
+    n = int(np.prod(shape))
+    calls = [0]
+    T = [0.0]
+
+    def mv(x):
+        calls[0] += 1
+        if calls[0] % 200 == 0:
+            print(f"matvec {calls[0]}", flush=True)
+        assert calls[0] <= 20000, "ARPACK did not converge within 20000 matvecs"
+        w, t = ch.linearized_matvec(x.reshape(shape), tend)
+        T[0] = t
+        return w.ravel()
+    lam = eigs(LinearOperator((n, n), matvec=mv, dtype=np.float64), k=20, ncv=60, which="LM", tol=1e-10,
+               v0=np.random.default_rng(7).standard_normal(n), maxiter=100000, return_eigenvectors=False)
+    ch.close()
+    print(f"{calls[0]} matvecs", flush=True)
+    assert abs(T[0] - ref["t_total"]) <= 1e-15
+    cr, ci = O.eig_map(lam, T[0])
+    fr, fi = ref["cr"], ref["ci"]
+    for a, b in zip(cr, ci):
+        d = np.maximum(np.abs(fr - a), np.abs(fi - b)).min()
+        assert d <= 1e-8, (a, b, d)
+    # distinct leading values of the fixture (pairs agree to ~1e-13)
+    distinct = []
+    for a, b in zip(fr, fi):
+        if all(max(abs(a - x), abs(b - y)) > 1e-9 for x, y in distinct):
+            distinct.append((a, b))
+        if len(distinct) == 10:
+            break
+    for a, b in distinct:
+        d = np.maximum(np.abs(cr - a), np.abs(ci - b)).min()
+        assert d <= 1e-8, (a, b, d)

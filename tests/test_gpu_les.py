@@ -176,3 +176,72 @@ def test_trained_network_gains_over_untrained_and_smagorinsky(g0):
     assert c_trained > abs(c_untrained) + 0.3, (c_trained, c_untrained)
     assert c_trained > c_dyn + 0.05, (c_trained, c_dyn)
     ch.close(); dyn.close(); net.close()
+
+
+def test_les_learned_config4_steps():
+    """The learned-closure LES at its own shape and settings: 64x65x64 (96x96
+    dealiased), box 4 pi x 2 x 4 pi/3, Re_c 3300, dt 0.01, the committed
+    trained network with the dynamic-Smagorinsky gate (beta 1), the stress
+    evaluated once per step (and at the initial pressure), 3 RK3 steps from
+    a seeded turbulent-like field.
+
+    The solver around the closure is FP64 and is held to the oracle exactly:
+    the oracle takes the stress the device evaluated (read back after each
+    evaluation) as a prescribed stress, and its trajectory matches the
+    device's to relative L2 <= 1e-10 with divergence <= 1e-12.  The stress
+    itself is the closure of the oracle's own state: features, the device's
+    bf16 network and the gate of oracle/les_oracle.py reproduce each
+    evaluation to relative L2 <= 1e-3 (bf16 inputs may round differently at
+    a few points where FP64 round-off moves a float32 feature across a bf16
+    step)."""
+    import os
+    cf = chanflow()
+    g = O.make_grid(64, 65, 64, 96, 96, 4.0, 3.0, 3300.0, stretch=2.75, fix_mass=1)
+    s = __import__("helpers").synth()
+    U0 = s.turbulent_like(g.nx, g.ny, g.nz, g.yh, g.y, seed=9, rms=0.1, re_tau=180.0)
+    dt = 0.01
+    pkg = os.path.dirname(cf.__file__)
+    net = cf.Mlp(np.load(os.path.join(pkg, "les_net_64x65x64.npy")))
+    ch = make_channel(g)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=dt, p_grad=-2.0 / g.re)
+    ch.les_configure("learned", net=net, beta=1.0, every_substep=False)
+    ch.initialize_flow()
+    taus = [ch.les_tau()]
+    for k in (1, 2, 3):
+        ch.step(1, step0=k)
+        taus.append(ch.les_tau())
+    d1 = ch.get_data1()
+    div = ch.divergence_max()
+    ch.close()
+
+    orig = O.nonlinear
+    cur = {"tau": taus[0]}
+    states = []
+
+    def nl_prescribed(gg, U, step=1, rk_step=1, dt=0.0):
+        if rk_step == 1:
+            states.append(U.copy())
+        return orig(gg, U, step=step, rk_step=rk_step, dt=dt, tau=cur["tau"])
+
+    try:
+        O.nonlinear = nl_prescribed
+        st = O.initial_state(g, U0, dt=dt)
+        u_init = st.U.copy()
+        for k in (1, 2, 3):
+            cur["tau"] = taus[k]
+            O.rk3_step(g, st, k)
+    finally:
+        O.nonlinear = orig
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
+    assert div <= 1e-12
+    # each evaluation is the closure of the oracle's state at that point
+    for k, U in enumerate([u_init] + states):
+        Ph = L.les_fields(g, U)
+        X, sc = L.features(g, Ph)
+        Y = net.infer(X)
+        cs2 = L.dynamic_coefficient(g, U)
+        tref, _ = L.tau_learned(g, Ph, Y, sc, cs2, beta=1.0)
+        assert rel_l2(taus[k], tref) <= 1e-3, k
+    net.close()

@@ -83,6 +83,41 @@ def test_solvers(g0):
     ch.close()
 
 
+def test_strip_divergence_modes(g0):
+    """The per-mode strip_divergence and subtract_pressure_gradient of the
+    kept initialisation code (PFC/solve.f90:14-77; one (kx,kz) column per
+    reference call, here 48 columns per call, the x- and z-Nyquist columns
+    among them) against the oracle's whole-field routines at the same modes;
+    the mean mode, whose Neumann system is singular and which the reference's
+    callers never pass, is rejected."""
+    cf = chanflow()
+    rng = np.random.default_rng(21)
+    ch = make_channel(g0)
+    nyp = g0.ny + 1
+    cu = rng.standard_normal((3, nyp, g0.nxh, g0.nz)) + 1j * rng.standard_normal((3, nyp, g0.nxh, g0.nz))
+    coef = 0.37
+    ref_u, ref_p = O.strip_divergence(g0, cu, coef)
+    modes = [(0, 1), (0, g0.nz // 2), (g0.nxh - 1, 0), (g0.nxh - 1, g0.nz - 1), (3, 5), (1, g0.nz // 2)]
+    modes += [(int(a), int(b)) for a, b in zip(rng.integers(1, g0.nxh, 42), rng.integers(0, g0.nz, 42))]
+    cols = np.stack([cu[:, :, ix, iz] for ix, iz in modes])
+    p = [ix + 1 for ix, _ in modes]
+    q = [iz + 1 for _, iz in modes]
+    got_u, got_p = ch.strip_divergence_modes(cols, coef, p, q)
+    for k, (ix, iz) in enumerate(modes):
+        assert rel_l2(got_u[k], ref_u[:, :, ix, iz]) <= TOL1, (ix, iz)
+        assert rel_l2(got_p[k], ref_p[:, ix, iz]) <= TOL1, (ix, iz)
+    cpf = rng.standard_normal((nyp, g0.nxh, g0.nz)) + 1j * rng.standard_normal((nyp, g0.nxh, g0.nz))
+    ref_g = O.subtract_pressure_gradient(g0, cu, cpf, coef)
+    got_g = ch.subtract_pressure_gradient_modes(cols, np.stack([cpf[:, ix, iz] for ix, iz in modes]), coef, p, q)
+    for k, (ix, iz) in enumerate(modes):
+        assert rel_l2(got_g[k], ref_g[:, :, ix, iz]) <= TOL1, (ix, iz)
+    with pytest.raises(cf.ChanError, match="mean mode"):
+        ch.strip_divergence_modes(cols[:1], coef, [1], [1])
+    with pytest.raises(cf.ChanError, match="out of range"):
+        ch.strip_divergence_modes(cols[:1], coef, [g0.nxh + 1], [1])
+    ch.close()
+
+
 def test_strip_divergence(g0):
     ch = make_channel(g0)
     U = init_cfg0(O, g0, seed=3, amp=1e-2)
@@ -331,9 +366,9 @@ def test_ten_steps_config0(g0):
 def test_ten_steps_reference_defaults(g0):
     """Config 0 at the reference's own defaults rather than the patched ones:
     grid stretching 2.75 (PFC/setup.f90:545) and the initial dt rule
-    0.25 lx/nx (PFC/setup.f90:1377, dt 0.154 here) held fixed (dt_flag 0);
+    0.25 lx/nx (PFC/setup.f90:1377, dt 0.0491 here) held fixed (dt_flag 0);
     10 RK3 steps, rel L2 <= 1e-10, divergence <= 1e-12."""
-    g = O.make_grid(g0.nx, g0.ny, g0.nz, g0.nx2, g0.nz2, g0.lx, g0.lz, g0.re, stretch=2.75, fix_mass=1)
+    g = O.make_grid(g0.nx, g0.ny, g0.nz, g0.nx2, g0.nz2, 2.0, 2.0, g0.re, stretch=2.75, fix_mass=1)
     dt = O.default_dt(g)
     ch = make_channel(g)
     U0 = init_cfg0(O, g, seed=5)
@@ -503,3 +538,28 @@ def test_headline_shape_one_step(g1):
     assert ch.divergence_max() <= 1e-12
     assert abs(ch.get_scalars()["p_grad"] - st.p_grad) <= 1e-10 * abs(st.p_grad)
     ch.close()
+
+
+def test_headline_shape_ten_steps(g1):
+    """The headline workload's parity criterion: 10 RK3 steps of the
+    192x129x192 channel (PFC/main_mpi.f90:70-87) from the same seeded field
+    as the oracle: relative L2 <= 1e-10 for u, v, w and the mean profile,
+    discrete divergence <= 1e-12 (PFC/pass2.f90:380-401)."""
+    ch = make_channel(g1)
+    U0 = _headline_field(g1)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.005, p_grad=-2.0 / g1.re)
+    ch.initialize_flow()
+    ch.step(10, step0=1)
+    d1 = ch.get_data1()
+    div = ch.divergence_max()
+    ch.close()
+    st = O.initial_state(g1, U0, dt=0.005)
+    for s in range(1, 11):
+        print(f"oracle step {s}/10 at 192x129x192", flush=True)
+        O.rk3_step(g1, st, s)
+    for c in range(3):
+        assert rel_l2(d1[c], st.U[c]) <= 1e-10, c
+    assert rel_l2(d1[0, :, 0, 0].real, st.U[0, :, 0, 0].real) <= 1e-10
+    assert rel_l2(d1, U0) > 1e-4            # the flow moved
+    assert div <= 1e-12

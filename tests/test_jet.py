@@ -73,3 +73,35 @@ def test_gpu_jet_with_feedback_steps():
     # the jet is there: v at the first interior face follows the wall blowing
     assert np.abs(d1[1, 0]).max() > 1e-3
     ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_jet_control_config_64x97x64():
+    """The jet-control run's own settings: 64x97x64 (96x96 dealiased, box
+    2 pi x 2 x pi, stretching 2.75), Re_c = 2800, a steady Gaussian jet of
+    radius 0.1 and peak wall-normal velocity 0.2 at the lower wall, and the
+    wall-shear feedback law of boundary (PFC/read_write.f90:1465-1532) with
+    gain ctrlu = ctrlw = 0.5; 3 RK3 steps at dt 0.002 from a perturbed
+    laminar profile against the oracle: relative L2 <= 1e-10, divergence
+    <= 1e-12."""
+    g = O.make_grid(64, 97, 64, 96, 96, 2.0, 2.0, 2800.0, stretch=2.75, fix_mass=1, ctrlu=0.5, ctrlw=0.5)
+    A, r, xj, zj = 0.2, 0.1, 0.25 * g.lx, 0.5 * g.lz
+    dt = 0.002
+    s = __import__("helpers").synth()
+    U0 = s.laminar_perturbed(g.nx, g.ny, g.nz, g.yh, g.y, seed=23, amp=1e-2, kmax=4.0)
+    ch = make_channel(g)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=dt)
+    ch.jet_configure(0, A, r, xj, zj, 0.0, 0.0)
+    ch.initialize_flow()
+    st = O.initial_state(g, U0, dt=dt)
+    ch.step(3, step0=1)
+    jet = (0, O.jet_profile_spec(g, 0, A, r, xj, zj), 0.0, 0.0)
+    for k in (1, 2, 3):
+        O.rk3_step(g, st, k, jet=jet)
+    d1 = ch.get_data1()
+    assert rel_l2(d1, st.U) < 1e-10
+    assert ch.divergence_max() <= 1e-12
+    vwall = O.small_c2r(g, d1[1, 0][None])[0]     # wall-normal velocity on the lower wall
+    assert 0.19 < vwall.max() <= 0.2              # the jet's peak (less the uniform suction)
+    ch.close()

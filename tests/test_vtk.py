@@ -9,9 +9,8 @@ import struct
 
 import numpy as np
 
-from helpers import ROOT, chanflow
+from helpers import chanflow
 
-WORK = os.path.join(ROOT, "_build", "test_vtk")
 
 
 def _c30(text, a, b, fmt_vals):
@@ -46,14 +45,13 @@ def _expected(nx, ny, nz, lx, lz, yh, phys):
     return out
 
 
-def test_vtk_bytes_match_reference_format():
-    os.makedirs(WORK, exist_ok=True)
+def test_vtk_bytes_match_reference_format(tmp_path):
     cf = chanflow()
     nx, ny, nz = 8, 6, 4
     rng = np.random.default_rng(0)
     phys = rng.standard_normal((3, ny + 1, nz, nx))
     yh = np.linspace(-1, 1, ny + 1)
-    path = os.path.join(WORK, "t_001.vtk")
+    path = os.path.join(tmp_path, "t_001.vtk")
     cf.vtk_write(path, phys, 2.5, 1.25, yh)
     got = open(path, "rb").read()
     assert got == _expected(nx, ny, nz, 2.5, 1.25, yh, phys)
