@@ -60,6 +60,7 @@ ENTRY_POINTS = (
     "chan_les_configure", "chan_les_update", "chan_les_set_tau", "chan_les_get_tau", "chan_les_get_coefficient",
     "chan_les_get_fields", "chan_les_features", "chan_les_gated_points", "chan_jet_configure",
     "chan_strip_divergence_modes", "chan_subtract_pressure_gradient_modes",
+    "chan_les_filter_dns",
 )
 
 _lib = None
@@ -101,6 +102,7 @@ def load_library(path: str = LIBPATH):
         "chan_xz_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_xz_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_compute_divergence_of_fluxes": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_int]),
+        "chan_les_filter_dns": (C.c_int, [vp, vp, _dp]),
         "chan_small_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_small_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_solve_helmholtz": (C.c_int, [vp, C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp,
@@ -282,6 +284,14 @@ class Channel:
         self._les_net = net
         self._chk(self.lib.chan_les_configure(self.h, m, net.h if net is not None else None, float(beta),
                                               1 if every_substep else 0))
+
+    def les_filter_from(self, dns, tau=True):
+        """Set this (coarse) channel's velocity to the filtered velocity of
+        the fine channel `dns`; with tau, also return the exact subgrid stress
+        [6][ny+1][nz2][nx2] of that filter (see chan_les_filter_dns)."""
+        t = np.empty(self._phys_shape(6)) if tau else None
+        self._chk(self.lib.chan_les_filter_dns(self.h, dns.h, _ptr(t)))
+        return t
 
     def les_update(self):
         self._chk(self.lib.chan_les_update(self.h))
@@ -538,9 +548,13 @@ class Channel:
 
 class Ensemble:
     """Independent Channel instances advanced together by one captured graph
-    per timestep (members run concurrently on `nstreams` streams)."""
+    per timestep.  nstreams = 0 (the default): batched launches, every phase
+    of a substage one launch per transform length over all members (members
+    with the default launch shapes, no LES closure, jet or feedback gains);
+    nstreams >= 1: each member's own kernels, members spread over that many
+    concurrent streams."""
 
-    def __init__(self, members, nstreams=8, linearized=False):
+    def __init__(self, members, nstreams=0, linearized=False):
         self.lib = load_library()
         self.members = list(members)
         arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])

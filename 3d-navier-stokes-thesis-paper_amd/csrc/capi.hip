@@ -454,6 +454,15 @@ FFTPlan make_plan(int n, int tw_offset) {
   return p;
 }
 
+// One batched launch: the channels of one transform length (or segment
+// length) and the work list that covers all their workgroups.
+struct BatchLaunch {
+  int len = 0;                 // nz2 (z-transforms), nx2/2 (x-pass) or rows per segment (solve)
+  void* args = nullptr;        // device array of per-channel arguments
+  BatchItem* items = nullptr;  // device work list
+  int nitems = 0;
+};
+
 struct chan_ensemble {
   std::vector<chan_ctx*> m;
   std::vector<hipStream_t> streams;
@@ -464,6 +473,16 @@ struct chan_ensemble {
   hipGraphExec_t exec = nullptr;
   int lin = 0;
   std::string last_error;
+  // batched mode (nstreams == 0): every phase of a substage is one launch per
+  // transform length over all members
+  bool batched = false;
+  std::vector<BatchLaunch> zinv, zfwd, xpass[2], solve[3];  // xpass[1]: with the CFL maxima; solve[rk-1]
+  CflBatchArgs* cfl_args = nullptr;
+  FillBatchArgs* fill_cbc = nullptr;
+  FillBatchArgs* fill_cfl = nullptr;
+  DevScalars** scs = nullptr;
+  size_t cbc_words = 0, cfl_words = 0;
+  std::vector<void*> dmem;     // device tables of the batched launches
 };
 
 namespace {
@@ -473,13 +492,219 @@ void ensemble_release(chan_ensemble* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
+  for (void* p : e->dmem) (void)hipFree(p);
+  e->dmem.clear();
+  e->zinv.clear();
+  e->zfwd.clear();
+  for (auto& v : e->xpass) v.clear();
+  for (auto& v : e->solve) v.clear();
+}
+
+// ---- batched mode ----
+// Why a member cannot join the batched launches ("" if it can).
+std::string batch_ineligible(const chan_ctx* c, bool lin) {
+  const XZGeom& g = c->gd;
+  if (c->les) return "an LES closure is configured";
+  if (c->d_jet) return "a wall jet is configured";
+  if (c->prm.ctrlu != 0.0 || c->prm.ctrlw != 0.0) return "wall feedback gains are set";
+  if (lin && !c->d_base) return "no base flow (chan_set_baseflow)";
+  if (!zt_batchable(g.nz2)) return "nz2 has no compile-time transform plan";
+  if (!xpass_batchable(g.nx2 / 2)) return "nx2/2 has no wavefront x-pass";
+  if (g.nz > 192) return "nz above 192";
+  if (!solve_seg_rows(c->prm.ny)) return "ny+1 rows do not fit the segmented solve";
+  if (3.0 * c->nyp * (double)c->nmodes * 16.0 >= 4.0e9) return "the segmented solve's 32-bit offsets";
+  const Tuning& t = c->tune;
+  if (!(t.xpass_wave == 1 && t.zfwd_v2 == 1 && t.zinv_kx == 8 && t.solve_seg == 1 && t.nl_chunks == 1 &&
+        t.xpass_prefetch == 1))
+    return "its launch shapes are not the defaults";
+  if (c->timing) return "kernel timing is on";
+  return "";
+}
+
+template <class A>
+void* upload_table(chan_ensemble* e, const std::vector<A>& v) {
+  void* d = dalloc(sizeof(A) * std::max<size_t>(v.size(), 1));
+  e->dmem.push_back(d);
+  if (!v.empty()) CHAN_CHECK(hipMemcpy(d, v.data(), sizeof(A) * v.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+// Members grouped by a length; the group's args and work list.  items(i)
+// appends member i's workgroups (bx, by) in launch order; larger members
+// first so that the short ones fill the tail.
+template <class A, class LenF, class ArgF, class ItemF, class WorkF>
+std::vector<BatchLaunch> plan_batches(chan_ensemble* e, LenF len, ArgF arg, ItemF items, WorkF work) {
+  std::map<int, std::vector<int>> by_len;
+  for (int i = 0; i < (int)e->m.size(); ++i) by_len[len(e->m[i])].push_back(i);
+  std::vector<BatchLaunch> out;
+  for (auto& kv : by_len) {
+    std::vector<int> mem = kv.second;
+    std::stable_sort(mem.begin(), mem.end(), [&](int a, int b) { return work(e->m[a]) > work(e->m[b]); });
+    std::vector<A> args;
+    std::vector<BatchItem> it;
+    for (int k = 0; k < (int)mem.size(); ++k) {
+      args.push_back(arg(e->m[mem[k]]));
+      items(e->m[mem[k]], (unsigned short)k, it);
+    }
+    BatchLaunch b;
+    b.len = kv.first;
+    b.args = upload_table(e, args);
+    b.items = (BatchItem*)upload_table(e, it);
+    b.nitems = (int)it.size();
+    out.push_back(b);
+  }
+  return out;
+}
+
+void ensemble_plan_batched(chan_ensemble* e) {
+  const bool lin = e->lin != 0;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    const std::string why = batch_ineligible(e->m[i], lin);
+    if (!why.empty()) throw chan_error("member " + std::to_string(i) + " cannot join the batched launches: " + why);
+  }
+  if (e->m.size() > 65535) throw chan_error("too many members for one batched launch");
+  auto zps = [](chan_ctx* c) { return (long)c->gd.nz2 * c->gd.kxpad; };
+  auto push_grid = [](std::vector<BatchItem>& it, unsigned short k, int gx, int gy) {
+    for (int y = 0; y < gy; ++y)
+      for (int x = 0; x < gx; ++x) it.push_back(BatchItem{k, (unsigned short)x, (unsigned)y});
+  };
+  auto zwork = [](chan_ctx* c) { return (long)c->gd.nz2 * c->nxh * c->nyp; };
+  // z-inverse: d_vel -> d_Z, every plane of every component
+  e->zinv = plan_batches<ZBatchArgs>(
+      e, [](chan_ctx* c) { return c->gd.nz2; },
+      [&](chan_ctx* c) {
+        ZBatchArgs a{};
+        a.g = c->gd; a.tw = c->d_tw; a.in = c->d_vel; a.in_ps = nmodes(c); a.out = c->d_Z; a.out_ps = zps(c);
+        a.pm = PlaneMap{3 * c->nyp, 0, 0};
+        return a;
+      },
+      [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
+        push_grid(it, k, (c->nxh + 7) / 8, 3 * c->nyp);
+      }, zwork);
+  // z-forward: A, B -> d_nl with the NL planes the reference leaves zero
+  e->zfwd = plan_batches<ZBatchArgs>(
+      e, [](chan_ctx* c) { return c->gd.nz2; },
+      [&](chan_ctx* c) {
+        ZBatchArgs a{};
+        a.g = c->gd; a.tw = c->d_tw; a.in = c->d_AB; a.B = c->d_AB + zps(c); a.in_ps = 2 * zps(c);
+        a.kz = c->ops.kz; a.out = c->d_nl; a.out_ps = nmodes(c); a.pm = PlaneMap{3 * c->nyp, 0, 0};
+        a.nl_zero_ny = c->prm.ny;
+        return a;
+      },
+      [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
+        push_grid(it, k, (c->nxh + 7) / 8, 3 * c->nyp);
+      }, zwork);
+  // x-pass: plane ranges from the resident workgroups of the whole batch
+  std::map<int, long> rows;  // per length: z-row groups x planes of all its members
+  for (chan_ctx* c : e->m) rows[c->gd.nx2 / 2] += (long)((c->gd.nz2 + 3) / 4) * (c->prm.ny - 1);
+  std::map<int, int> chunk;
+  for (auto& kv : rows) {
+    const long res = xpass_batched_resident(kv.first, lin);
+    chunk[kv.first] = (int)std::max(2L, (kv.second + res - 1) / res);
+  }
+  for (int w = 0; w < 2; ++w) {
+    e->xpass[w] = plan_batches<XBatchArgs>(
+        e, [](chan_ctx* c) { return c->gd.nx2 / 2; },
+        [&](chan_ctx* c) {
+          XBatchArgs x{};
+          x.g = c->gd; x.tw = c->d_tw; x.ops = c->ops;
+          XPassArgs& a = x.a;
+          a.Z = c->d_Z; a.AB = c->d_AB; a.base = lin ? c->d_base : nullptr;
+          a.cflbits = w ? c->d_cfl : nullptr;
+          a.ny = c->prm.ny; a.ka = 2; a.kb = c->prm.ny;
+          a.chunk = std::min(chunk[c->gd.nx2 / 2], c->prm.ny - 1);
+          a.rz = 0; a.wave = 1; a.maxthr = c->tune.xpass_threads; a.prefetch = 1; a.tau = nullptr;
+          return x;
+        },
+        [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
+          const int np = c->prm.ny - 1, ch = std::min(chunk[c->gd.nx2 / 2], np);
+          push_grid(it, k, (c->gd.nz2 + 3) / 4, (np + ch - 1) / ch);
+        },
+        [](chan_ctx* c) { return (long)c->gd.nz2 * c->gd.nx2 * c->nyp; });
+  }
+  // the segmented solve, one table per substep
+  for (int rk = 1; rk <= 3; ++rk) {
+    e->solve[rk - 1] = plan_batches<SBatchArgs>(
+        e, [](chan_ctx* c) { return solve_seg_rows(c->prm.ny); },
+        [&](chan_ctx* c) {
+          SBatchArgs x{};
+          x.ops = c->ops;
+          SolveArgs& a = x.a;
+          a.nl = c->d_nl; a.vel = c->d_vel; a.R = c->d_R; a.P = c->d_P; a.cbc = c->d_cbc; a.dudy = c->d_dudy;
+          a.sc = c->d_sc; a.scr = c->d_scr; a.rk = rk; a.ny = c->prm.ny; a.nxh = c->nxh; a.nz = c->prm.nz;
+          a.re = c->prm.re; a.cosb = std::cos(c->prm.beta); a.sinb = std::sin(c->prm.beta);
+          a.fix_mass = c->prm.fix_mass;
+          a.lanes = c->tune.solve_lanes; a.bl = c->tune.solve_bl; a.dma = c->tune.solve_dma; a.seg = 1;
+          return x;
+        },
+        [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
+          push_grid(it, k, (int)((nmodes(c) + 3) / 4), 1);
+        },
+        [](chan_ctx* c) { return (long)c->nmodes * c->nyp; });
+  }
+  std::vector<CflBatchArgs> cf;
+  std::vector<FillBatchArgs> fc, fl;
+  std::vector<DevScalars*> sc;
+  for (chan_ctx* c : e->m) {
+    cf.push_back(CflBatchArgs{c->ops, c->prm, c->d_cfl, c->d_sc});
+    const size_t cw = 6 * (size_t)nmodes(c) * sizeof(double2) / 8, lw = 3 * (size_t)c->nyp;
+    fc.push_back(FillBatchArgs{(unsigned long long*)c->d_cbc, cw});
+    fl.push_back(FillBatchArgs{c->d_cfl, lw});
+    e->cbc_words = std::max(e->cbc_words, cw);
+    e->cfl_words = std::max(e->cfl_words, lw);
+    sc.push_back(c->d_sc);
+  }
+  e->cfl_args = (CflBatchArgs*)upload_table(e, cf);
+  e->fill_cbc = (FillBatchArgs*)upload_table(e, fc);
+  e->fill_cfl = (FillBatchArgs*)upload_table(e, fl);
+  e->scs = (DevScalars**)upload_table(e, sc);
+}
+
+// One timestep of every member as batched launches (PFC/main_mpi.f90:73-90
+// for all of them at once): boundary with zero gains (wall data cleared),
+// three substeps of z-inverse, x-pass, z-forward (CFL and dt at the first)
+// and solve, then t_total += dt.
+void ensemble_step_batched(chan_ensemble* e, hipStream_t s) {
+  const int n = (int)e->m.size();
+  const bool lin = e->lin != 0;
+  launch_fill_zero_batched(e->fill_cbc, n, e->cbc_words, s);
+  for (int rk = 1; rk <= 3; ++rk) {
+    if (rk == 1) launch_fill_zero_batched(e->fill_cfl, n, e->cfl_words, s);
+    for (const BatchLaunch& b : e->zinv)
+      launch_zinv_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, s);
+    for (const BatchLaunch& b : e->xpass[rk == 1 ? 1 : 0])
+      launch_xpass_batched(b.len, lin, (const XBatchArgs*)b.args, b.items, b.nitems, s);
+    for (const BatchLaunch& b : e->zfwd)
+      launch_zfwd_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, s);
+    if (rk == 1) launch_cfl_finalize_batched(e->cfl_args, n, 1, s);
+    for (const BatchLaunch& b : e->solve[rk - 1])
+      launch_solve_batched(b.len, (const SBatchArgs*)b.args, b.items, b.nitems, s);
+  }
+  launch_advance_time_batched(e->scs, n, s);
 }
 
 // Capture one timestep of every member: the members are spread round-robin
 // over the ensemble's streams, which fork from and join into the master
-// stream, so independent members run concurrently inside one graph.
+// stream, so independent members run concurrently inside one graph.  In
+// batched mode the step is the batched launches on the master stream.
 void ensemble_capture(chan_ensemble* e) {
   for (chan_ctx* c : e->m) prepare_capture(c);
+  if (e->batched) {
+    if (e->zinv.empty()) ensemble_plan_batched(e);
+    for (chan_ctx* c : e->m) c->data1_is_nl = false;
+    CHAN_CHECK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+    try {
+      ensemble_step_batched(e, e->master);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(e->master, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    CHAN_CHECK(hipStreamEndCapture(e->master, &e->graph));
+    CHAN_CHECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return;
+  }
   CHAN_CHECK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
   std::vector<hipStream_t> saved(e->m.size());
   try {
@@ -551,6 +776,7 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     geo(c->gd, p.nx2, p.nz2, 0);
     geo(c->gs, p.nx, p.nz, 1);
     c->kxpad = c->gd.kxpad;
+    CHAN_CHECK(hipGetDevice(&c->device));
     CHAN_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     c->d_tw = (double2*)dalloc(sizeof(double2) * roots.size());
@@ -1000,6 +1226,16 @@ int chan_compute_divergence_of_fluxes(chan_ctx* c, const double* fx, const doubl
   });
 }
 
+int chan_les_filter_dns(chan_ctx* les, chan_ctx* dns, double* tau) {
+  return guarded(les, [&] {
+    if (!dns) throw chan_error("filter: null fine channel");
+    check_ops(les);
+    check_ops(dns);
+    ensure_vel_in_vel(dns);
+    filter_dns_device(les, dns, tau, dev_c2r, dev_r2c);
+  });
+}
+
 int chan_small_fft_complex_to_real(chan_ctx* c, const double* cdata, double* data, int np) {
   return guarded(c, [&] {
     check_ops(c); check_np(c, np);
@@ -1443,7 +1679,14 @@ int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int line
         if (members[k] == members[i]) throw chan_error("ensemble member given twice");
       e->m.push_back(members[i]);
     }
-    const int ns = std::max(1, std::min(nstreams, n));
+    if (nstreams < 0) throw chan_error("nstreams must be >= 0");
+    e->batched = (nstreams == 0);
+    if (e->batched)
+      for (int i = 0; i < n; ++i) {
+        const std::string why = batch_ineligible(members[i], linearized != 0);
+        if (!why.empty()) throw chan_error("member " + std::to_string(i) + " cannot join the batched launches: " + why);
+      }
+    const int ns = e->batched ? 0 : std::max(1, std::min(nstreams, n));
     CHAN_CHECK(hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
     CHAN_CHECK(hipEventCreateWithFlags(&e->start, hipEventDisableTiming));
     e->streams.resize(ns, nullptr);

@@ -88,6 +88,41 @@ struct XPassArgs {
                           // (11, 12, 13, 22, 23, 33; zero on the wall planes) or nullptr
 };
 
+// Plane of a z-transform workgroup: grid row y covers planes off .. off+cnt-1
+// of each field, fields `stride` planes apart (identity: cnt = nplanes, stride = 0).
+struct PlaneMap {
+  int cnt, stride, off;
+};
+
+// Batched launches (the batched ensemble): workgroup i of a launch runs entry
+// i of a work list, the workgroup (bx, by) of channel m's own grid.
+struct BatchItem {
+  unsigned short m, bx;
+  unsigned int by;
+};
+// Per-channel arguments of a batched z-transform: z-inverse in -> out;
+// z-forward A = in, B -> out (+ i kz B), in_ps the plane stride of A and B.
+struct ZBatchArgs {
+  XZGeom g;
+  const double2* tw;
+  const double2* in;
+  const double2* B;
+  long in_ps;
+  double2* out;
+  long out_ps;
+  const double* kz;
+  PlaneMap pm;
+  int nl_zero_ny;
+};
+
+// Per-channel arguments of a batched wavefront x-pass.
+struct XBatchArgs {
+  XZGeom g;
+  const double2* tw;
+  DevOps ops;
+  XPassArgs a;
+};
+
 // Launch-shape choices of one context (chan_set_tuning, chan_autotune);
 // the defaults can be overridden with CHANFLOW_XPASS_WAVE, CHANFLOW_XPASS_THREADS,
 // CHANFLOW_ZFWD_V2, CHANFLOW_SOLVE_LANES and CHANFLOW_SOLVE_BL.
@@ -118,6 +153,7 @@ struct chan_ctx {
   XZGeom gs;     // small transforms (nx x nz grid)
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  int device = 0;                // the HIP device the context's memory lives on
   std::string last_error;
 
   // twiddle roots
@@ -185,9 +221,16 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
 void launch_zfwd(const XZGeom& g, const double2* tw, const double2* A, const double2* B, long ab_ps,
                  const double* kz, double2* out, long out_ps, int nplanes, int nl_zero_ny,
                  hipStream_t s, int v2 = 1, int kx = 8, int pcnt = 0, int pstride = 0, int poff = 0);
+bool zt_batchable(int nz2);
+void launch_zinv_batched(int nz2, const ZBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s);
+void launch_zfwd_batched(int nz2, const ZBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s);
 // (k_nonlinear.hip)
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
                   hipStream_t s);
+bool xpass_batchable(int M);
+void launch_xpass_batched(int M, bool lin, const XBatchArgs* args, const BatchItem* items, int nitems,
+                          hipStream_t s);
+long xpass_batched_resident(int M, bool lin);
 // (k_solve.hip)
 struct SolveArgs {
   const double2* nl;  // [3][ny+1][nxh][nz]
@@ -207,6 +250,25 @@ struct SolveArgs {
   int seg;            // segmented column solve where the row count allows (Tuning::solve_seg)
 };
 void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s);
+// batched forms (the batched ensemble)
+struct SBatchArgs {
+  DevOps ops;
+  SolveArgs a;
+};
+struct CflBatchArgs {
+  DevOps ops;
+  chan_params prm;
+  unsigned long long* cfl;
+  DevScalars* sc;
+};
+struct FillBatchArgs {
+  unsigned long long* p;
+  size_t n;   // 8-byte words
+};
+void launch_solve_batched(int seg, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s);
+void launch_cfl_finalize_batched(const CflBatchArgs* args, int n, int step, hipStream_t s);
+void launch_fill_zero_batched(const FillBatchArgs* args, int n, size_t max_words, hipStream_t s);
+void launch_advance_time_batched(DevScalars* const* sc, int n, hipStream_t s);
 int solve_seg_rows(int ny);  // rows per segment of the segmented solve, 0 if ny+1 rows do not fit
 void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
                    long nm, hipStream_t s);
@@ -261,6 +323,12 @@ int guarded_les(chan_ctx* c, F&& f) {
 }
 void check_ops_les(chan_ctx* c);        // operators uploaded, else throws
 void les_prepare_state(chan_ctx* c);    // the velocity (not the NL term) is in d_vel
+
+// filtered DNS (k_filter.hip): the coarse velocity and, when tau_host is
+// given, the exact subgrid stress [6][ny_c+1][nz2_c][nx2_c] on the host
+void filter_dns_device(chan_ctx* les, chan_ctx* dns, double* tau_host,
+                       void (*c2r)(chan_ctx*, const XZGeom&, const double2*, int, double*),
+                       void (*r2c)(chan_ctx*, const XZGeom&, const double*, int, double2*));
 
 // subgrid closure (k_les.hip)
 void les_destroy(LesState* s);

@@ -269,9 +269,11 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
 // writes cells base + i) as soon as the current rows have been consumed, so
 // their latency overlaps a whole plane of work instead of stalling the
 // wavefront at the start of every plane.
+// (bx, by): the workgroup's group of four z-rows and its plane range
+// (blockIdx, or a work-list entry of the batched form)
 template <bool LIN, int M, bool PF, bool LES = false>
-__global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
-                                                  XPassArgs a) {
+__device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __restrict__ tw, const DevOps& ops,
+                                             const XPassArgs& a, int bx, int by) {
   extern __shared__ double2 lds[];
   constexpr int SEQ = fft_seq_cells(M), PPL = (M + 63) / 64;
   constexpr int RAWN = ((M + 1 + 63) / 64) * 64;          // cells per raw row (nx/2+1 <= M+1)
@@ -280,14 +282,14 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   double2* rl = lds;
   load_tw2<M>(rl, tw + g.px.tw, tid, 256);
   __syncthreads();
-  const int zz = blockIdx.x * 4 + wv;
+  const int zz = bx * 4 + wv;
   if (zz >= g.nz2) return;  // after the only workgroup barrier
   double2* W = lds + M + wv * WSZ;
   double2* RAW = W + 4 * SEQ;
   const double2* rg = tw + g.px.tw;
   const double2* rootx = tw + g.twx;
   const int ny = a.ny, nyp = ny + 1;
-  const int ka = a.ka + blockIdx.y * a.chunk;
+  const int ka = a.ka + by * a.chunk;
   const int kb = min(a.kb, ka + a.chunk - 1);
   const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
   const long rowoff = (long)zz * kxp;
@@ -619,7 +621,72 @@ __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __rest
   }
 }
 
+template <bool LIN, int M, bool PF, bool LES = false>
+__global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
+                                                  XPassArgs a) {
+  xpass_w_body<LIN, M, PF, LES>(g, tw, ops, a, blockIdx.x, blockIdx.y);
+}
+
+// Batched form: workgroup i runs work-list entry it[i] of channel it[i].m.
+template <bool LIN, int M, bool PF>
+__global__ __launch_bounds__(256) void k_xpass_w_b(const XBatchArgs* __restrict__ args,
+                                                    const BatchItem* __restrict__ it) {
+  const BatchItem w = it[blockIdx.x];
+  const XBatchArgs& x = args[w.m];
+  xpass_w_body<LIN, M, PF, false>(x.g, x.tw, x.ops, x.a, w.bx, w.by);
+}
+
 using XPassFn = void (*)(XZGeom, const double2*, DevOps, XPassArgs);
+
+// The wavefront x-pass of several channels with the same nx2/2 = M in one
+// launch (the batched ensemble); the row prefetch is on wherever the single
+// launch uses it (M <= 144).
+bool xpass_batchable(int M) { return fft2_split(M).n1 > 0 && M <= 192; }
+
+void launch_xpass_batched(int M, bool lin, const XBatchArgs* args, const BatchItem* items, int nitems,
+                          hipStream_t s) {
+  if (nitems <= 0) return;
+  if (!xpass_batchable(M)) throw chan_error("batched x-pass: nx2/2 = " + std::to_string(M) + " has no wavefront kernel");
+  const bool pf = M <= 144;
+  const int rawn = ((M + 1 + 63) / 64) * 64;
+  const size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
+#define XPASSB_CASE(N)                                                                                   \
+  if constexpr (N > 0 && N <= 192) {                                                                     \
+    if constexpr (N <= 144) {                                                                            \
+      if (lin) hipLaunchKernelGGL((k_xpass_w_b<true, N, true>), dim3(nitems), dim3(256), ldw, s, args, items); \
+      else hipLaunchKernelGGL((k_xpass_w_b<false, N, true>), dim3(nitems), dim3(256), ldw, s, args, items);    \
+    } else {                                                                                             \
+      if (lin) hipLaunchKernelGGL((k_xpass_w_b<true, N, false>), dim3(nitems), dim3(256), ldw, s, args, items); \
+      else hipLaunchKernelGGL((k_xpass_w_b<false, N, false>), dim3(nitems), dim3(256), ldw, s, args, items);    \
+    }                                                                                                    \
+  }
+  FFT_DISPATCH(M, XPASSB_CASE)
+#undef XPASSB_CASE
+  LAUNCH_CHECK("k_xpass_w_b");
+}
+
+// Resident workgroups of the batched wavefront x-pass at length M on the
+// current device (the plane-range rule of the single launch, over the whole batch).
+long xpass_batched_resident(int M, bool lin) {
+  const bool pf = M <= 144;
+  const int rawn = ((M + 1 + 63) / 64) * 64;
+  const size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
+  const void* fn = nullptr;
+#define XPASSR_CASE(N)                                                                              \
+  if constexpr (N > 0 && N <= 192) {                                                                \
+    if constexpr (N <= 144) fn = lin ? (const void*)k_xpass_w_b<true, N, true> : (const void*)k_xpass_w_b<false, N, true>; \
+    else fn = lin ? (const void*)k_xpass_w_b<true, N, false> : (const void*)k_xpass_w_b<false, N, false>; \
+  }
+  FFT_DISPATCH(M, XPASSR_CASE)
+#undef XPASSR_CASE
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, ldw) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return (long)per_cu * cus;
+}
 
 // Planes per workgroup of the wavefront x-pass when no chunk is set: the
 // plane-rows of the pass spread evenly over the `res` workgroups resident at

@@ -588,8 +588,10 @@ __device__ __forceinline__ void seg_backsub(double2 (&X)[NC][SEG], const SegSys<
   }
 }
 
+// bx: the wavefront's mode group; gx: the number of groups of the launch
+// (0: no XCD pairing, the batched form)
 template <int MODES, int SEG>
-__global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
+__device__ __forceinline__ void solve_seg_body(const DevOps& ops, const SolveArgs& a, long bx, long gx) {
   constexpr int SEGS = 64 / MODES;
   // NL and P rows of this lane's segment, kept from the right-hand-side pass
   // for the pressure update and the next RK register: [row][comp][lane]
@@ -602,9 +604,9 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
   const int rl = nyp - (ns - 1) * SEG;    // rows of the last one (3..SEG)
   // Mode groups whose rows share 128-byte lines (MODES < 8) go to workgroups
   // b and b+8, which land on the same XCD one after the other.
-  long grp = blockIdx.x;
-  if (MODES < 8) {
-    const long full = (long)(gridDim.x / 16) * 16;
+  long grp = bx;
+  if (MODES < 8 && gx > 0) {
+    const long full = (gx / 16) * 16;
     if (grp < full) {
       const long xcd = grp & 7, k = grp >> 3;
       grp = 16 * (k >> 1) + 2 * xcd + (k & 1);
@@ -963,6 +965,20 @@ __global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
       sto(R, OC(2, j), o2);
     }
   }
+}
+
+template <int MODES, int SEG>
+__global__ __launch_bounds__(64) void k_solve_seg(DevOps ops, SolveArgs a) {
+  solve_seg_body<MODES, SEG>(ops, a, blockIdx.x, gridDim.x);
+}
+
+// Batched form: workgroup i runs mode group it[i].bx of channel it[i].m.
+template <int MODES, int SEG>
+__global__ __launch_bounds__(64) void k_solve_seg_b(const SBatchArgs* __restrict__ args,
+                                                    const BatchItem* __restrict__ it) {
+  const BatchItem w = it[blockIdx.x];
+  const SBatchArgs& x = args[w.m];
+  solve_seg_body<MODES, SEG>(x.ops, x.a, w.bx, 0);
 }
 
 // Rows per segment of the segmented solve for ny+1 rows: the shortest of
@@ -1653,9 +1669,8 @@ void launch_subtract_grad_cols(const DevOps& ops, int ny, double2* crhs, const d
 // ---------------------------------------------------------------------------
 // CFL number and dt adaptation (check_cfl PFC/pass1.f90:202-250, :106-124)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_cfl_finalize(DevOps ops, chan_params prm,
-                                                     const unsigned long long* cfl, DevScalars* sc,
-                                                     int step) {
+__device__ __forceinline__ void cfl_finalize_body(const DevOps& ops, const chan_params& prm,
+                                                  const unsigned long long* cfl, DevScalars* sc, int step) {
   // one wavefront: lane l takes the planes k = 2 + l, 2 + l + 64, ...; the
   // maximum and the abort flag do not depend on the order of the planes
   const int lane = threadIdx.x;
@@ -1690,6 +1705,24 @@ __global__ __launch_bounds__(64) void k_cfl_finalize(DevOps ops, chan_params prm
   }
 }
 
+__global__ __launch_bounds__(64) void k_cfl_finalize(DevOps ops, chan_params prm,
+                                                     const unsigned long long* cfl, DevScalars* sc,
+                                                     int step) {
+  cfl_finalize_body(ops, prm, cfl, sc, step);
+}
+
+// One wavefront per channel: the CFL maxima and dt of every channel of a batch.
+__global__ __launch_bounds__(64) void k_cfl_finalize_b(const CflBatchArgs* __restrict__ args, int step) {
+  const CflBatchArgs& x = args[blockIdx.x];
+  cfl_finalize_body(x.ops, x.prm, x.cfl, x.sc, step);
+}
+
+void launch_cfl_finalize_batched(const CflBatchArgs* args, int n, int step, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cfl_finalize_b, dim3(n), dim3(64), 0, s, args, step);
+  LAUNCH_CHECK("k_cfl_finalize_b");
+}
+
 void launch_cfl_finalize(const DevOps& ops, chan_params prm, unsigned long long* cfl, DevScalars* sc,
                          int step, hipStream_t s) {
   hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(64), 0, s, ops, prm, cfl, sc, step);
@@ -1708,6 +1741,47 @@ void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
   unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, (unsigned long long*)p, n);
   LAUNCH_CHECK("k_fill_zero");
+}
+
+// Zero fill of several buffers in one launch (the batched ensemble): y-row i
+// of the grid clears buffer i (8-byte words).
+__global__ void k_fill_zero_b(const FillBatchArgs* __restrict__ args) {
+  const FillBatchArgs f = args[blockIdx.y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n; i += (size_t)gridDim.x * blockDim.x)
+    f.p[i] = 0ull;
+}
+
+void launch_fill_zero_batched(const FillBatchArgs* args, int n, size_t max_words, hipStream_t s) {
+  if (n <= 0 || max_words == 0) return;
+  const unsigned bx = (unsigned)std::min<size_t>((max_words + 255) / 256, 256);
+  hipLaunchKernelGGL(k_fill_zero_b, dim3(bx, n), dim3(256), 0, s, args);
+  LAUNCH_CHECK("k_fill_zero_b");
+}
+
+// t_total += dt of every channel of a batch
+__global__ void k_advance_time_b(DevScalars* const* __restrict__ sc, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sc[i]->t_total = sc[i]->t_total + sc[i]->dt;
+}
+
+void launch_advance_time_batched(DevScalars* const* sc, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_advance_time_b, dim3((n + 63) / 64), dim3(64), 0, s, sc, n);
+  LAUNCH_CHECK("k_advance_time_b");
+}
+
+// The segmented solve of several channels with the same segment length in one launch.
+void launch_solve_batched(int seg, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return;
+  switch (seg) {
+    case 5: hipLaunchKernelGGL((k_solve_seg_b<4, 5>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 6: hipLaunchKernelGGL((k_solve_seg_b<4, 6>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 7: hipLaunchKernelGGL((k_solve_seg_b<4, 7>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 8: hipLaunchKernelGGL((k_solve_seg_b<4, 8>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 9: hipLaunchKernelGGL((k_solve_seg_b<4, 9>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    default: throw chan_error("batched solve: no segmented solve with " + std::to_string(seg) + "-row segments");
+  }
+  LAUNCH_CHECK("k_solve_seg_b");
 }
 
 // max |ikx u + dh (v_j - v_{j-1}) + ikz w| over non-mean modes, j = 2..ny

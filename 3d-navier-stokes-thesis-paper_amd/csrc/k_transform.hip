@@ -15,14 +15,10 @@
 
 #define ZT_KX 8
 
-// Plane of a workgroup: grid row y covers planes off .. off+cnt-1 of each
-// field, fields `stride` planes apart (identity: cnt = nplanes, stride = 0).
-struct PlaneMap {
-  int cnt, stride, off;
-};
-__device__ __forceinline__ long map_plane(const PlaneMap& pm) {
-  return (long)(blockIdx.y / pm.cnt) * pm.stride + pm.off + blockIdx.y % pm.cnt;
+__device__ __forceinline__ long map_plane(const PlaneMap& pm, unsigned by) {
+  return (long)(by / pm.cnt) * pm.stride + pm.off + by % pm.cnt;
 }
+__device__ __forceinline__ long map_plane(const PlaneMap& pm) { return map_plane(pm, blockIdx.y); }
 
 // Source of padded row element t (0 <= t < nz2) of a spectral column of nz
 // stored kz modes, for a load with a valid address: index, scale and
@@ -66,14 +62,17 @@ __device__ __forceinline__ void tile8_ts(int e, int& t, int& s) {
   s = (r & 3) | ((r >> 3) << 2);
 }
 
+// (bx, by): the workgroup's kx group and plane row (blockIdx, or a work-list
+// entry of the batched form)
 template <int NC>
-__global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
-                                              const double2* __restrict__ in, long in_ps,
-                                              double2* __restrict__ out, long out_ps, PlaneMap pm) {
+__device__ __forceinline__ void zinv_body(const XZGeom& g, const double2* __restrict__ tw,
+                                          const double2* __restrict__ in, long in_ps,
+                                          double2* __restrict__ out, long out_ps, const PlaneMap& pm,
+                                          int bx, unsigned by) {
   extern __shared__ double2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int kx0 = blockIdx.x * ZT_KX;
-  const long P = map_plane(pm);
+  const int kx0 = bx * ZT_KX;
+  const long P = map_plane(pm, by);
   const int nz = g.nz, nz2 = NC > 0 ? NC : g.nz2, ss = fft_seq_cells(nz2);
   double2* rl = lds + ZT_KX * ss;
   if constexpr (NC == 0) load_tw2<NC>(rl, tw + g.pz.tw, tid, nthr);
@@ -130,6 +129,22 @@ __global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restric
     int t = e / ZT_KX, s = e - t * ZT_KX;
     if (s < ncol) dst[(long)t * g.kxpad + kx0 + s] = lds[s * ss + fft_out_pos<NC>(t)];
   }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_zinv(XZGeom g, const double2* __restrict__ tw,
+                                              const double2* __restrict__ in, long in_ps,
+                                              double2* __restrict__ out, long out_ps, PlaneMap pm) {
+  zinv_body<NC>(g, tw, in, in_ps, out, out_ps, pm, blockIdx.x, blockIdx.y);
+}
+
+// Batched form: workgroup i runs work-list entry it[i] of member it[i].m.
+template <int NC>
+__global__ __launch_bounds__(256) void k_zinv_b(const ZBatchArgs* __restrict__ args,
+                                                const BatchItem* __restrict__ it) {
+  const BatchItem w = it[blockIdx.x];
+  const ZBatchArgs& a = args[w.m];
+  zinv_body<NC>(a.g, a.tw, a.in, a.in_ps, a.out, a.out_ps, a.pm, w.bx, w.by);
 }
 
 // Hermitian half-length packing for an unnormalised c2r of length nx2 = 2M:
@@ -363,19 +378,19 @@ __global__ __launch_bounds__(32 * KX) void k_zinv2(XZGeom g, const double2* __re
 }
 
 template <int NC, int KX>
-__global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __restrict__ tw,
-                                               const double2* __restrict__ A,
-                                               const double2* __restrict__ B, long ab_ps,
-                                               const double* __restrict__ kzv,
-                                               double2* __restrict__ out, long out_ps,
-                                               int nl_zero_ny, PlaneMap pm) {
+__device__ __forceinline__ void zfwd2_body(const XZGeom& g, const double2* __restrict__ tw,
+                                           const double2* __restrict__ A,
+                                           const double2* __restrict__ B, long ab_ps,
+                                           const double* __restrict__ kzv,
+                                           double2* __restrict__ out, long out_ps,
+                                           int nl_zero_ny, const PlaneMap& pm, int bx, unsigned by) {
   extern __shared__ double2 lds[];
   constexpr int NZ2 = NC, SS = fft_seq_cells(NC);
   constexpr int NT = 32 * KX;
   constexpr int NLD = (NZ2 * KX + NT - 1) / NT;       // loads per thread per field
   const int tid = threadIdx.x;
-  const int kx0 = blockIdx.x * KX;
-  const long P = map_plane(pm);
+  const int kx0 = bx * KX;
+  const long P = map_plane(pm, by);
   const int nz = g.nz;
   const int ncol = min(KX, g.nxh - kx0);
   double2* dst = out + P * out_ps;
@@ -461,6 +476,24 @@ __global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __re
   }
 }
 
+template <int NC, int KX>
+__global__ __launch_bounds__(32 * KX) void k_zfwd2(XZGeom g, const double2* __restrict__ tw,
+                                               const double2* __restrict__ A,
+                                               const double2* __restrict__ B, long ab_ps,
+                                               const double* __restrict__ kzv,
+                                               double2* __restrict__ out, long out_ps,
+                                               int nl_zero_ny, PlaneMap pm) {
+  zfwd2_body<NC, KX>(g, tw, A, B, ab_ps, kzv, out, out_ps, nl_zero_ny, pm, blockIdx.x, blockIdx.y);
+}
+
+template <int NC, int KX>
+__global__ __launch_bounds__(32 * KX) void k_zfwd2_b(const ZBatchArgs* __restrict__ args,
+                                                 const BatchItem* __restrict__ it) {
+  const BatchItem w = it[blockIdx.x];
+  const ZBatchArgs& a = args[w.m];
+  zfwd2_body<NC, KX>(a.g, a.tw, a.in, a.B, a.in_ps, a.kz, a.out, a.out_ps, a.nl_zero_ny, a.pm, w.bx, w.by);
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -499,6 +532,33 @@ void launch_zinv(const XZGeom& g, const double2* tw, const double2* in, long in_
   FFT_DISPATCH(g.nz2, ZINV_CASE)
 #undef ZINV_CASE
   LAUNCH_CHECK("k_zinv");
+}
+
+// Batched z-transforms: one launch runs the work lists of several channels
+// of the same transform length (the batched ensemble).  Only the
+// compile-time two-pass lengths are batched.
+bool zt_batchable(int nz2) { return fft2_split(nz2).n1 > 0; }
+
+void launch_zinv_batched(int nz2, const ZBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return;
+  if (!zt_batchable(nz2)) throw chan_error("batched z-inverse: length " + std::to_string(nz2) + " has no compile-time plan");
+  const size_t lds = sizeof(double2) * ZT_KX * fft_seq_cells(nz2);
+#define ZINVB_CASE(N) \
+  if constexpr (N > 0) hipLaunchKernelGGL(k_zinv_b<N>, dim3(nitems), dim3(256), lds, s, args, items)
+  FFT_DISPATCH(nz2, ZINVB_CASE)
+#undef ZINVB_CASE
+  LAUNCH_CHECK("k_zinv_b");
+}
+
+void launch_zfwd_batched(int nz2, const ZBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return;
+  if (!zt_batchable(nz2)) throw chan_error("batched z-forward: length " + std::to_string(nz2) + " has no compile-time plan");
+  const size_t lds = sizeof(double2) * (8 * fft_seq_cells(nz2) + nz2);
+#define ZFWDB_CASE(N) \
+  if constexpr (N > 0) hipLaunchKernelGGL((k_zfwd2_b<N, 8>), dim3(nitems), dim3(256), lds, s, args, items)
+  FFT_DISPATCH(nz2, ZFWDB_CASE)
+#undef ZFWDB_CASE
+  LAUNCH_CHECK("k_zfwd2_b");
 }
 
 static int rows_for(int M, int nz2, int nf) {

@@ -4,38 +4,37 @@ reference, which has no working LES path).
     python 3d-navier-stokes-thesis-paper_amd/les_eval.py [--out results.json]
 
 1. DNS: the 192 x 129 x 192 turbulent channel (Re_c 3300, box 4pi x 2 x 4pi/3,
-   stretch 2.75, dt 0.005) from the seeded synthetic field, run `--spinup`
-   steps, then snapshots every `--every` steps.
+   stretch 2.75, dt 0.005) from the seeded synthetic field, spun up in windows
+   of `--window` time units until the window means of the wall shear of two
+   consecutive windows agree to `--stationary-tol` (statistically stationary),
+   then `--nsnap` snapshots every `--every` steps.
 2. Filter to the LES grid 64 x 65 x 64 (3/2 dealiased to 96 x 96, same box and
-   stretching): a box filter over 3 DNS points in x and z (the 3-point
-   average, transfer (1 + 2 cos(k dx))/3), a cut-off to the LES modes (the LES
-   Nyquist modes dropped) and, in y, the finite-volume restriction of the
-   staggered grid:
-   the tanh grid of 65 faces is every other face of the 129-face grid, so u
-   and w are averaged over the two fine cells of a coarse cell (weights
-   y(k)-y(k-1)) and v is taken at the shared faces.  The restriction keeps the
-   discrete divergence exactly zero.
-3. Exact subgrid stress at the LES cell centres:
-   tau_ij = F(u_i u_j) - F(u_i) F(u_j), with F the same filter (cut-off, then
-   cell averages in y) applied to fields at the DNS cell centres (v averaged
-   to them from its faces).
-4. Training set: (network inputs, tau / (D^2 |du/dx|^2)) at every interior LES
-   point of 80 % of the snapshots; the network is trained on the GPU (float32
-   Adam) on targets standardised per component, and the standard deviations
-   are folded into its last layer.
+   stretching) on the device (chan_les_filter_dns): a box filter over 3 DNS
+   points in x and z, a cut-off to the LES modes (the LES Nyquist modes
+   dropped) and, in y, the finite-volume restriction of the staggered grid
+   (the 65 LES faces are every other DNS face: u and w averaged over the two
+   fine cells of a coarse cell, v taken at the shared faces), which keeps the
+   discrete divergence zero.
+3. Exact subgrid stress at the LES cell centres, also on the device:
+   tau_ij = F(u_i u_j) - F(u_i) F(u_j), products of the DNS cell-centre
+   velocities on the DNS dealiased grid.
+4. Training set: (network inputs, tau / (D^2 |du/dx|^2)) at a random
+   `--points` fraction of the interior LES points of 80 % of the snapshots;
+   the network is trained on the GPU (float32 Adam) on targets standardised
+   per component, and the standard deviations are folded into its last layer.
 5. A priori (the held-out 20 %): correlation and relative L2 error of the
    deviatoric stress and of the plane-averaged subgrid dissipation
    <-tau:S>(y), for the learned closure (with its gate), dynamic Smagorinsky
-   and no model.  The learned closure is kept only when its a-priori error is
-   below the threshold (default: dynamic Smagorinsky's own error); otherwise
-   dynamic Smagorinsky is the closure of the a-posteriori run.
-6. A posteriori: LES runs from the filtered last snapshot (no model, dynamic
-   Smagorinsky, learned) against the DNS continued over the same time, both
-   sampled every `--sample` time units: relative L2 errors of the
-   time-averaged mean profile, the rms profiles and the Reynolds shear stress
-   <u'v'> of the filtered DNS, and the speed-up in simulated time per wall
-   second.  Both runs step through a captured one-member ensemble (graph
-   replay); the LES evaluates its closure once per time step.
+   and no model.
+6. A posteriori (`aposteriori`, also run by bench.py): LES runs from the
+   filtered last snapshot (no model, dynamic Smagorinsky, learned) against the
+   DNS continued over the same time, both sampled every `--sample` time units
+   and filtered on the device: relative L2 errors of the time-averaged mean
+   profile, the rms profiles and the Reynolds shear stress <u'v'>, and the
+   speed-up in simulated time per wall second, at each LES time step of
+   `--les-dts` (the first, 0.01, is the configuration's).  The DNS steps as a
+   one-member batched graph, the LES through a captured graph of its own
+   kernels with the closure evaluated once per time step.
 """
 import argparse
 import importlib.util
@@ -63,7 +62,7 @@ def load_pkg():
 
 PI = math.atan2(0.0, -1.0)
 DNS = dict(nx=192, ny=129, nz=192, nx2=288, nz2=288, dt=0.005)
-LES = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.03)
+LES = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.01)
 BOX = dict(lx=4.0 * PI, lz=4.0 * PI / 3.0, re=3300.0, stretch=2.75)
 SYM = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
 
@@ -73,107 +72,6 @@ def channel(cf, c):
                     fix_mass=1, dt_flag=0)
     y, yh = ch.build_operators(BOX["stretch"])
     return ch, y, yh
-
-
-def kz_map(nz_from, nz_to):
-    """Indices into the fine kz axis of the coarse stored modes, and a mask
-    that drops the coarse Nyquist."""
-    iz = np.arange(nz_to)
-    kzi = np.where(iz <= nz_to // 2, iz, iz - nz_to)
-    src = np.where(kzi >= 0, kzi, nz_from + kzi)
-    keep = np.abs(kzi) < nz_to // 2
-    return src, keep
-
-
-def box3(C):
-    """3-point box filter in x and z of spectral fields [..., nxh, nz]:
-    transfer (1 + 2 cos(k dx)) / 3 per direction on the grid of the fields."""
-    nxh, nz = C.shape[-2], C.shape[-1]
-    nx = 2 * (nxh - 1)
-    ix = np.arange(nxh)
-    iz = np.arange(nz)
-    kzi = np.where(iz <= nz // 2, iz, iz - nz)
-    tx = (1.0 + 2.0 * np.cos(2.0 * PI * ix / nx)) / 3.0
-    tz = (1.0 + 2.0 * np.cos(2.0 * PI * kzi / nz)) / 3.0
-    return C * (tx[:, None] * tz[None, :])
-
-
-def cutoff(C, nxh_to, nz_to, box=True):
-    """(Box filter, then) cut-off [..., nxh_from, nz_from] -> [..., nxh_to, nz_to]
-    without the coarse Nyquist modes."""
-    if box:
-        C = box3(C)
-    src, keep = kz_map(C.shape[-1], nz_to)
-    out = C[..., :nxh_to, :][..., src] * keep
-    out[..., nxh_to - 1, :] = 0.0
-    return out
-
-
-def restrict_cc(F, hh_f, ny_to):
-    """Cell averages of fine cell-centre planes [..., ny_f+1, ...] onto the
-    coarse cells 1..ny_to-1 (coarse cell K = fine cells 2K-1, 2K); the wall
-    planes are copied."""
-    out = np.zeros(F.shape[:-3] + (ny_to + 1,) + F.shape[-2:], dtype=F.dtype)
-    K = np.arange(1, ny_to)
-    a, b = hh_f[2 * K - 1], hh_f[2 * K]
-    w = (a + b)
-    out[..., K, :, :] = (a[:, None, None] * F[..., 2 * K - 1, :, :] + b[:, None, None] * F[..., 2 * K, :, :]) / \
-        w[:, None, None]
-    out[..., 0, :, :] = F[..., 0, :, :]
-    out[..., ny_to, :, :] = F[..., -1, :, :]
-    return out
-
-
-def filter_velocity(d1, y_f, ny_to, nxh_to, nz_to):
-    """LES velocity data1 from the DNS data1."""
-    ny_f = len(y_f)
-    hh = np.empty(ny_f + 1)
-    hh[1:ny_f] = y_f[1:] - y_f[:-1]
-    hh[0] = hh[ny_f] = 0.0
-    Cc = cutoff(d1, nxh_to, nz_to)
-    U = np.zeros((3, ny_to + 1, nxh_to, nz_to), np.complex128)
-    U[0] = restrict_cc(Cc[0], hh, ny_to)
-    U[2] = restrict_cc(Cc[2], hh, ny_to)
-    U[1, :ny_to] = Cc[1, 0:2 * ny_to:2]
-    U[1, ny_to] = 0.0
-    return U, hh
-
-
-def exact_tau(dns, les, d1, y_f, hh_f):
-    """F(u_i u_j) - u^_i u^_j at the LES points [6][ny_l+1][nz2][nx2] and the
-    LES closure inputs (from the LES context, whose state must be the
-    filtered field)."""
-    ny_f, ny_l = dns.ny, les.ny
-    vc = np.zeros_like(d1[1])
-    vc[1:ny_f] = 0.5 * (d1[1, 0:ny_f - 1] + d1[1, 1:ny_f])
-    u = dns.xz_c2r(d1[0])
-    v = dns.xz_c2r(vc)
-    w = dns.xz_c2r(d1[2])
-    comps = (u, v, w)
-    FP = np.zeros((6, ny_l + 1, les.nxh, les.nz), np.complex128)
-    for t, (i, j) in enumerate(SYM):
-        P = comps[i] * comps[j]
-        P[0] = 0.0
-        P[ny_f] = 0.0
-        S = dns.xz_r2c(P)
-        FP[t] = restrict_cc(cutoff(S, les.nxh, les.nz), hh_f, ny_l)
-        del P, S
-    FP[:, 0] = 0.0
-    FP[:, ny_l] = 0.0
-    Fph = np.concatenate([les.xz_c2r(FP[t]) for t in range(6)]).reshape(6, ny_l + 1, les.nz2, les.nx2)
-    ub = []
-    for spec in (d1[0], vc, d1[2]):
-        F = restrict_cc(cutoff(spec, les.nxh, les.nz), hh_f, ny_l)
-        F[0] = 0.0
-        F[ny_l] = 0.0
-        ub.append(les.xz_c2r(F))
-    Ph = les.les_fields()
-    tau = np.zeros_like(Fph)
-    for t, (i, j) in enumerate(SYM):
-        tau[t] = Fph[t] - ub[i] * ub[j]
-    tau[:, 0] = 0.0
-    tau[:, ny_l] = 0.0
-    return tau, Ph
 
 
 def strain(Ph):
@@ -259,15 +157,106 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def spinup(dns, dens, max_time, window, tol, dt, log=print):
+    """Run the DNS in windows of `window` time units until the window means
+    of the wall shear (drag = d_um_dy(1) - d_um_dy(ny), sampled every 0.5
+    time units) of two consecutive windows differ by less than `tol`
+    (relative), or `max_time` is reached.  Returns (time run, window means,
+    whether the criterion was met)."""
+    per = int(round(0.5 / dt))
+    nper = int(round(window / 0.5))
+    means, t = [], 0.0
+    while t < max_time - 1e-9:
+        acc = []
+        for _ in range(nper):
+            dens.step(per)
+            acc.append(dns.l2_norm()[1])
+        t += window
+        means.append(float(np.mean(acc)))
+        log(f"spin-up t = {t:.0f}: mean wall shear {means[-1]:.6f}")
+        if len(means) >= 2 and abs(means[-1] - means[-2]) <= tol * abs(means[-1]):
+            return t, means, True
+    return t, means, False
+
+
+def aposteriori(cf, dns, dens, les, net, Ul0, horizon, sample, dt_dns, les_dt, beta, modes=("off", "dynamic", "learned"),
+                log=print):
+    """LES runs from the filtered field Ul0 (no model, dynamic Smagorinsky,
+    learned) against the DNS continued over `horizon` time units, both
+    sampled every `sample` time units and filtered on the device: relative L2
+    errors of the time-averaged mean profile, rms profiles and <u'v'> over the
+    interior planes, and the simulated time per wall second of each run."""
+    nsamp = int(round(horizon / sample))
+    k_dns = int(round(sample / dt_dns))
+    acc_d, t_dns = [], 0.0
+    for _ in range(nsamp):
+        dns.sync()
+        t0 = time.perf_counter()
+        dens.step(k_dns)
+        dns.sync()
+        t_dns += time.perf_counter() - t0
+        les.les_configure("off")
+        les.les_filter_from(dns, tau=False)
+        acc_d.append(profiles(les.get_data1()))
+    mean_d, rms_d, uv_d = [np.mean([x[j] for x in acc_d], axis=0) for j in range(3)]
+    ny = les.ny
+    out = {"time_units": nsamp * sample, "dns_seconds": t_dns, "dns_time_units_per_s": nsamp * sample / t_dns}
+    for mode in modes:
+        k_les = int(round(sample / les_dt))
+        les.les_configure(mode, net=net if mode == "learned" else None, beta=beta, every_substep=False)
+        les.set_state(data1=Ul0)
+        les.set_scalars(dt=les_dt, t_total=0.0, p_grad=-2.0 / BOX["re"])
+        les.initialize_flow()
+        lens = cf.Ensemble([les], nstreams=1)   # the closure's kernels are the member's own
+        try:
+            lens.step(1)      # the graph is captured and replayed once before the timing
+            les.sync()
+            acc_l, t_les = [], 0.0
+            for i in range(nsamp):
+                t0 = time.perf_counter()
+                lens.step(k_les if i else k_les - 1)
+                les.sync()
+                t_les += time.perf_counter() - t0
+                acc_l.append(profiles(les.get_data1()))
+        except cf.ChanError as e:  # e.g. the reference's CFL abort at too large a time step
+            out[mode] = {"dt": les_dt, "failed": str(e)}
+            lens.close()
+            continue
+        lens.close()
+        mean_l, rms_l, uv_l = [np.mean([x[j] for x in acc_l], axis=0) for j in range(3)]
+        k = slice(1, ny)
+        ent = {"dt": les_dt,
+               "rel_err_mean_profile": rel(mean_l[k], mean_d[k]),
+               "rel_err_rms_u": rel(rms_l[0, k], rms_d[0, k]),
+               "rel_err_rms_v": rel(rms_l[1, k], rms_d[1, k]),
+               "rel_err_rms_w": rel(rms_l[2, k], rms_d[2, k]),
+               "rel_err_uv": rel(uv_l[k], uv_d[k]),
+               "les_seconds": t_les,
+               "les_time_units_per_s": (nsamp * sample - les_dt) / t_les}
+        ent["reynolds_stress_max"] = max(ent["rel_err_rms_u"], ent["rel_err_rms_v"], ent["rel_err_rms_w"],
+                                         ent["rel_err_uv"])
+        ent["speedup_vs_dns"] = ent["les_time_units_per_s"] / out["dns_time_units_per_s"]
+        if mode == "learned":
+            g, n = les.les_gated_points()
+            ent["fallback_fraction_last"] = g / n
+        out[mode] = ent
+        log(f"a posteriori dt={les_dt} {mode}: {json.dumps(ent)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--spinup", type=int, default=4000)
-    ap.add_argument("--every", type=int, default=100)
-    ap.add_argument("--nsnap", type=int, default=10, help="snapshots; 80 %% train, 20 %% test")
-    ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--spinup-max", type=float, default=200.0, help="longest spin-up (time units)")
+    ap.add_argument("--window", type=float, default=10.0, help="spin-up window (time units)")
+    ap.add_argument("--stationary-tol", type=float, default=0.02,
+                    help="relative change of the window-mean wall shear taken as stationary")
+    ap.add_argument("--every", type=int, default=100, help="DNS steps between snapshots")
+    ap.add_argument("--nsnap", type=int, default=200, help="snapshots; 80 %% train, 20 %% test")
+    ap.add_argument("--points", type=float, default=0.25, help="fraction of each training snapshot's points")
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--post-time", type=float, default=20.0, help="a-posteriori horizon (time units)")
+    ap.add_argument("--post-time", type=float, default=40.0, help="a-posteriori horizon (time units)")
     ap.add_argument("--sample", type=float, default=1.0, help="a-posteriori sampling interval (time units)")
     ap.add_argument("--beta", type=float, default=1.0)
     ap.add_argument("--threshold", type=float, default=-1.0,
@@ -275,14 +264,14 @@ def main():
                          "(default: dynamic Smagorinsky's own a-priori error)")
     ap.add_argument("--out", default="les_eval.json")
     ap.add_argument("--params-out", default="")
-    ap.add_argument("--les-dts", default="",
-                    help="comma-separated LES time steps for the a-posteriori runs (default: the LES dt); "
-                         "the first is the reported one, all are kept under dt_sweep")
+    ap.add_argument("--les-dts", default="0.01,0.03",
+                    help="comma-separated LES time steps for the a-posteriori runs; the first is the reported one")
     args = ap.parse_args()
 
     cf = load_pkg()
     import chanflow.synth as synth
-    res = {"dns": DNS, "les": LES, "box": BOX, "filter": "3-point box in x and z, cut-off, y cell average"}
+    res = {"dns": DNS, "les": LES, "box": BOX, "filter": "3-point box in x and z, cut-off, y cell average "
+                                                         "(on the device, chan_les_filter_dns)"}
 
     def log(msg):
         print(f"[les_eval] {msg}", flush=True)
@@ -294,44 +283,48 @@ def main():
     dns.set_state(data1=U0)
     dns.set_scalars(dt=DNS["dt"], t_total=0.0, p_grad=-2.0 / BOX["re"])
     dns.initialize_flow()
-    dens = cf.Ensemble([dns], nstreams=1)
+    dens = cf.Ensemble([dns])
     t0 = time.perf_counter()
-    dens.step(args.spinup)
-    dns.sync()
-    res["dns_spinup_s"] = time.perf_counter() - t0
-    log(f"DNS spin-up {args.spinup} steps in {res['dns_spinup_s']:.1f} s")
+    t_spin, means, ok = spinup(dns, dens, args.spinup_max, args.window, args.stationary_tol, DNS["dt"], log)
+    res["spinup"] = {"time_units": t_spin, "window": args.window, "window_mean_wall_shear": means,
+                     "stationary": ok, "tol": args.stationary_tol, "seconds": time.perf_counter() - t0}
+    log(f"spin-up {t_spin:.0f} time units, stationary {ok}")
 
     # ---- snapshots: inputs / targets; the last 20 % are held out
     ntest = max(1, int(round(0.2 * args.nsnap)))
     ntrain = args.nsnap - ntest
+    rng = np.random.default_rng(1)
     Xs, Ts, held = [], [], []
     placeholder = cf.Mlp()
-    les.les_configure("learned", net=placeholder, beta=args.beta)
+    les.les_configure("learned", net=placeholder, beta=args.beta)   # the network inputs of the snapshots
+    t0 = time.perf_counter()
     for s_ in range(args.nsnap):
         if s_:
             dens.step(args.every)
-        d1 = dns.get_data1()
-        Ul, hh_f = filter_velocity(d1, y_f, LES["ny"], les.nxh, les.nz)
-        les.set_state(data1=Ul)
-        tau_x, Ph = exact_tau(dns, les, d1, y_f, hh_f)
+        tau_x = les.les_filter_from(dns, tau=True)
+        Ph = les.les_fields()
         X, sc = les.les_features()
         tx = tau_x[:, 1:LES["ny"]].reshape(6, -1).T
         ok = sc > 0
         T = np.zeros_like(tx)
         T[ok] = tx[ok] / sc[ok, None]
         if s_ < ntrain:
-            Xs.append(X[ok])
-            Ts.append(T[ok].astype(np.float32))
+            pick = ok & (rng.random(ok.shape[0]) < args.points)
+            Xs.append(X[pick])
+            Ts.append(T[pick].astype(np.float32))
         else:
-            held.append((Ul, tau_x, Ph))
-        log(f"snapshot {s_}: {X.shape[0]} points, |tau| rms {np.sqrt((tau_x ** 2).mean()):.3e}")
+            held.append((les.get_data1(), tau_x, Ph))
+        if s_ % 20 == 0:
+            log(f"snapshot {s_}: |tau| rms {np.sqrt((tau_x ** 2).mean()):.3e}")
+    res["snapshots_seconds"] = time.perf_counter() - t0
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
     del Xs, Ts
     net, losses, t_train, sd = train_net(cf, X, T, args.epochs, args.batch, args.lr)
-    res["training"] = {"snapshots": ntrain, "points": int(X.shape[0]), "epochs": args.epochs, "batch": args.batch,
-                       "lr": args.lr, "loss_per_epoch_standardised": [float(v) for v in losses],
-                       "seconds": t_train, "target_std": [float(v) for v in sd]}
+    res["training"] = {"snapshots": ntrain, "snapshot_every_steps": args.every, "points": int(X.shape[0]),
+                       "point_fraction": args.points, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+                       "loss_per_epoch_standardised": [float(v) for v in losses], "seconds": t_train,
+                       "target_std": [float(v) for v in sd]}
     log(f"trained on {X.shape[0]} points in {t_train:.1f} s, losses {losses}")
     del X, T
     if args.params_out:
@@ -356,92 +349,17 @@ def main():
     pri["learned_fallback_fraction"] = float(np.mean(fb))
     pri["test_snapshots"] = len(held)
     thr = args.threshold if args.threshold >= 0 else pri["dynamic_smagorinsky"]["rel_l2_deviatoric"]
-    choice = "learned" if pri["learned"]["rel_l2_deviatoric"] <= thr else "dynamic"
     pri["threshold"] = thr
-    pri["closure_choice"] = choice
+    pri["closure_choice"] = "learned" if pri["learned"]["rel_l2_deviatoric"] <= thr else "dynamic"
     res["apriori"] = pri
     log(f"a priori: {json.dumps(pri)}")
 
-    # ---- a posteriori: LES runs against the continued DNS (graph replay for both)
-    nsamp = int(round(args.post_time / args.sample))
-    k_dns = int(round(args.sample / DNS["dt"]))
+    # ---- a posteriori: LES runs from the last held-out field against the continued DNS
     Ul0 = held[-1][0]
-    dns.sync()
-    acc_d = []
-    t_dns = 0.0
-    for i in range(nsamp):
-        t0 = time.perf_counter()
-        dens.step(k_dns)
-        dns.sync()
-        t_dns += time.perf_counter() - t0
-        Uf, _ = filter_velocity(dns.get_data1(), y_f, LES["ny"], les.nxh, les.nz)
-        acc_d.append(profiles(Uf))
-    mean_d, rms_d, uv_d = [np.mean([a[j] for a in acc_d], axis=0) for j in range(3)]
-    post = {"time_units": nsamp * args.sample, "dns_seconds": t_dns,
-            "dns_time_units_per_s": nsamp * args.sample / t_dns, "dt_sweep": {}}
-    dts = [float(v) for v in args.les_dts.split(",")] if args.les_dts else [LES["dt"]]
-    probe, _, _ = channel(cf, LES)  # CFL of a state: one nonlinear evaluation on a copy
-    def run_les(mode, dt_l):
-        """One a-posteriori LES run; profiles sampled every `sample` time units."""
-        k_les = int(round(args.sample / dt_l))
-        les.les_configure(mode, net=net if mode == "learned" else None, beta=args.beta, every_substep=False)
-        les.set_state(data1=Ul0)
-        les.set_scalars(dt=dt_l, t_total=0.0, p_grad=-2.0 / BOX["re"])
-        les.initialize_flow()
-        lens = cf.Ensemble([les], nstreams=1)
-        try:
-            lens.step(1)
-            les.sync()
-            acc_l = []
-            t_les = 0.0
-            cfl_max = 0.0
-            for i in range(nsamp):
-                t0 = time.perf_counter()
-                lens.step(k_les if i else k_les - 1)
-                les.sync()
-                t_les += time.perf_counter() - t0
-                d1 = les.get_data1()
-                acc_l.append(profiles(d1))
-                probe.set_state(data1=d1)
-                try:
-                    _, cfl = probe.nonlinear(1, 1, dt=dt_l)
-                except cf.ChanError:
-                    cfl = float("inf")
-                cfl_max = max(cfl_max, cfl) if np.isfinite(cfl) else float("inf")
-        finally:
-            lens.close()
-        mean_l, rms_l, uv_l = [np.mean([a[j] for a in acc_l], axis=0) for j in range(3)]
-        k = slice(1, LES["ny"])
-        ent = {
-            "dt": dt_l,
-            "cfl_max_sampled": cfl_max,
-            "rel_err_mean_profile": rel(mean_l[k], mean_d[k]),
-            "rel_err_rms_u": rel(rms_l[0, k], rms_d[0, k]),
-            "rel_err_rms_v": rel(rms_l[1, k], rms_d[1, k]),
-            "rel_err_rms_w": rel(rms_l[2, k], rms_d[2, k]),
-            "rel_err_uv": rel(uv_l[k], uv_d[k]),
-            "les_seconds": t_les,
-            "les_time_units_per_s": (nsamp * args.sample - dt_l) / t_les,
-        }
-        ent["speedup_vs_dns"] = ent["les_time_units_per_s"] / post["dns_time_units_per_s"]
-        if mode == "learned":
-            g, n = les.les_gated_points()
-            ent["fallback_fraction_last"] = g / n
-        return ent
-
-    for di, dt_l in enumerate(dts):
-        sweep = {}
-        for mode in ("off", "dynamic", "learned"):
-            try:
-                ent = run_les(mode, dt_l)
-            except cf.ChanError as e:  # e.g. the reference's CFL abort at too large a time step
-                ent = {"dt": dt_l, "failed": str(e)}
-            sweep[mode] = ent
-            if di == 0:
-                post[mode] = ent
-            log(f"a posteriori dt={dt_l} {mode}: {json.dumps(ent)}")
-        post["dt_sweep"][str(dt_l)] = sweep
-    probe.close()
+    post = {}
+    for dt_l in [float(v) for v in args.les_dts.split(",")]:
+        post[str(dt_l)] = aposteriori(cf, dns, dens, les, net, Ul0, args.post_time, args.sample, DNS["dt"], dt_l,
+                                      args.beta, log=log)
     res["aposteriori"] = post
     dens.close()
     with open(args.out, "w") as f:

@@ -116,3 +116,19 @@ def turbulent_like(nx, ny, nz, yh, y, seed=0, rms=0.1, re_tau=180.0):
     U[0, 0, 0, 0] = 0.0
     U[0, ny, 0, 0] = 0.0
     return make_hermitian(U)
+
+
+def ensemble_draw(n=48, seed=3):
+    """Members of a seeded ensemble of channels of uneven size and Reynolds
+    number: Nx, Nz in {32, 48, 64, 96}, Ny in {49, 65, 97}, Re_c uniform in
+    [2000, 3500], 3/2 dealiasing, box 2 pi x 2 x pi.  Returns a list of dicts
+    (nx, ny, nz, nx2, nz2, re, seed) with a per-member field seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.choice([32, 48, 64, 96]))
+        nz = int(rng.choice([32, 48, 64, 96]))
+        ny = int(rng.choice([49, 65, 97]))
+        re = float(rng.uniform(2000.0, 3500.0))
+        out.append(dict(nx=nx, ny=ny, nz=nz, nx2=3 * nx // 2, nz2=3 * nz // 2, re=re, seed=seed * 1000 + i))
+    return out

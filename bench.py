@@ -83,10 +83,10 @@ def cpu_baseline(c, U0, nsteps=2):
     return nsteps / (t1 - t0), t1 - t0
 
 
-LES_CFG = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.03)
+LES_CFG = dict(nx=64, ny=65, nz=64, nx2=96, nz2=96, dt=0.01)
 
 
-def les_rate(cf, synth, c, steps):
+def les_rate(cf, synth, c, steps, dt=None):
     """RK3 steps/s of the 64 x 65 x 64 LES (same box and Reynolds number) with
     the trained learned closure and its dynamic-Smagorinsky gate, and the
     fraction of points that fell back at the last evaluation."""
@@ -101,7 +101,7 @@ def les_rate(cf, synth, c, steps):
     net = cf.Mlp(params)
     U0 = synth.turbulent_like(L["nx"], L["ny"], L["nz"], yh, y, seed=c["seed"], rms=c["rms"], re_tau=c["re_tau"])
     ch.set_state(data1=U0)
-    ch.set_scalars(dt=L["dt"], t_total=0.0, p_grad=-2.0 / c["re"])
+    ch.set_scalars(dt=L["dt"] if dt is None else dt, t_total=0.0, p_grad=-2.0 / c["re"])
     ch.les_configure("learned", net=net, beta=1.0, every_substep=False)
     ch.initialize_flow()
     ens = cf.Ensemble([ch], nstreams=1)   # the step replayed as one captured graph
@@ -122,26 +122,25 @@ def ensemble_rate(cf, synth, steps, n=48, seed=3):
     """The batched ensemble: n independent channels drawn from a seed
     (Nx, Nz in {32, 48, 64, 96}, Ny in {49, 65, 97}, Re_c uniform in
     [2000, 3500], box 2pi x 2 x pi, dt 0.005, laminar + 1e-2 perturbation),
-    advanced together by one captured graph per step over 8 streams.
-    Returns (ensemble steps/s, grid-point steps/s summed over the members)."""
+    advanced together by one captured graph per step whose every phase is one
+    launch per transform length over all members.  Returns (ensemble steps/s,
+    grid-point steps/s summed over the members, grid points, algorithmic
+    bytes per ensemble step = sum over members of 960 N_c)."""
     import math
     pi = math.atan2(0.0, -1.0)
-    rng = np.random.default_rng(seed)
-    mem, pts = [], 0
-    for i in range(n):
-        nx = int(rng.choice([32, 48, 64, 96]))
-        nz = int(rng.choice([32, 48, 64, 96]))
-        ny = int(rng.choice([49, 65, 97]))
-        re = float(rng.uniform(2000.0, 3500.0))
-        ch = cf.Channel(nx, ny, nz, 3 * nx // 2, 3 * nz // 2, 2.0 * pi, pi, re, fix_mass=1, dt_flag=0)
+    mem, pts, byts = [], 0, 0
+    for d in synth.ensemble_draw(n, seed):
+        nx, ny, nz, re = d["nx"], d["ny"], d["nz"], d["re"]
+        ch = cf.Channel(nx, ny, nz, d["nx2"], d["nz2"], 2.0 * pi, pi, re, fix_mass=1, dt_flag=0)
         y, yh = ch.build_operators(1.5)
-        U0 = synth.laminar_perturbed(nx, ny, nz, yh, y, seed=seed * 1000 + i, amp=1e-2, kmax=4.0)
+        U0 = synth.laminar_perturbed(nx, ny, nz, yh, y, seed=d["seed"], amp=1e-2, kmax=4.0)
         ch.set_state(data1=U0)
         ch.set_scalars(dt=0.005, t_total=0.0, p_grad=-2.0 / re)
         ch.initialize_flow()
         mem.append(ch)
         pts += nx * ny * nz
-    ens = cf.Ensemble(mem, nstreams=8)
+        byts += 960 * nz * (nx // 2 + 1) * (ny + 1)
+    ens = cf.Ensemble(mem)
     ens.step(3)
     for ch in mem:
         ch.sync()
@@ -154,18 +153,87 @@ def ensemble_rate(cf, synth, steps, n=48, seed=3):
     for ch in mem:
         ch.close()
     sps = steps / (t1 - t0)
-    return sps, sps * pts, pts
+    return sps, sps * pts, pts, byts
 
 
-def les_eval_results():
-    """Errors of the committed filtered-DNS evaluation (les_eval.py)."""
-    path = os.path.join(ROOT, "profiles", "les_eval_64x65x64.md")
-    if not os.path.exists(path):
+def dropin_rate(cf, c, U0, steps=400, window=200):
+    """The reference's own driver (PFC/main_mpi.f90 and its kept modules,
+    built from the reference's sources against the replacement modules and
+    libchanflow.so; absent unless built) on the headline workload: its own
+    WTIME line over the last `window` steps of `steps` (each window includes
+    the one host copy of the state the driver takes at its output step).
+    Returns (steps/s, seconds of the window) or None."""
+    import resource
+    import subprocess
+    import tempfile
+    exe = os.path.join(PKG_DIR, "fsrc", "_build", "ref", "mpiflow")
+    if not os.path.exists(exe):
         return None
-    txt = open(path).read()
-    if "```json" not in txt:
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import write_flow_dat
+    with tempfile.TemporaryDirectory() as d:
+        cf.savevel_write(os.path.join(d, "savevel.out"), U0, c["lx_over_pi"] * np.pi, c["lx_over_pi"] * np.pi /
+                         c["aspect"], c["re"], c["dt"], 0.0)
+        write_flow_dat(os.path.join(d, "flow.dat"), c["re"], c["lx_over_pi"], c["aspect"], c["nx"], c["ny"],
+                       c["nz"], c["nx2"], c["nz2"], tend=steps, output_step=window, save_step=10 * steps,
+                       l2_step=10 * steps, read_vel=1, save_vel=0)
+
+        def big_stack():
+            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=300, preexec_fn=big_stack)
+    if r.returncode != 0:
         return None
-    return json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+    w = [float(l.split("=")[1].split()[0]) for l in r.stdout.splitlines() if l.strip().startswith("WTIME")]
+    if not w:
+        return None
+    return window / w[-1], w[-1]
+
+
+def jet_control_rate(cf, synth, steps=1000):
+    """The jet-control run of the linearized-stability configuration:
+    64 x 97 x 64 (96 x 96 dealiased, box 2pi x 2 x pi, stretch 2.75), Re_c
+    2800, a steady Gaussian jet (radius 0.1, peak 0.2) at the lower wall and
+    the wall-shear feedback with gains 0.5, dt 0.002, from a perturbed laminar
+    profile.  Returns (steps/s, max divergence, finite)."""
+    pi = np.pi
+    ch = cf.Channel(64, 97, 64, 96, 96, 2.0 * pi, pi, 2800.0, fix_mass=1, dt_flag=0, ctrlu=0.5, ctrlw=0.5)
+    y, yh = ch.build_operators(2.75)
+    U0 = synth.laminar_perturbed(64, 97, 64, yh, y, seed=23, amp=1e-2, kmax=4.0)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.002, t_total=0.0, p_grad=-2.0 / 2800.0)
+    ch.jet_configure(0, 0.2, 0.1, 0.25 * 2.0 * pi, 0.5 * pi, 0.0, 0.0)
+    ch.initialize_flow()
+    ch.step(5, step0=1)
+    ch.sync()
+    t0 = time.perf_counter()
+    ch.step(steps, step0=6)
+    ch.sync()
+    t1 = time.perf_counter()
+    div = ch.divergence_max()
+    fin = bool(np.isfinite(ch.get_data1()).all())
+    ch.close()
+    return steps / (t1 - t0), div, fin
+
+
+def les_aposteriori_live(cf, dns, spinup_tu, horizon_tu):
+    """A-posteriori errors measured in this run: the headline DNS (after the
+    timed steps) continued `spinup_tu` time units, then filtered on the device
+    onto the 64 x 65 x 64 LES grid; LES runs at dt 0.01 (no model, dynamic
+    Smagorinsky, the committed learned closure with its gate) against the DNS
+    continued over `horizon_tu` time units (les_eval.aposteriori)."""
+    import chanflow.les_eval as E
+    dens = cf.Ensemble([dns])
+    E.spinup(dns, dens, spinup_tu, spinup_tu, 1.0, E.DNS["dt"], log=lambda m: None)
+    les, _, _ = E.channel(cf, E.LES)
+    les.les_filter_from(dns, tau=False)
+    Ul0 = les.get_data1()
+    net = cf.Mlp(np.load(os.path.join(PKG_DIR, "les_net_64x65x64.npy"), allow_pickle=False))
+    res = E.aposteriori(cf, dns, dens, les, net, Ul0, horizon_tu, 1.0, E.DNS["dt"], LES_CFG["dt"], 1.0,
+                        log=lambda m: None)
+    dens.close()
+    les.close()
+    net.close()
+    return res
 
 
 def main():
@@ -177,6 +245,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 48-member batched ensemble timed (0: skip)")
+    ap.add_argument("--dropin-steps", type=int, default=400,
+                    help="steps of the reference's own driver on the headline workload (0: skip)")
+    ap.add_argument("--jet-steps", type=int, default=1000,
+                    help="steps of the 64x97x64 jet-control run (0: skip)")
+    ap.add_argument("--les-spinup", type=float, default=40.0,
+                    help="time units the DNS runs on before the live a-posteriori LES comparison")
+    ap.add_argument("--les-horizon", type=float, default=20.0,
+                    help="time units of the live a-posteriori comparison (0: skip)")
     ap.add_argument("--les-steps", type=int, default=200,
                     help="RK3 steps of the learned-closure LES timed for the LES speed-up (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -262,27 +338,58 @@ def main():
     les = None
     if args.les_steps > 0:
         les_sps, fb = les_rate(cf, synth, c, args.les_steps)
+        les_sps3, _ = les_rate(cf, synth, c, args.les_steps, dt=0.03)
         dns_tu = steps_per_s * c["dt"]
         les_tu = les_sps * LES_CFG["dt"]
-        ev = les_eval_results()
         les = {"metric": "learned-LES speed-up (simulated time per wall second, LES / DNS)",
-               "config": "LES 64x65x64 (96x96 dealiased), dt 0.03 (CFL about 0.7), learned closure with the dynamic-"
+               "config": "LES 64x65x64 (96x96 dealiased), dt 0.01, learned closure with the dynamic-"
                          "Smagorinsky gate (beta 1), stress evaluated once per step, step replayed as a "
                          "captured graph; vs the 192x129x192 DNS above, same box and Re",
                "les_steps_per_s": les_sps, "les_time_units_per_s": les_tu, "dns_time_units_per_s": dns_tu,
                "speedup": les_tu / dns_tu, "fallback_fraction": fb,
-               "error": None if ev is None else ev.get("learned_error"),
-               "error_source": None if ev is None else "profiles/les_eval_64x65x64.md (les_eval.py, "
-                                                       "a posteriori against the filtered DNS)"}
+               "at_dt_0.03": {"les_steps_per_s": les_sps3, "speedup": les_sps3 * 0.03 / dns_tu}}
+        if args.les_horizon > 0:
+            ap_ = les_aposteriori_live(cf, ch, args.les_spinup, args.les_horizon)
+            les["aposteriori"] = {
+                "source": f"measured in this run: the DNS above continued {args.les_spinup:g} time units, then "
+                          f"{args.les_horizon:g} time units of DNS and of each LES (dt 0.01) from the same "
+                          f"filtered field, profiles sampled every time unit (les_eval.aposteriori)",
+                "errors": {m: {k: v for k, v in ap_[m].items() if k.startswith("rel_err") or
+                               k in ("reynolds_stress_max", "failed")}
+                           for m in ("off", "dynamic", "learned") if m in ap_}}
+            le = ap_.get("learned", {})
+            if "rel_err_mean_profile" in le:
+                les["error"] = {"mean_profile": le["rel_err_mean_profile"],
+                                "reynolds_stress_max": le["reynolds_stress_max"]}
 
     ensemble = None
     if args.ensemble_steps > 0:
-        e_sps, e_pts, npts = ensemble_rate(cf, synth, args.ensemble_steps)
+        e_sps, e_pts, npts, e_bytes = ensemble_rate(cf, synth, args.ensemble_steps)
+        e_gbs = e_bytes * e_sps / 1e9
         ensemble = {"metric": "RK3 steps/s of a 48-member batched ensemble (one captured graph per step)",
                     "config": "48 channels from seed 3: Nx, Nz in {32,48,64,96}, Ny in {49,65,97}, Re_c "
-                              "uniform in [2000,3500], box 2pi x 2 x pi, 3/2 dealiased, dt 0.005, 8 streams",
+                              "uniform in [2000,3500], box 2pi x 2 x pi, 3/2 dealiased, dt 0.005; every phase "
+                              "of a substage one launch per transform length over all members",
                     "ensemble_steps_per_s": e_sps, "grid_point_steps_per_s": e_pts, "grid_points": npts,
+                    "roofline": {"bound": "hbm", "achieved": e_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": e_gbs / HBM_PEAK_GBS, "bytes_per_step": e_bytes},
                     "steps": args.ensemble_steps}
+
+    dropin = None
+    if args.dropin_steps > 0:
+        dr = dropin_rate(cf, c, U0, args.dropin_steps, args.dropin_steps // 2)
+        if dr is not None:
+            dropin = {"metric": "RK3 steps/s of the reference's own driver (main_mpi.f90 and its kept modules over "
+                                "the replacement modules), by its WTIME line",
+                      "steps_per_s": dr[0], "window_s": dr[1], "window_steps": args.dropin_steps // 2,
+                      "note": "the window includes the one copy of data1/data2/data4 to the host that the "
+                              "driver's output step takes"}
+    jet = None
+    if args.jet_steps > 0:
+        jsps, jdiv, jfin = jet_control_rate(cf, synth, args.jet_steps)
+        jet = {"config": "64x97x64, Re_c 2800, Gaussian jet radius 0.1 peak 0.2 at the lower wall, feedback "
+                         "gains 0.5, dt 0.002", "steps": args.jet_steps, "steps_per_s": jsps,
+               "max_divergence": jdiv, "finite": jfin}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -319,6 +426,8 @@ def main():
         "cpu_baseline": cpu,
         "les": les,
         "ensemble": ensemble,
+        "dropin": dropin,
+        "jet_control": jet,
     }
     print(json.dumps(out), flush=True)
 

@@ -247,9 +247,17 @@ int chan_autotune(chan_ctx* ctx, int steps, double* best_ms_per_step);
 /* Batched ensemble (no counterpart in the reference): independent channel
  * contexts of any sizes and parameters, each set up as usual (operators,
  * state, chan_initialize_flow), advanced together.  One timestep of every
- * member is captured once into a HIP graph whose members run concurrently on
- * nstreams streams; chan_ensemble_step replays it nsteps times and then
- * reports any member's CFL abort.  Changing a member's tuning or gains
+ * member is captured once into a HIP graph; chan_ensemble_step replays it
+ * nsteps times and then reports any member's CFL abort.
+ * nstreams = 0: batched launches.  Each phase of each RK substage (z-inverse,
+ *   x-pass, z-forward, segmented solve; CFL/dt, wall-data and time updates)
+ *   is one launch per transform (or segment) length covering the work lists
+ *   of every member of that length, whatever their sizes.  Members need the
+ *   default launch shapes, no LES closure, no jet and zero feedback gains;
+ *   lengths nz2, nx2/2 with compile-time plans and ny+1 rows that fit the
+ *   segmented solve.
+ * nstreams >= 1: each member's own kernels, the members spread over nstreams
+ *   concurrent streams.  Changing a member's tuning or gains
  * needs chan_ensemble_invalidate (the next step captures again).  Members
  * must not be used on their own while an ensemble step runs. */
 typedef struct chan_ensemble chan_ensemble;
@@ -318,6 +326,18 @@ int chan_les_get_coefficient(chan_ctx* ctx, double* cs2);
 int chan_les_get_fields(chan_ctx* ctx, double* fields);
 int chan_les_features(chan_ctx* ctx, float* x, double* scale);
 int chan_les_gated_points(chan_ctx* ctx, long* count, long* total);
+/* Filtered DNS (no counterpart in the reference): the velocity of the fine
+ * channel dns filtered onto the coarse channel les, which must have the same
+ * box and every other face of the fine grid (ny_dns = 2 ny_les - 1, both with
+ * their y coordinates set): a 3-point box average on the fine grid in x and z,
+ * the cut-off to the coarse modes (coarse Nyquist modes dropped) and in y
+ * the height-weighted average of the two fine cells of a coarse cell (u, w)
+ * or the shared faces (v), which keeps the filtered field divergence-free.
+ * The result becomes les's velocity.  tau (may be NULL) receives the exact
+ * subgrid stress F(u_i u_j) - F(u_i) F(u_j) at the coarse cell centres,
+ * [6][ny+1][nz2][nx2] of the coarse grid (components 11, 12, 13, 22, 23, 33,
+ * products on the fine dealiased grid, zero on the wall planes). */
+int chan_les_filter_dns(chan_ctx* les, chan_ctx* dns, double* tau);
 
 /* Legacy VTK output as output_hdf_dfsd writes it with xskip = yskip = zskip = 1
  * (PFC/read_write.f90:296-500): big-endian RECTILINEAR_GRID, nx x (ny+1) x nz

@@ -300,3 +300,70 @@ def tau_learned(g, Ph, Y, scale, cs2, beta=1.0):
     tau = np.zeros_like(ts)
     tau[:, 1:ny] = np.where(fb[None], ts[:, 1:ny], tm)
     return tau, int(fb.sum())
+
+
+# ---------------------------------------------------------------------------
+# Filtered DNS: restatement of the device filter (chan_les_filter_dns) for
+# its GPU test.  Fine grid gf, coarse grid gc with the same box and every
+# other fine face (gf.ny = 2 gc.ny - 1).  Pinned by the mathematics too:
+# tests/test_les_oracle.py checks that the filter keeps the discrete
+# divergence zero, passes a constant, and that tau vanishes for a field with
+# no modes above the coarse cut-off and no y structure inside coarse cells.
+# ---------------------------------------------------------------------------
+def filter_spec(gf, gc, C, kind):
+    """One fine spectral field C [ny_f+1, nxh_f, nz_f] -> coarse [ny_c+1,
+    nxh_c, nz_c]: 3-point box in x, z (transfer (1 + 2 cos(k dx))/3 on the
+    fine grid), cut-off without the coarse Nyquist modes, then in y
+    kind 0: height-weighted average of fine cells 2K-1, 2K, walls copied;
+    kind 1: the same, walls zero; kind 2: faces 2K (plane ny_c zero)."""
+    nyf, nyc = gf.ny, gc.ny
+    ix = np.arange(gc.nxh)
+    iz = np.arange(gc.nz)
+    kzi = np.where(iz <= gc.nz // 2, iz, iz - gc.nz)
+    src = np.where(kzi >= 0, kzi, gf.nz + kzi)
+    keep = (ix[:, None] < gc.nxh - 1) & (np.abs(kzi)[None, :] < gc.nz // 2)
+    tx = (1.0 + 2.0 * np.cos(2.0 * math.pi * ix / gf.nx)) / 3.0
+    tz = (1.0 + 2.0 * np.cos(2.0 * math.pi * kzi / gf.nz)) / 3.0
+    f = np.where(keep, tx[:, None] * tz[None, :], 0.0)
+    Cc = C[:, :gc.nxh, :][:, :, src] * f[None]
+    out = np.zeros((nyc + 1, gc.nxh, gc.nz), np.complex128)
+    if kind == 2:
+        out[:nyc] = Cc[0:2 * nyc:2]
+        return out
+    hh = np.zeros(nyf + 1)
+    hh[1:nyf] = gf.y[1:] - gf.y[:-1]
+    K = np.arange(1, nyc)
+    a, b = hh[2 * K - 1], hh[2 * K]
+    out[1:nyc] = (a[:, None, None] * Cc[2 * K - 1] + b[:, None, None] * Cc[2 * K]) / (a + b)[:, None, None]
+    if kind == 0:
+        out[0] = Cc[0]
+        out[nyc] = Cc[nyf]
+    return out
+
+
+def filter_dns(gf, gc, U):
+    """Filtered velocity [3, ny_c+1, nxh_c, nz_c] and the exact subgrid stress
+    tau_ij = F(u_i u_j) - F(u_i) F(u_j) [6, ny_c+1, nz2_c, nx2_c] of the fine
+    velocity U (products of the cell-centre velocities, v averaged from its
+    faces, on the fine dealiased grid; tau zero on the wall planes)."""
+    co = _co()
+    nyf, nyc = gf.ny, gc.ny
+    Uc = np.zeros((3, nyc + 1, gc.nxh, gc.nz), np.complex128)
+    Uc[0] = filter_spec(gf, gc, U[0], 0)
+    Uc[1] = filter_spec(gf, gc, U[1], 2)
+    Uc[2] = filter_spec(gf, gc, U[2], 0)
+    vc = np.zeros_like(U[1])
+    vc[1:nyf] = 0.5 * (U[1, 0:nyf - 1] + U[1, 1:nyf])
+    spec = (U[0], vc, U[2])
+    ph = [co.xz_c2r(gf, s) for s in spec]
+    ub = [co.xz_c2r(gc, filter_spec(gf, gc, s, 1)) for s in spec]
+    tau = np.zeros((6, nyc + 1, gc.nz2, gc.nx2))
+    for t, (i, j) in enumerate(zip((0, 0, 0, 1, 1, 2), (0, 1, 2, 1, 2, 2))):
+        P = ph[i] * ph[j]
+        P[0] = 0.0
+        P[nyf] = 0.0
+        Fp = co.xz_c2r(gc, filter_spec(gf, gc, co.xz_r2c(gf, P), 1))
+        tau[t] = Fp - ub[i] * ub[j]
+        tau[t, 0] = 0.0
+        tau[t, nyc] = 0.0
+    return Uc, tau

@@ -102,3 +102,4 @@ def test_plain_c_client(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "even" in r.stdout
+

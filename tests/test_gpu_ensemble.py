@@ -3,8 +3,10 @@
 * Every replay of the captured timestep graph is checked against twins of
   the members stepped one by one with chan_step, with host synchronisation
   between the replays (the graph holds only kernels; bitwise equality is
-  expected, 1e-14 is allowed).
-* One member is also checked against the oracle after the replays.
+  expected, 1e-14 is allowed): the stream-parallel graph on 5 members, and
+  the batched launches (one launch per phase and transform length over all
+  members) on the 48 members of the seeded ensemble of uneven sizes.
+* Members are also checked against the oracle after the replays.
 * chan_autotune leaves the state unchanged and its choice gives the same
   step to round-off.
 """
@@ -40,7 +42,7 @@ def test_ensemble_replays_match_eager():
     cf = __import__("helpers").chanflow()
     mem = [_member(s, 20 + i) for i, s in enumerate(SHAPES)]
     twins = [_member(s, 20 + i) for i, s in enumerate(SHAPES)]
-    ens = cf.Ensemble([m[2] for m in mem], nstreams=3)
+    ens = cf.Ensemble([m[2] for m in mem], nstreams=3)   # members' own kernels on 3 streams
     for step in range(1, 4):
         ens.step(1)
         for (g, _, a), (_, _, b) in zip(mem, twins):
@@ -81,4 +83,58 @@ def test_autotune_restores_state_and_keeps_results():
         assert rel_l2(x, y) <= 1e-12
     with pytest.raises(Exception):
         a.set_tuning("solve_bl", 5)
+    a.close(); b.close()
+
+
+def _draw_member(d, pi=np.pi):
+    g = O.make_grid(d["nx"], d["ny"], d["nz"], d["nx2"], d["nz2"], 2.0, 2.0, d["re"], stretch=1.5, fix_mass=1)
+    U0 = synth().laminar_perturbed(d["nx"], d["ny"], d["nz"], g.yh, g.y, seed=d["seed"], amp=1e-2, kmax=4.0)
+    ch = make_channel(g)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.005, t_total=0.0, p_grad=-2.0 / d["re"])
+    ch.initialize_flow()
+    return g, U0, ch
+
+
+def test_batched_ensemble_48_members():
+    """The seeded 48-member ensemble (Nx, Nz in {32, 48, 64, 96}, Ny in {49, 65,
+    97}, Re_c in [2000, 3500]) advanced by the batched launches: two replays,
+    each checked against the 48 twins stepped one by one (bitwise equality
+    expected, 1e-14 allowed), then the two smallest members against the
+    oracle's RK3 steps (relative L2 <= 1e-10, divergence <= 1e-12)."""
+    cf = __import__("helpers").chanflow()
+    draws = synth().ensemble_draw(48, seed=3)
+    mem = [_draw_member(d) for d in draws]
+    twins = [_draw_member(d) for d in draws]
+    ens = cf.Ensemble([m[2] for m in mem])            # batched launches
+    for step in (1, 2):
+        print(f"batched ensemble replay {step}", flush=True)
+        ens.step(1)
+        for (g, _, a), (_, _, b) in zip(mem, twins):
+            b.step(1, step0=step)
+            for x, y in zip(a.get_state(), b.get_state()):
+                assert rel_l2(x, y) <= 1e-14, (g.nx, g.ny, g.nz)
+            assert a.get_scalars()["t_total"] == b.get_scalars()["t_total"]
+    small = sorted(range(len(draws)), key=lambda i: draws[i]["nx"] * draws[i]["ny"] * draws[i]["nz"])[:2]
+    for i in small:
+        g, U0, ch = mem[i]
+        st = O.initial_state(g, U0, dt=0.005)
+        for s in (1, 2):
+            O.rk3_step(g, st, s)
+        d1 = ch.get_data1()
+        for c in range(3):
+            assert rel_l2(d1[c], st.U[c]) <= 1e-10, (i, c)
+        assert ch.divergence_max() <= 1e-12
+    ens.close()
+    for _, _, ch in mem + twins:
+        ch.close()
+
+
+def test_batched_ensemble_rejects_ineligible_member():
+    cf = __import__("helpers").chanflow()
+    g, _, a = _member(SHAPES[0], 40)
+    _, _, b = _member(SHAPES[1], 41)
+    b.set_tuning("solve_seg", 0)
+    with pytest.raises(cf.ChanError, match="member 1 cannot join the batched launches"):
+        cf.Ensemble([a, b])
     a.close(); b.close()

@@ -245,3 +245,28 @@ def test_les_learned_config4_steps():
         tref, _ = L.tau_learned(g, Ph, Y, sc, cs2, beta=1.0)
         assert rel_l2(taus[k], tref) <= 1e-3, k
     net.close()
+
+
+def test_filtered_dns_on_device():
+    """chan_les_filter_dns (the filtered velocity and the exact subgrid stress
+    of a fine channel on a coarse one) against the restatement in
+    oracle/les_oracle.py: 32x33x32 (48x48 dealiased) onto 16x17x16 (24x24),
+    same box and stretching; relative L2 <= 1e-12 for the velocity and the
+    stress, and the filtered field is divergence-free."""
+    gf = O.make_grid(32, 33, 32, 48, 48, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1)
+    gc = O.make_grid(16, 17, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1)
+    s = __import__("helpers").synth()
+    U = O.remove_initial_divergence(gf, s.turbulent_like(gf.nx, gf.ny, gf.nz, gf.yh, gf.y, seed=5, rms=0.2))
+    dns = make_channel(gf)
+    les = make_channel(gc)
+    dns.set_state(data1=U)
+    tau = les.les_filter_from(dns)
+    Uc = les.get_data1()
+    Ur, tr = L.filter_dns(gf, gc, U)
+    assert rel_l2(Uc, Ur) <= 1e-12
+    assert rel_l2(tau, tr) <= 1e-12
+    assert np.abs(tr).max() > 1e-4
+    assert les.divergence_max() <= 1e-12
+    les.les_filter_from(dns, tau=False)
+    assert rel_l2(les.get_data1(), Ur) <= 1e-12
+    dns.close(); les.close()

@@ -70,3 +70,70 @@ def test_smagorinsky_linear_fields():
     tau = L.smagorinsky_tau(G, np.array([D]), cs)
     np.testing.assert_allclose(tau[0], [-4 * (cs * D) ** 2 * abs(a) * a, 0, 0, 4 * (cs * D) ** 2 * abs(a) * a, 0, 0],
                                atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Filtered DNS (oracle/les_oracle.py: filter_spec, filter_dns)
+# ---------------------------------------------------------------------------
+def _filter_grids():
+    from oracle import chan_oracle as O
+    gf = O.make_grid(32, 33, 32, 48, 48, 2.0, 2.0, 2500.0, stretch=1.5)
+    gc = O.make_grid(16, 17, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5)
+    return O, gf, gc
+
+
+def test_filter_grids_share_every_other_face():
+    O, gf, gc = _filter_grids()
+    assert np.allclose(gc.y, gf.y[::2], rtol=0, atol=1e-15)
+
+
+def test_filter_keeps_divergence_zero():
+    """The y restriction (height-weighted cell averages for u and w, shared
+    faces for v) with a mode-wise x-z filter maps a divergence-free fine field
+    onto a divergence-free coarse one: the coarse cell's flux balance is the
+    sum of its two fine cells'."""
+    O, gf, gc = _filter_grids()
+    rng = np.random.default_rng(4)
+    U = rng.standard_normal((3, gf.ny + 1, gf.nxh, gf.nz)) + 1j * rng.standard_normal((3, gf.ny + 1, gf.nxh, gf.nz))
+    U = O.remove_initial_divergence(gf, U)
+    assert O.max_divergence(gf, U) < 1e-10
+    Uc = np.stack([L.filter_spec(gf, gc, U[0], 0), L.filter_spec(gf, gc, U[1], 2), L.filter_spec(gf, gc, U[2], 0)])
+    assert np.abs(Uc).max() > 0.1
+    assert O.max_divergence(gc, Uc) < 1e-10
+    # unfiltered random data has divergence of order one: the check has teeth
+    V = rng.standard_normal(U.shape) + 0j
+    Vc = np.stack([L.filter_spec(gf, gc, V[0], 0), L.filter_spec(gf, gc, V[1], 2), L.filter_spec(gf, gc, V[2], 0)])
+    assert O.max_divergence(gc, Vc) > 1.0
+
+
+def test_filter_transfer_and_constants():
+    """A single Fourier mode constant in y comes out multiplied by the box
+    transfer (1 + 2 cos(kx dx))/3 (1 + 2 cos(kz dz))/3 of the fine grid; a
+    constant passes unchanged; the coarse Nyquist modes are dropped."""
+    O, gf, gc = _filter_grids()
+    C = np.zeros((gf.ny + 1, gf.nxh, gf.nz), np.complex128)
+    ix, kz = 3, -2
+    C[:, ix, kz % gf.nz] = 0.7 - 0.2j
+    C[:, 0, 0] = 1.5
+    out = L.filter_spec(gf, gc, C, 0)
+    f = (1 + 2 * np.cos(2 * np.pi * ix / gf.nx)) / 3 * (1 + 2 * np.cos(2 * np.pi * kz / gf.nz)) / 3
+    assert np.allclose(out[:, ix, kz % gc.nz], f * (0.7 - 0.2j), rtol=0, atol=1e-15)
+    assert np.allclose(out[:, 0, 0], 1.5, rtol=0, atol=1e-15)
+    C2 = np.zeros_like(C)
+    C2[:, gc.nxh - 1, 1] = 1.0
+    C2[:, 1, gc.nz // 2] = 1.0
+    o2 = L.filter_spec(gf, gc, C2, 0)
+    assert np.abs(o2).max() == 0.0
+
+
+def test_filter_stress_vanishes_for_cellwise_constant_mean_flow():
+    """tau = F(u u) - F(u) F(u) is zero for a mean flow that is constant over
+    each coarse cell (equal on its two fine cells): nothing is unresolved."""
+    O, gf, gc = _filter_grids()
+    U = np.zeros((3, gf.ny + 1, gf.nxh, gf.nz), np.complex128)
+    prof = np.zeros(gf.ny + 1)
+    for K in range(1, gc.ny):
+        prof[2 * K - 1] = prof[2 * K] = 1.0 - gc.yh[K] ** 2
+    U[0, :, 0, 0] = prof
+    _, tau = L.filter_dns(gf, gc, U)
+    assert np.abs(tau).max() < 1e-14
