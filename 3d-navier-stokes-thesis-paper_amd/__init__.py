@@ -60,8 +60,11 @@ ENTRY_POINTS = (
     "chan_les_configure", "chan_les_update", "chan_les_set_tau", "chan_les_get_tau", "chan_les_get_coefficient",
     "chan_les_get_fields", "chan_les_features", "chan_les_gated_points", "chan_jet_configure",
     "chan_strip_divergence_modes", "chan_subtract_pressure_gradient_modes",
-    "chan_les_filter_dns",
+    "chan_compute_divergence_of_fluxes_noalloc", "chan_les_filter_dns",
 )
+
+# bytes of the fixed per-GPU scratch array of chan_compute_divergence_of_fluxes_noalloc
+CHAN_NOALLOC_SCRATCH_BYTES = 8 << 20
 
 _lib = None
 _fp = C.POINTER(C.c_float)
@@ -102,6 +105,7 @@ def load_library(path: str = LIBPATH):
         "chan_xz_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_xz_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_compute_divergence_of_fluxes": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_int]),
+        "chan_compute_divergence_of_fluxes_noalloc": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, vp]),
         "chan_les_filter_dns": (C.c_int, [vp, vp, _dp]),
         "chan_small_fft_complex_to_real": (C.c_int, [vp, _dp, _dp, C.c_int]),
         "chan_small_fft_real_to_complex": (C.c_int, [vp, _dp, _dp, C.c_int]),
@@ -447,6 +451,15 @@ class Channel:
         out = np.empty((npl, self.nxh, self.nz), np.complex128)
         self._chk(self.lib.chan_xz_fft_real_to_complex(self.h, _ptr(data), _ptr(out), npl))
         return out
+
+    def div_fluxes_noalloc(self, fx_dev, fy_dev, fz_dev, out_dev, nplanes, stream=None):
+        """compute_divergence_of_fluxes on device arrays (integer device
+        addresses, e.g. a torch tensor's data_ptr()) with no allocation: fx,
+        fy, fz real [nplanes][nz2][nx2], out complex [nplanes][nx/2+1][nz];
+        queued on `stream` (a hipStream_t address; None: the context's
+        stream) without synchronising."""
+        self._chk(self.lib.chan_compute_divergence_of_fluxes_noalloc(self.h, fx_dev, fy_dev, fz_dev, out_dev,
+                                                                     int(nplanes), stream))
 
     def div_fluxes(self, fx, fy, fz):
         fx, fy, fz = _f64(fx), _f64(fy), _f64(fz)

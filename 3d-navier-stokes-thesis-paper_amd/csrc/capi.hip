@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <algorithm>
 #include "chan_internal.h"
 
@@ -1223,6 +1224,69 @@ int chan_compute_divergence_of_fluxes(chan_ctx* c, const double* fx, const doubl
     dev_div(c, (const double*)(s + cb), (const double*)(s + cb + rb), (const double*)(s + cb + 2 * rb), np,
             (double2*)s);
     d2h(c, cdata, s, cb);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of compute_divergence_of_fluxes for callers that cannot
+// allocate device memory.  The x-transform intermediates (A = F(fy) + i kx
+// F(fx), B = F(fz), [2][nplanes][nz2][kxpad] complex) live in `scratch`, a
+// fixed array in the code object's device memory: each GPU holds its own copy
+// for the life of the process, and every call on that GPU uses it, so calls
+// that overlap on one GPU race.  Every element the z-forward reads is written
+// by the x-transforms of the same call, so nothing needs clearing between
+// calls.  Host code never passes `scratch` itself to a kernel (its name is a
+// host-side stand-in); it uses the address hipGetSymbolAddress returns for
+// the current device, looked up once per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double2 scratch[CHAN_NOALLOC_SCRATCH_BYTES / sizeof(double2)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+
+double2* scratch_for_device(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw chan_error("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    CHAN_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double2*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int chan_compute_divergence_of_fluxes_noalloc(chan_ctx* c, const double* fx, const double* fy, const double* fz,
+                                              double* cdata, int np, void* stream) {
+  return guarded(c, [&] {
+    check_ops(c);
+    if (np < 1) throw chan_error("nplanes must be >= 1");
+    if (!fx || !fy || !fz || !cdata) throw chan_error("device arrays datax, datay, dataz and cdata are required");
+    const XZGeom& g = c->gd;
+    const long aps = (long)g.nz2 * g.kxpad;
+    const size_t need = sizeof(double2) * 2 * (size_t)np * (size_t)aps;
+    if (need > CHAN_NOALLOC_SCRATCH_BYTES)
+      throw chan_error("compute_divergence_of_fluxes_noalloc: " + std::to_string(np) + " planes need " +
+                       std::to_string(need) + " bytes of scratch, more than the " +
+                       std::to_string((size_t)CHAN_NOALLOC_SCRATCH_BYTES) + " available (at most " +
+                       std::to_string(CHAN_NOALLOC_SCRATCH_BYTES / (sizeof(double2) * 2 * aps)) +
+                       " planes per call at this grid)");
+    int cur = 0;
+    CHAN_CHECK(hipGetDevice(&cur));
+    if (cur != c->device) CHAN_CHECK(hipSetDevice(c->device));
+    struct Restore {
+      int d, was;
+      ~Restore() { if (d != was) (void)hipSetDevice(was); }
+    } restore{c->device, cur};
+    double2* A = scratch_for_device(c->device);
+    double2* B = A + (long)np * aps;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const long ips = (long)g.nz2 * g.nx2, ops_ = (long)g.nxh * g.nz;
+    launch_xr2c(g, c->d_tw, fy, fx, c->ops.kx, ips, A, aps, np, s);
+    launch_xr2c(g, c->d_tw, fz, nullptr, c->ops.kx, ips, B, aps, np, s);
+    launch_zfwd(g, c->d_tw, A, B, aps, c->ops.kz, (double2*)cdata, ops_, np, 0, s);
   });
 }
 

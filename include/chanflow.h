@@ -143,6 +143,19 @@ int chan_xz_fft_complex_to_real(chan_ctx* ctx, const double* cdata, double* data
 int chan_xz_fft_real_to_complex(chan_ctx* ctx, const double* data, double* cdata, int nplanes);
 int chan_compute_divergence_of_fluxes(chan_ctx* ctx, const double* datax, const double* datay,
                                       const double* dataz, double* cdata, int nplanes);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): compute_divergence_of_fluxes on DEVICE arrays, with no
+ * allocation, copy or synchronisation.  datax, datay, dataz real(nx2, nz2,
+ * nplanes) and cdata complex(nz, nx/2+1, nplanes) are device pointers on the
+ * context's GPU; stream is a hipStream_t (NULL: the context's stream).  The
+ * x-transform intermediates live in a fixed per-GPU device array of
+ * CHAN_NOALLOC_SCRATCH_BYTES, shared by every call on that GPU, so calls that
+ * overlap on one GPU race; nplanes * 2 * nz2 * 16 * (nx/2+1 rounded up to 8)
+ * bytes must fit (8 planes at nx2 = nz2 = 288), else an error is returned. */
+#define CHAN_NOALLOC_SCRATCH_BYTES (8u << 20)
+int chan_compute_divergence_of_fluxes_noalloc(chan_ctx* ctx, const double* datax, const double* datay,
+                                              const double* dataz, double* cdata, int nplanes, void* stream);
 /* small_fft_* (PFC/transform.f90:950-1000, 1071-1142): data real(nx, nz, nplanes). */
 int chan_small_fft_complex_to_real(chan_ctx* ctx, const double* cdata, double* data, int nplanes);
 int chan_small_fft_real_to_complex(chan_ctx* ctx, const double* data, double* cdata, int nplanes);

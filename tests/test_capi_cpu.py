@@ -103,3 +103,16 @@ def test_plain_c_client(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "even" in r.stdout
 
+
+def test_noalloc_scratch_size_matches_header():
+    import re as _re
+    m = _re.search(r"#define\s+CHAN_NOALLOC_SCRATCH_BYTES\s+\((\d+)u\s*<<\s*(\d+)\)", open(HEADER).read())
+    assert m, "CHAN_NOALLOC_SCRATCH_BYTES not found in the header"
+    assert chanflow().CHAN_NOALLOC_SCRATCH_BYTES == int(m.group(1)) << int(m.group(2))
+
+
+def test_noalloc_entry_point_rejects_bad_arguments_without_gpu():
+    cf = chanflow()
+    lib = cf.load_library()
+    # a null context is refused before any device call
+    assert lib.chan_compute_divergence_of_fluxes_noalloc(None, None, None, None, None, 1, None) != 0
