@@ -409,7 +409,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
     case 9: if (v != 0 && (v < 2 || v > 64)) return false; t.xpass_chunk = v; return true;
     case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
-    case 11: if (v != 0 && v != 1) return false; t.solve_seg = v; return true;
+    case 11: if (v < 0 || v > 2) return false; t.solve_seg = v; return true;
     default: return false;
   }
 }
@@ -484,6 +484,11 @@ struct chan_ensemble {
   DevScalars** scs = nullptr;
   size_t cbc_words = 0, cfl_words = 0;
   std::vector<void*> dmem;     // device tables of the batched launches
+  // the launches of one phase touch disjoint members: they run as parallel
+  // branches of the graph (fork from the master stream, join back)
+  std::vector<hipStream_t> bstreams;
+  std::vector<hipEvent_t> bjoin;
+  hipEvent_t bfork = nullptr;
 };
 
 namespace {
@@ -668,18 +673,37 @@ void ensemble_plan_batched(chan_ensemble* e) {
 void ensemble_step_batched(chan_ensemble* e, hipStream_t s) {
   const int n = (int)e->m.size();
   const bool lin = e->lin != 0;
+  // one phase: its launches (one per length, disjoint members) side by side
+  auto phase = [&](const std::vector<BatchLaunch>& v, auto launch) {
+    if (v.size() <= 1 || e->bstreams.empty()) {
+      for (const BatchLaunch& b : v) launch(b, s);
+      return;
+    }
+    CHAN_CHECK(hipEventRecord(e->bfork, s));
+    for (size_t i = 0; i < v.size(); ++i) {
+      hipStream_t bs = e->bstreams[i % e->bstreams.size()];
+      CHAN_CHECK(hipStreamWaitEvent(bs, e->bfork, 0));
+      launch(v[i], bs);
+      CHAN_CHECK(hipEventRecord(e->bjoin[i % e->bjoin.size()], bs));
+      CHAN_CHECK(hipStreamWaitEvent(s, e->bjoin[i % e->bjoin.size()], 0));
+    }
+  };
   launch_fill_zero_batched(e->fill_cbc, n, e->cbc_words, s);
   for (int rk = 1; rk <= 3; ++rk) {
     if (rk == 1) launch_fill_zero_batched(e->fill_cfl, n, e->cfl_words, s);
-    for (const BatchLaunch& b : e->zinv)
-      launch_zinv_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, s);
-    for (const BatchLaunch& b : e->xpass[rk == 1 ? 1 : 0])
-      launch_xpass_batched(b.len, lin, (const XBatchArgs*)b.args, b.items, b.nitems, s);
-    for (const BatchLaunch& b : e->zfwd)
-      launch_zfwd_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, s);
+    phase(e->zinv, [&](const BatchLaunch& b, hipStream_t st) {
+      launch_zinv_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, st);
+    });
+    phase(e->xpass[rk == 1 ? 1 : 0], [&](const BatchLaunch& b, hipStream_t st) {
+      launch_xpass_batched(b.len, lin, (const XBatchArgs*)b.args, b.items, b.nitems, st);
+    });
+    phase(e->zfwd, [&](const BatchLaunch& b, hipStream_t st) {
+      launch_zfwd_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, st);
+    });
     if (rk == 1) launch_cfl_finalize_batched(e->cfl_args, n, 1, s);
-    for (const BatchLaunch& b : e->solve[rk - 1])
-      launch_solve_batched(b.len, (const SBatchArgs*)b.args, b.items, b.nitems, s);
+    phase(e->solve[rk - 1], [&](const BatchLaunch& b, hipStream_t st) {
+      launch_solve_batched(b.len, (const SBatchArgs*)b.args, b.items, b.nitems, st);
+    });
   }
   launch_advance_time_batched(e->scs, n, s);
 }
@@ -1752,6 +1776,16 @@ int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int line
       }
     const int ns = e->batched ? 0 : std::max(1, std::min(nstreams, n));
     CHAN_CHECK(hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
+    if (e->batched) {
+      const int nb = std::min(n, 4);
+      e->bstreams.resize(nb, nullptr);
+      e->bjoin.resize(nb, nullptr);
+      CHAN_CHECK(hipEventCreateWithFlags(&e->bfork, hipEventDisableTiming));
+      for (int k = 0; k < nb; ++k) {
+        CHAN_CHECK(hipStreamCreateWithFlags(&e->bstreams[k], hipStreamNonBlocking));
+        CHAN_CHECK(hipEventCreateWithFlags(&e->bjoin[k], hipEventDisableTiming));
+      }
+    }
     CHAN_CHECK(hipEventCreateWithFlags(&e->start, hipEventDisableTiming));
     e->streams.resize(ns, nullptr);
     e->done.resize(ns, nullptr);
@@ -1819,6 +1853,11 @@ int chan_ensemble_destroy(chan_ensemble* e) {
     if (ev) (void)hipEventDestroy(ev);
   for (hipStream_t st : e->streams)
     if (st) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : e->bjoin)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipStream_t st : e->bstreams)
+    if (st) (void)hipStreamDestroy(st);
+  if (e->bfork) (void)hipEventDestroy(e->bfork);
   if (e->start) (void)hipEventDestroy(e->start);
   if (e->master) (void)hipStreamDestroy(e->master);
   delete e;

@@ -138,7 +138,7 @@ struct Tuning {
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
   int xpass_chunk = 0;    // planes per x-pass workgroup (2..64; 0: from the resident workgroups)
   int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
-  int solve_seg = 1;      // segmented column solve (columns in registers, no scratch)
+  int solve_seg = 1;      // segmented column solve (columns in registers, no scratch); 2: two modes per wavefront
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -269,7 +269,7 @@ void launch_solve_batched(int seg, const SBatchArgs* args, const BatchItem* item
 void launch_cfl_finalize_batched(const CflBatchArgs* args, int n, int step, hipStream_t s);
 void launch_fill_zero_batched(const FillBatchArgs* args, int n, size_t max_words, hipStream_t s);
 void launch_advance_time_batched(DevScalars* const* sc, int n, hipStream_t s);
-int solve_seg_rows(int ny);  // rows per segment of the segmented solve, 0 if ny+1 rows do not fit
+int solve_seg_rows(int ny, int maxsegs = 16);  // rows per segment of the segmented solve, 0 if ny+1 rows do not fit
 void launch_jet_bc(const DevScalars* sc, int rk, double forcing, double freq, const double2* prof, double2* cbcv,
                    long nm, hipStream_t s);
 void launch_l2_planes(const double* phys, long ps, int nplanes, const double* lam, const double* dy, double* part,

@@ -602,14 +602,16 @@ __device__ __forceinline__ void solve_seg_body(const DevOps& ops, const SolveArg
   const int ny = a.ny, nyp = ny + 1;
   const int ns = (nyp + SEG - 1) / SEG;   // active segments
   const int rl = nyp - (ns - 1) * SEG;    // rows of the last one (3..SEG)
-  // Mode groups whose rows share 128-byte lines (MODES < 8) go to workgroups
-  // b and b+8, which land on the same XCD one after the other.
+  // The G = 8 / MODES mode groups whose rows share 128-byte lines (MODES < 8)
+  // go to workgroups b, b+8, .., b+8(G-1), which land on the same XCD one
+  // after the other.
   long grp = bx;
   if (MODES < 8 && gx > 0) {
-    const long full = (gx / 16) * 16;
+    constexpr long G = 8 / MODES;
+    const long full = (gx / (8 * G)) * (8 * G);
     if (grp < full) {
       const long xcd = grp & 7, k = grp >> 3;
-      grp = 16 * (k >> 1) + 2 * xcd + (k & 1);
+      grp = 8 * G * (k / G) + G * xcd + (k % G);
     }
   }
   const long mg = grp * MODES + q;
@@ -982,14 +984,15 @@ __global__ __launch_bounds__(64) void k_solve_seg_b(const SBatchArgs* __restrict
 }
 
 // Rows per segment of the segmented solve for ny+1 rows: the shortest of
-// 5..9 with at most 16 segments and 3 or more rows in the last one (the wall
-// rows 0..2 and ny-2..ny each lie in one segment); 0 if none fits.
-int solve_seg_rows(int ny) {
+// 5..9 with at most maxsegs segments (64 / modes per wavefront) and 3 or more
+// rows in the last one (the wall rows 0..2 and ny-2..ny each lie in one
+// segment); 0 if none fits.
+int solve_seg_rows(int ny, int maxsegs) {
   const int nyp = ny + 1;
   if (ny < 8) return 0;
   for (int seg = 5; seg <= 9; ++seg) {
     const int ns = (nyp + seg - 1) / seg, rl = nyp - (ns - 1) * seg;
-    if (ns <= 16 && rl >= 3) return seg;
+    if (ns <= maxsegs && rl >= 3) return seg;
   }
   return 0;
 }
@@ -1463,6 +1466,15 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
   // the segmented and component-parallel kernels need the byte offsets of one
   // array to fit 32 bits
   const bool off32 = 3.0 * (a.ny + 1) * (double)nm * 16.0 < 4.0e9;
+  // two modes per wavefront (32 segments): the 5-row segments keep the
+  // wavefront within 256 registers, two wavefronts per SIMD
+  const int seg2 = (a.seg == 2) ? solve_seg_rows(a.ny, 32) : 0;
+  if (seg2 == 5 && off32) {
+    dim3 gs((unsigned)((nm + 1) / 2));
+    hipLaunchKernelGGL((k_solve_seg<2, 5>), gs, dim3(64), 0, s, ops, a);
+    LAUNCH_CHECK("k_solve_seg");
+    return;
+  }
   const int seg = a.seg ? solve_seg_rows(a.ny) : 0;
   if (seg && off32) {
     dim3 gs((unsigned)((nm + 3) / 4));

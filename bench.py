@@ -59,6 +59,11 @@ def measured_traffic(kernel):
     return (float(v), "profiles/hbm_traffic_192x129x192.md") if v is not None else (None, None)
 
 
+def note(msg):
+    """Progress line on stderr (the JSON line is the only output on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_channel(cf, c):
     import math
     pi = math.atan2(0.0, -1.0)
@@ -299,6 +304,7 @@ def main():
         return
 
     # ---- timed region: exactly K steps, synchronised on both sides
+    note(f"headline: {args.warmup} warm-up steps done, timing {args.steps}")
     ch.sync()
     t0 = time.perf_counter()
     ch.step(args.steps, step0=args.warmup + 1)
@@ -335,8 +341,10 @@ def main():
     step_gbs = alg_step * steps_per_s / 1e9
     traffic, traffic_src = measured_traffic(name)
 
+    note(f"headline: {steps_per_s:.1f} steps/s")
     les = None
     if args.les_steps > 0:
+        note("LES speed at dt 0.01 and 0.03")
         les_sps, fb = les_rate(cf, synth, c, args.les_steps)
         les_sps3, _ = les_rate(cf, synth, c, args.les_steps, dt=0.03)
         dns_tu = steps_per_s * c["dt"]
@@ -349,6 +357,7 @@ def main():
                "speedup": les_tu / dns_tu, "fallback_fraction": fb,
                "at_dt_0.03": {"les_steps_per_s": les_sps3, "speedup": les_sps3 * 0.03 / dns_tu}}
         if args.les_horizon > 0:
+            note(f"LES a posteriori: {args.les_spinup:g} + {args.les_horizon:g} time units")
             ap_ = les_aposteriori_live(cf, ch, args.les_spinup, args.les_horizon)
             les["aposteriori"] = {
                 "source": f"measured in this run: the DNS above continued {args.les_spinup:g} time units, then "
@@ -364,6 +373,7 @@ def main():
 
     ensemble = None
     if args.ensemble_steps > 0:
+        note("48-member ensemble")
         e_sps, e_pts, npts, e_bytes = ensemble_rate(cf, synth, args.ensemble_steps)
         e_gbs = e_bytes * e_sps / 1e9
         ensemble = {"metric": "RK3 steps/s of a 48-member batched ensemble (one captured graph per step)",
@@ -377,6 +387,7 @@ def main():
 
     dropin = None
     if args.dropin_steps > 0:
+        note("the reference's own driver")
         dr = dropin_rate(cf, c, U0, args.dropin_steps, args.dropin_steps // 2)
         if dr is not None:
             dropin = {"metric": "RK3 steps/s of the reference's own driver (main_mpi.f90 and its kept modules over "
@@ -386,6 +397,7 @@ def main():
                               "driver's output step takes"}
     jet = None
     if args.jet_steps > 0:
+        note("jet-control run")
         jsps, jdiv, jfin = jet_control_rate(cf, synth, args.jet_steps)
         jet = {"config": "64x97x64, Re_c 2800, Gaussian jet radius 0.1 peak 0.2 at the lower wall, feedback "
                          "gains 0.5, dt 0.002", "steps": args.jet_steps, "steps_per_s": jsps,
@@ -393,6 +405,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        note("CPU baseline (oracle)")
         sps, secs = cpu_baseline(c, U0, args.cpu_steps)
         cpu = {"value": sps, "unit": "RK3 steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} RK3 steps of the 192x129x192 workload by the numpy oracle "

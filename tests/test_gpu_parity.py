@@ -281,14 +281,17 @@ def test_rhs_solve_variants(g0, variant):
             assert rel_l2(x, y) <= 1e-14, rk
 
 
-@pytest.mark.parametrize("ny", [33, 65, 97, 129])
-def test_rhs_solve_segmented(ny):
+@pytest.mark.parametrize("ny,layout", [(33, 1), (65, 1), (97, 1), (129, 1), (33, 2), (97, 2), (129, 2)])
+def test_rhs_solve_segmented(ny, layout):
     """The segmented column solve (partitioned tridiagonal solves, columns in
     registers) against the oracle for the three substeps, with a perturbed RK
-    register and wall data on both walls, at row counts that pick segments of
-    5 rows (ny = 33: 7 segments, idle lanes past them), 6 (65: 11), 7 (97: 14)
-    and 9 (129: 15), 4 modes per wavefront; and against the sweep-based
-    3-lane solve to 1e-12."""
+    register and wall data on both walls, and against the sweep-based 3-lane
+    solve to 1e-12.  Layout 1, 4 modes per wavefront: segments of 5 rows
+    (ny = 33: 7 segments, idle lanes past them), 6 (65: 11), 7 (97: 14) and
+    9 (129: 15).  Layout 2, 2 modes per wavefront: 5-row segments, 7 of 32
+    at ny = 33, 20 at 97 (3 rows in the last), 26 at 129."""
+    # 9 x 16 = 144 modes: 72 two-mode groups, 64 in the XCD-grouped prefix of
+    # the grid and 8 past it
     g = O.make_grid(16, ny, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1, dt_flag=0)
     rng = np.random.default_rng(ny)
     s = __import__("helpers").synth()
@@ -306,7 +309,7 @@ def test_rhs_solve_segmented(ny):
         s0.R[1, g.ny] = 0.0
         s0.cbc_top, s0.cbc_bot = cbc_top, cbc_bot
         outs = []
-        for seg in (1, 0):
+        for seg in (layout, 0):
             ch = make_channel(g)
             ch.set_tuning("solve_seg", seg)
             ch.set_state(data1=NL, data2=O.to_fortran_data2(s0.R), data4=O.to_fortran_data4(s0.P))
