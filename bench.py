@@ -59,6 +59,21 @@ def measured_traffic(kernel):
     return (float(v), "profiles/hbm_traffic_192x129x192.md") if v is not None else (None, None)
 
 
+def measured_counters():
+    """FP64 VALU, VALU-busy, LDS-busy and MFMA figures of the step kernels and
+    the closure network from the committed rocprofv3 counter summary (None
+    when absent)."""
+    path = os.path.join(ROOT, "profiles", "counters_192x129x192.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    if "```json" not in txt:
+        return None
+    data = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+    data["source"] = "profiles/counters_192x129x192.md"
+    return data
+
+
 def note(msg):
     """Progress line on stderr (the JSON line is the only output on stdout)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -167,7 +182,8 @@ def dropin_rate(cf, c, U0, steps=400, window=200):
     libchanflow.so; absent unless built) on the headline workload: its own
     WTIME line over the last `window` steps of `steps` (each window includes
     the one host copy of the state the driver takes at its output step).
-    Returns (steps/s, seconds of the window) or None."""
+    Returns (steps/s, seconds of the window), None when the driver is not
+    built, or a dict naming the failure."""
     import resource
     import subprocess
     import tempfile
@@ -186,11 +202,10 @@ def dropin_rate(cf, c, U0, steps=400, window=200):
         def big_stack():
             resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
         r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=300, preexec_fn=big_stack)
-    if r.returncode != 0:
-        return None
     w = [float(l.split("=")[1].split()[0]) for l in r.stdout.splitlines() if l.strip().startswith("WTIME")]
-    if not w:
-        return None
+    if r.returncode != 0 or not w:
+        return {"error": f"driver exit status {r.returncode}, {len(w)} WTIME lines",
+                "stdout_tail": r.stdout[-1500:], "stderr_tail": r.stderr[-1500:]}
     return window / w[-1], w[-1]
 
 
@@ -389,7 +404,9 @@ def main():
     if args.dropin_steps > 0:
         note("the reference's own driver")
         dr = dropin_rate(cf, c, U0, args.dropin_steps, args.dropin_steps // 2)
-        if dr is not None:
+        if isinstance(dr, dict):
+            dropin = dr
+        elif dr is not None:
             dropin = {"metric": "RK3 steps/s of the reference's own driver (main_mpi.f90 and its kept modules over "
                                 "the replacement modules), by its WTIME line",
                       "steps_per_s": dr[0], "window_s": dr[1], "window_steps": args.dropin_steps // 2,
@@ -434,6 +451,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "bytes_per_launch": bytes_launch, "avg_launch_ms": dur_ms},
         "kernels_ms_per_launch": per_launch,
+        "counters": measured_counters(),
         "kernel_launches_measured": {k: v[1] for k, v in kt.items()},
         "max_divergence": div,
         "cpu_baseline": cpu,
