@@ -182,6 +182,11 @@ def dropin_rate(cf, c, U0, steps=400, window=200):
     libchanflow.so; absent unless built) on the headline workload: its own
     WTIME line over the last `window` steps of `steps` (each window includes
     the one host copy of the state the driver takes at its output step).
+    The driver starts from its own dt rule, 0.25 lx/nx (PFC/setup.f90:1376),
+    0.0164 at this box, which is above the CFL limit of this field, so the
+    run uses dt_flag = 1: dt adapted to CFL 0.95 at every first substep
+    (PFC/pass1.f90:202-250), which also makes every step wait for its CFL
+    number, as the reference does.
     Returns (steps/s, seconds of the window), None when the driver is not
     built, or a dict naming the failure."""
     import resource
@@ -197,7 +202,7 @@ def dropin_rate(cf, c, U0, steps=400, window=200):
                          c["aspect"], c["re"], c["dt"], 0.0)
         write_flow_dat(os.path.join(d, "flow.dat"), c["re"], c["lx_over_pi"], c["aspect"], c["nx"], c["ny"],
                        c["nz"], c["nx2"], c["nz2"], tend=steps, output_step=window, save_step=10 * steps,
-                       l2_step=10 * steps, read_vel=1, save_vel=0)
+                       l2_step=10 * steps, read_vel=1, save_vel=0, dt_flag=1)
 
         def big_stack():
             resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
@@ -409,6 +414,8 @@ def main():
         elif dr is not None:
             dropin = {"metric": "RK3 steps/s of the reference's own driver (main_mpi.f90 and its kept modules over "
                                 "the replacement modules), by its WTIME line",
+                      "config": "the headline field and grid, dt_flag 1 (CFL-adapted dt, the driver's fixed "
+                                "initial dt 0.25 lx/nx = 0.0164 being above the CFL limit of this field)",
                       "steps_per_s": dr[0], "window_s": dr[1], "window_steps": args.dropin_steps // 2,
                       "note": "the window includes the one copy of data1/data2/data4 to the host that the "
                               "driver's output step takes"}

@@ -249,8 +249,9 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * 9 planes per x-pass workgroup (2..64; 0, the default: the plane-rows of the
  * pass spread evenly over the workgroups resident at a time), 10 plane
  * ranges the nonlinear pass runs in turn (1..64; 1: every kernel over all
- * planes), 11 segmented column solve (1 on, 0 off) wherever ny+1 rows fit 16
- * segments of 5 to 9 rows (keys 3, 4 and 8 then do not apply).
+ * planes), 11 segmented column solve (1 on with 4 modes per wavefront, 2
+ * with 2 modes per wavefront, 0 off) wherever ny+1 rows fit 16 (32) segments
+ * of 5 to 9 (5) rows (keys 3, 4 and 8 then do not apply).
  * chan_autotune times `steps`
  * timesteps of every candidate, keeps the fastest and restores the state. */
 int chan_set_tuning(chan_ctx* ctx, int key, int value);
