@@ -260,6 +260,11 @@ __global__ __launch_bounds__(256) void k_les_tau_mlp(int ny, long ps, const doub
     const long cs = (long)nyp * ps;
     double* o = tau + (long)kk * ps + i;
     for (int t = 0; t < 6; ++t) o[t * cs] = fb ? ts[t] : tm[t];
+    if (kk == 1)  // the wall planes carry no stress
+      for (int t = 0; t < 6; ++t) {
+        tau[t * cs + i] = 0.0;
+        tau[t * cs + (long)ny * ps + i] = 0.0;
+      }
   }
   // fallback count of the block (one slot per block: no contended atomics)
   __shared__ unsigned int wc[4];
@@ -267,16 +272,6 @@ __global__ __launch_bounds__(256) void k_les_tau_mlp(int ny, long ps, const doub
   if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (unsigned int)__popcll(b);
   __syncthreads();
   if (threadIdx.x == 0) gated[(long)blockIdx.y * gridDim.x + blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
-}
-
-__global__ void k_les_zero_walls(int ny, long ps, double* __restrict__ tau) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ps) return;
-  const long cs = (long)(ny + 1) * ps;
-  for (int t = 0; t < 6; ++t) {
-    tau[t * cs + i] = 0.0;
-    tau[t * cs + (long)ny * ps + i] = 0.0;
-  }
 }
 
 }  // namespace
@@ -366,9 +361,6 @@ void les_compute(chan_ctx* c, LesState* L, bool dyn) {
   hipLaunchKernelGGL(k_les_tau_mlp, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->Y, L->scale, L->cs2,
                      L->delta2, L->beta, L->tau, L->gated);
   LAUNCH_CHECK("k_les_tau_mlp");
-  hipLaunchKernelGGL(k_les_zero_walls, dim3((unsigned)((L->ps + 255) / 256)), dim3(256), 0, c->stream, ny, L->ps,
-                     L->tau);
-  LAUNCH_CHECK("k_les_zero_walls");
 }
 
 LesState* les_need(chan_ctx* c) {

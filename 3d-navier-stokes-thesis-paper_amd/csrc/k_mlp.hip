@@ -96,35 +96,41 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ pac
   packed[blk * 64 + l] = to_bf16x4(w[0], w[1], w[2], w[3]);
 }
 
-// GELU for the bf16 inference path, without transcendental instructions:
-// GELU(x) = x/2 + h(x) with h(x) = x erf(x/sqrt 2)/2, an even function, taken
-// as a degree-10 polynomial in s = 2x^2/4.5^2 - 1 for |x| <= 4.5 (a
-// least-squares Chebyshev fit made for this kernel, converted to monomials in
-// s; error <= 2.4e-5 in float32) and GELU(x) = max(x, 0) beyond (error
-// <= 1.6e-5).  Both are far inside the 2^-9 relative rounding of the bf16
-// result the value feeds.
+// GELU for the bf16 inference path, without transcendental instructions or
+// branches: GELU(x) = x/2 + |x| e(t), e(t) = erf(t/sqrt 2)/2 at t = min(|x|, 4.5),
+// with e a degree-9 polynomial in u = 2t/4.5 - 1 (a least-squares Chebyshev
+// fit weighted by t, made for this kernel, converted to monomials in u).  In
+// float32 its GELU error is <= 6.4e-5 absolute for |x| <= 8 and below 1e-5
+// relative beyond (erf(4.5/sqrt 2) differs from 1 by 7e-6), far inside the
+// 2^-9 relative rounding of the bf16 result the value feeds.  11 VALU
+// operations per value (|x| is a source modifier).
 __device__ __forceinline__ float gelu_bf(float x) {
-  const float s = fminf(fmaf(x * x, 2.0f / 20.25f, -1.0f), 1.0f);
-  float h = -1.629356321e-02f;
-  h = fmaf(h, s, 3.052472295e-02f);
-  h = fmaf(h, s, -1.959422663e-02f);
-  h = fmaf(h, s, 3.656191228e-02f);
-  h = fmaf(h, s, -8.185058262e-02f);
-  h = fmaf(h, s, 1.049085769e-01f);
-  h = fmaf(h, s, -1.180014507e-01f);
-  h = fmaf(h, s, 1.458142942e-01f);
-  h = fmaf(h, s, -2.279429362e-01f);
-  h = fmaf(h, s, 8.071679794e-01f);
-  h = fmaf(h, s, 1.588666266e+00f);
-  const float g = fmaf(0.5f, x, h);
-  return (fabsf(x) > 4.5f) ? fmaxf(x, 0.0f) : g;
+  const float t = fminf(fabsf(x), 4.5f);
+  const float u = fmaf(t, 2.0f / 4.5f, -1.0f);
+  float e = 2.060715482e-02f;
+  e = fmaf(e, u, -3.009064123e-02f);
+  e = fmaf(e, u, -5.218758062e-02f);
+  e = fmaf(e, u, 1.301724166e-01f);
+  e = fmaf(e, u, -3.599490970e-02f);
+  e = fmaf(e, u, -1.570081264e-01f);
+  e = fmaf(e, u, 2.462200224e-01f);
+  e = fmaf(e, u, -1.808419824e-01f);
+  e = fmaf(e, u, 7.135432214e-02f);
+  e = fmaf(e, u, 4.877772331e-01f);
+  return fmaf(fabsf(x), e, 0.5f * x);
 }
 
+// GELU on a result tile (its bias already in the accumulator), rounded to
+// bf16 as the next B operand.
+__device__ __forceinline__ bf16x4 act_tile(f32x4 acc) {
+  return to_bf16x4(gelu_bf(acc[0]), gelu_bf(acc[1]), gelu_bf(acc[2]), gelu_bf(acc[3]));
+}
 
-// Bias + GELU on a result tile, rounded to bf16 as the next B operand.
-__device__ __forceinline__ bf16x4 act_tile(f32x4 acc, const float* __restrict__ bias, int ot, int lane) {
+// The bias of rows 4(lane/16)..+3 of output tile ot: the accumulator's
+// starting value, so the products add onto it.
+__device__ __forceinline__ f32x4 bias_tile(const float* __restrict__ bias, int ot, int lane) {
   const float* bb = bias + ot * 16 + 4 * (lane >> 4);
-  return to_bf16x4(gelu_bf(acc[0] + bb[0]), gelu_bf(acc[1] + bb[1]), gelu_bf(acc[2] + bb[2]), gelu_bf(acc[3] + bb[3]));
+  return f32x4{bb[0], bb[1], bb[2], bb[3]};
 }
 
 #define MLP_NT 3  // point tiles of 16 per wavefront (each weight operand read from LDS serves all of them)
@@ -163,10 +169,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
 #pragma unroll
     for (int ot = 0; ot < 8; ++ot) {
       const bf16x4 a = W[(MLP_BLK1 + ot) * 64 + lane];
+      const f32x4 b0 = bias_tile(bias, ot, lane);
 #pragma unroll
       for (int nt = 0; nt < MLP_NT; ++nt) {
-        acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[nt][0], f32x4{0, 0, 0, 0}, 0, 0, 0);
-        g[nt][ot] = act_tile(acc[nt][0], bias, ot, lane);
+        acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[nt][0], b0, 0, 0, 0);
+        g[nt][ot] = act_tile(acc[nt][0]);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -181,8 +188,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
       const float* bl = bias + 128 * (layer + 1);
 #pragma unroll
       for (int ot = 0; ot < 8; ++ot) {
+        const f32x4 b0 = bias_tile(bl, ot, lane);
 #pragma unroll
-        for (int nt = 0; nt < MLP_NT; ++nt) acc[nt][0] = f32x4{0, 0, 0, 0};
+        for (int nt = 0; nt < MLP_NT; ++nt) acc[nt][0] = b0;
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
           const bf16x8 a = W8[(blk0 + (ot * 4 + kb) * 2) * 32 + lane];
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
                                                                  acc[nt][0], 0, 0, 0);
         }
 #pragma unroll
-        for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[nt][0], bl, ot, lane);
+        for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[nt][0]);
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
@@ -201,9 +209,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
         for (int ot = 0; ot < 8; ++ot) h[nt][ot] = g[nt][ot];
     }
     // layer 4 (linear): rows 4q..4q+3 of the output tile are outputs, column = point
+    const f32x4 b4 = bias_tile(bias + 384, 0, lane);
 #pragma unroll
     for (int nt = 0; nt < MLP_NT; ++nt) {
-      f32x4 o = f32x4{0, 0, 0, 0};
+      f32x4 o = b4;
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb)
         o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W8[(MLP_BLK4 + kb * 2) * 32 + lane],
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
       if (pt < n)
         for (int i = 0; i < 4; ++i) {
           const int f = 4 * q + i;
-          if (f < 6) y[pt * 6 + f] = o[i] + bias[384 + f];
+          if (f < 6) y[pt * 6 + f] = o[i];
         }
     }
   }

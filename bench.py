@@ -276,7 +276,7 @@ def main():
                     help="steps of the 64x97x64 jet-control run (0: skip)")
     ap.add_argument("--les-spinup", type=float, default=40.0,
                     help="time units the DNS runs on before the live a-posteriori LES comparison")
-    ap.add_argument("--les-horizon", type=float, default=20.0,
+    ap.add_argument("--les-horizon", type=float, default=60.0,
                     help="time units of the live a-posteriori comparison (0: skip)")
     ap.add_argument("--les-steps", type=int, default=200,
                     help="RK3 steps of the learned-closure LES timed for the LES speed-up (0: skip)")
