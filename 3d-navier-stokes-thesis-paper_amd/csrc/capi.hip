@@ -600,12 +600,18 @@ void ensemble_plan_batched(chan_ensemble* e) {
       [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
         push_grid(it, k, (c->nxh + 7) / 8, 3 * c->nyp);
       }, zwork);
-  // x-pass: plane ranges from the resident workgroups of the whole batch
+  // x-pass: plane ranges from the resident workgroups of the whole batch.
+  // The launches of the L lengths run side by side and share the GPU, so
+  // together they make L waves of workgroups if each length's z-row groups x
+  // planes are spread over its resident workgroups; spreading them over L
+  // times as many keeps the workgroups short, so that the last ones to start
+  // leave a short tail (at the price of one halo plane per shorter range).
   std::map<int, long> rows;  // per length: z-row groups x planes of all its members
   for (chan_ctx* c : e->m) rows[c->gd.nx2 / 2] += (long)((c->gd.nz2 + 3) / 4) * (c->prm.ny - 1);
   std::map<int, int> chunk;
+  const long nlen = (long)rows.size();
   for (auto& kv : rows) {
-    const long res = xpass_batched_resident(kv.first, lin);
+    const long res = xpass_batched_resident(kv.first, lin) * nlen;
     chunk[kv.first] = (int)std::max(2L, (kv.second + res - 1) / res);
   }
   for (int w = 0; w < 2; ++w) {
@@ -673,19 +679,22 @@ void ensemble_plan_batched(chan_ensemble* e) {
 void ensemble_step_batched(chan_ensemble* e, hipStream_t s) {
   const int n = (int)e->m.size();
   const bool lin = e->lin != 0;
-  // one phase: its launches (one per length, disjoint members) side by side
+  // one phase: its launches (one per length, disjoint members) side by side,
+  // the longest length first (its workgroups are the longest), so that the
+  // shorter ones fill in behind it
   auto phase = [&](const std::vector<BatchLaunch>& v, auto launch) {
     if (v.size() <= 1 || e->bstreams.empty()) {
-      for (const BatchLaunch& b : v) launch(b, s);
+      for (size_t i = v.size(); i-- > 0;) launch(v[i], s);
       return;
     }
     CHAN_CHECK(hipEventRecord(e->bfork, s));
-    for (size_t i = 0; i < v.size(); ++i) {
-      hipStream_t bs = e->bstreams[i % e->bstreams.size()];
+    for (size_t j = 0; j < v.size(); ++j) {
+      const size_t i = v.size() - 1 - j;
+      hipStream_t bs = e->bstreams[j % e->bstreams.size()];
       CHAN_CHECK(hipStreamWaitEvent(bs, e->bfork, 0));
       launch(v[i], bs);
-      CHAN_CHECK(hipEventRecord(e->bjoin[i % e->bjoin.size()], bs));
-      CHAN_CHECK(hipStreamWaitEvent(s, e->bjoin[i % e->bjoin.size()], 0));
+      CHAN_CHECK(hipEventRecord(e->bjoin[j % e->bjoin.size()], bs));
+      CHAN_CHECK(hipStreamWaitEvent(s, e->bjoin[j % e->bjoin.size()], 0));
     }
   };
   launch_fill_zero_batched(e->fill_cbc, n, e->cbc_words, s);
@@ -1777,7 +1786,11 @@ int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int line
     const int ns = e->batched ? 0 : std::max(1, std::min(nstreams, n));
     CHAN_CHECK(hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking));
     if (e->batched) {
-      const int nb = std::min(n, 4);
+      // graph branches per phase (CHANFLOW_ENSEMBLE_BRANCHES, default 4, the
+      // hardware queues a process gets; 1: the launches of a phase in turn)
+      int nbr = 4;
+      if (const char* v = getenv("CHANFLOW_ENSEMBLE_BRANCHES")) nbr = std::max(1, std::min(8, atoi(v)));
+      const int nb = nbr > 1 ? std::min(n, nbr) : 0;
       e->bstreams.resize(nb, nullptr);
       e->bjoin.resize(nb, nullptr);
       CHAN_CHECK(hipEventCreateWithFlags(&e->bfork, hipEventDisableTiming));
