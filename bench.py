@@ -284,6 +284,8 @@ def main():
                     help="only run warmup + this many steps (for rocprof), print no JSON")
     ap.add_argument("--les-profile-steps", type=int, default=0,
                     help="only run this many steps of the learned-closure LES (for rocprof), print no JSON")
+    ap.add_argument("--ensemble-profile-steps", type=int, default=0,
+                    help="only run this many steps of the 48-member batched ensemble (for rocprof), print no JSON")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (a plain copy kernel "
                          "of known size, used to calibrate rocprof's FETCH_SIZE/WRITE_SIZE)")
@@ -306,6 +308,10 @@ def main():
     cf = load_pkg()
     import chanflow.synth as synth
     c = HEADLINE
+    if args.ensemble_profile_steps:
+        e_sps, _, _, _ = ensemble_rate(cf, synth, args.ensemble_profile_steps)
+        print(f"ensemble {args.ensemble_profile_steps} steps, {e_sps:.1f} steps/s", flush=True)
+        return
     if args.les_profile_steps:
         sps, fb = les_rate(cf, synth, c, args.les_profile_steps)
         print(f"LES {args.les_profile_steps} steps, {sps:.1f} steps/s, fallback fraction {fb:.3f}", flush=True)
