@@ -301,7 +301,7 @@ def main():
         print(f"copied {args.copy_mib} MiB x 5", flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "chanflow runs on one GPU only; --gpus must be 1 "
                           "(the multi-GPU sharding is a design note, not implemented)"}))
         sys.exit(2)
 
