@@ -122,7 +122,7 @@ __device__ __forceinline__ void zinv_body(const XZGeom& g, const double2* __rest
     }
   }
   __syncthreads();
-  if constexpr (NC > 0) fft2p_rec<NC, 1>(lds, ss, ZT_KX, tw + g.pz.tw, tid, nthr);
+  if constexpr (NC > 0) fft2p_rec<NC, 1>(lds, ss, ncol, tw + g.pz.tw, tid, nthr);  // the valid columns only
   else fft_run<NC, 1, FFT_MAXV>(lds, ss, ZT_KX, g.pz, tw, rl, tid, nthr);
   double2* dst = out + P * out_ps;
   for (int e = tid; e < nz2 * ZT_KX; e += nthr) {
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(32 * KX) void k_zinv2(XZGeom g, const double2* __re
     if (e < KX * NZ2) lds[s * SS + fft2_in_pos<NC>(t)] = v[i];
   }
   __syncthreads();
-  fft2p<NC, 1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
+  fft2p<NC, 1>(lds, SS, ncol, rl, tw + g.pz.tw, tid, NT);  // the valid columns only
   double2* dst = out + P * out_ps;
   for (int e = tid; e < NZ2 * KX; e += NT) {
     const int t = e / KX, s = e - t * KX;
@@ -433,7 +433,7 @@ __device__ __forceinline__ void zfwd2_body(const XZGeom& g, const double2* __res
     if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = ra[k];
   }
   __syncthreads();
-  fft2p<NC, -1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
+  fft2p<NC, -1>(lds, SS, ncol, rl, tw + g.pz.tw, tid, NT);  // the valid columns only
   const int h = nz / 2, off = NZ2 - nz;
   const double scale = 1.0 / (double)(g.nx2 * NZ2);
   auto fold = [&](const double2* a, int q) {
@@ -462,7 +462,7 @@ __device__ __forceinline__ void zfwd2_body(const XZGeom& g, const double2* __res
     if (e < NZ2 * KX) lds[s * SS + fft2_in_pos<NC>(t)] = rb[k];
   }
   __syncthreads();
-  fft2p<NC, -1>(lds, SS, KX, rl, tw + g.pz.tw, tid, NT);
+  fft2p<NC, -1>(lds, SS, ncol, rl, tw + g.pz.tw, tid, NT);  // the valid columns only
 #pragma unroll
   for (int k = 0; k < NOUT; ++k) {
     const int e = tid + NT * k;
