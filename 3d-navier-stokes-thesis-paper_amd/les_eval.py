@@ -264,6 +264,9 @@ def main():
                          "(default: dynamic Smagorinsky's own a-priori error)")
     ap.add_argument("--out", default="les_eval.json")
     ap.add_argument("--params-out", default="")
+    ap.add_argument("--calibrate", type=int, default=1,
+                    help="1: scale the network's output layer so that its gated mean dissipation matches the "
+                         "exact one on every 16th training snapshot")
     ap.add_argument("--les-dts", default="0.01,0.03",
                     help="comma-separated LES time steps for the a-posteriori runs; the first is the reported one")
     args = ap.parse_args()
@@ -294,7 +297,7 @@ def main():
     ntest = max(1, int(round(0.2 * args.nsnap)))
     ntrain = args.nsnap - ntest
     rng = np.random.default_rng(1)
-    Xs, Ts, held = [], [], []
+    Xs, Ts, held, calib = [], [], [], []
     placeholder = cf.Mlp()
     les.les_configure("learned", net=placeholder, beta=args.beta)   # the network inputs of the snapshots
     t0 = time.perf_counter()
@@ -312,6 +315,8 @@ def main():
             pick = ok & (rng.random(ok.shape[0]) < args.points)
             Xs.append(X[pick])
             Ts.append(T[pick].astype(np.float32))
+            if args.calibrate and s_ % 16 == 0:
+                calib.append((les.get_data1(), tau_x, Ph))
         else:
             held.append((les.get_data1(), tau_x, Ph))
         if s_ % 20 == 0:
@@ -327,6 +332,28 @@ def main():
                        "target_std": [float(v) for v in sd]}
     log(f"trained on {X.shape[0]} points in {t_train:.1f} s, losses {losses}")
     del X, T
+    if calib:
+        # dissipation calibration on training snapshots: the output layer is
+        # linear, so scaling it scales the learned stress; the gate depends on
+        # the scale, so the ratio is taken twice
+        off = 10 * 128 + 128 + 2 * (128 * 128 + 128)
+        scale = 1.0
+        for _ in range(2):
+            em, ex = [], []
+            for Ul, tau_x, Ph in calib:
+                les.les_configure("learned", net=net, beta=args.beta)
+                les.set_state(data1=Ul)
+                les.les_update()
+                a = apriori(les.les_tau(), tau_x, Ph, LES["ny"])
+                em.append(a["mean_dissipation_model"])
+                ex.append(a["mean_dissipation_exact"])
+            c = float(np.mean(ex) / np.mean(em))
+            p_ = net.get_params().astype(np.float64)
+            p_[off:off + 6 * 128 + 6] *= c
+            net.set_params(p_.astype(np.float32))
+            scale *= c
+            log(f"calibration: dissipation ratio {c:.4f}, output scale {scale:.4f}")
+        res["calibration"] = {"snapshots": len(calib), "output_scale": scale}
     if args.params_out:
         np.save(args.params_out, net.get_params())
 
