@@ -53,7 +53,7 @@ ENTRY_POINTS = (
     "chan_savevel_read_header", "chan_savevel_read", "chan_save_velocity", "chan_read_velocity",
     "chan_set_coordinates", "chan_l2_norm", "chan_mode_energy", "chan_plane_moments",
     "chan_set_tuning", "chan_get_tuning", "chan_autotune", "chan_ensemble_create", "chan_ensemble_step",
-    "chan_ensemble_invalidate", "chan_ensemble_destroy", "chan_ensemble_last_error",
+    "chan_ensemble_invalidate", "chan_ensemble_autotune", "chan_ensemble_destroy", "chan_ensemble_last_error",
     "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_num_params", "chan_mlp_set_params",
     "chan_mlp_get_params", "chan_mlp_infer", "chan_mlp_forward_fp32", "chan_mlp_gradient", "chan_mlp_train",
     "chan_vtk_write", "chan_write_vtk",
@@ -139,6 +139,7 @@ def load_library(path: str = LIBPATH):
         "chan_ensemble_create": (C.c_int, [C.c_int, C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]),
         "chan_ensemble_step": (C.c_int, [vp, C.c_int]),
         "chan_ensemble_invalidate": (C.c_int, [vp]),
+        "chan_ensemble_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
         "chan_ensemble_destroy": (C.c_int, [vp]),
         "chan_ensemble_last_error": (C.c_char_p, [vp]),
         "chan_mlp_create": (C.c_int, [C.POINTER(vp)]),
@@ -582,6 +583,15 @@ class Ensemble:
 
     def invalidate(self):
         self.lib.chan_ensemble_invalidate(self.h)
+
+    def autotune(self, steps=3):
+        """Batched ensembles: time each x-pass plane-range plan over `steps`
+        replayed steps (every member's state restored after each), keep the
+        fastest; returns its ms per step."""
+        ms = C.c_double(0.0)
+        if self.lib.chan_ensemble_autotune(self.h, steps, C.byref(ms)) != 0:
+            raise ChanError(self.lib.chan_ensemble_last_error(self.h).decode())
+        return ms.value
 
     def close(self):
         if getattr(self, "h", None):

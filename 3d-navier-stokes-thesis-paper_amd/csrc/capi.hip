@@ -9,6 +9,7 @@
 #include <mutex>
 #include <algorithm>
 #include "chan_internal.h"
+#include <array>
 
 namespace {
 
@@ -489,6 +490,9 @@ struct chan_ensemble {
   std::vector<hipStream_t> bstreams;
   std::vector<hipEvent_t> bjoin;
   hipEvent_t bfork = nullptr;
+  // x-pass plane ranges of the batched launches: each length's rows x planes
+  // spread over its resident workgroups times xmul (0: the number of lengths)
+  int xmul = 0;
 };
 
 namespace {
@@ -611,7 +615,7 @@ void ensemble_plan_batched(chan_ensemble* e) {
   std::map<int, int> chunk;
   const long nlen = (long)rows.size();
   for (auto& kv : rows) {
-    const long res = xpass_batched_resident(kv.first, lin) * nlen;
+    const long res = xpass_batched_resident(kv.first, lin) * (e->xmul > 0 ? (long)e->xmul : nlen);
     chunk[kv.first] = (int)std::max(2L, (kv.second + res - 1) / res);
   }
   for (int w = 0; w < 2; ++w) {
@@ -1856,6 +1860,99 @@ int chan_ensemble_invalidate(chan_ensemble* e) {
   if (!e) return 1;
   ensemble_release(e);
   return 0;
+}
+
+// Times `steps` replayed steps of each batched launch plan (the x-pass plane
+// ranges spread over 1, 2, 4 or 8 times each length's resident workgroups,
+// and the default, as many times as there are lengths), restoring every
+// member's state after each trial, and keeps the fastest.  The plans differ
+// only in how the x-pass planes are cut into workgroup ranges, so every
+// member's arithmetic is the same under each of them.
+int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
+  if (!e) return fail(nullptr, "null ensemble");
+  std::vector<void*> bk;
+  hipEvent_t a = nullptr, b = nullptr;
+  auto cleanup = [&] {
+    for (void* p : bk) dfree(p);
+    bk.clear();
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  };
+  try {
+    if (steps < 1) throw chan_error("steps must be >= 1");
+    if (!e->batched) throw chan_error("autotune applies to the batched launches (nstreams = 0); tune the members with chan_autotune");
+    for (chan_ctx* c : e->m) {
+      check_ops(c);
+      ensure_vel_in_vel(c);
+      c->h_dt_known = false;
+      CHAN_CHECK(hipStreamSynchronize(c->stream));
+    }
+    // every member's state: velocity, RK register, pressure, scalars, d_um_dy
+    auto sizes = [](chan_ctx* c) {
+      return std::array<size_t, 5>{vel_bytes(c), vel_bytes(c), p_bytes(c), sizeof(DevScalars),
+                                   sizeof(double) * (size_t)c->prm.ny};
+    };
+    auto ptrs = [](chan_ctx* c) {
+      return std::array<void*, 5>{c->d_vel, c->d_R, c->d_P, c->d_sc, c->d_dudy};
+    };
+    auto copy_state = [&](bool back) {
+      for (size_t i = 0; i < e->m.size(); ++i) {
+        chan_ctx* c = e->m[i];
+        const auto sz = sizes(c);
+        const auto p = ptrs(c);
+        char* q = static_cast<char*>(bk[i]);
+        for (int k = 0; k < 5; ++k) {
+          if (back) d2d(c, p[k], q, sz[k]);
+          else d2d(c, q, p[k], sz[k]);
+          q += sz[k];
+        }
+        CHAN_CHECK(hipStreamSynchronize(c->stream));
+      }
+    };
+    for (chan_ctx* c : e->m) {
+      const auto sz = sizes(c);
+      bk.push_back(dalloc(sz[0] + sz[1] + sz[2] + sz[3] + sz[4]));
+    }
+    CHAN_CHECK(hipEventCreate(&a));
+    CHAN_CHECK(hipEventCreate(&b));
+    const int deflt = e->xmul;
+    double best = 1e300;
+    int bestm = deflt;
+    copy_state(false);
+    for (int m : {0, 1, 2, 4, 8}) {
+      e->xmul = m;
+      ensemble_release(e);
+      ensemble_capture(e);
+      CHAN_CHECK(hipGraphLaunch(e->exec, e->master));  // first replay, untimed
+      CHAN_CHECK(hipEventRecord(a, e->master));
+      for (int t = 0; t < steps; ++t) CHAN_CHECK(hipGraphLaunch(e->exec, e->master));
+      CHAN_CHECK(hipEventRecord(b, e->master));
+      CHAN_CHECK(hipEventSynchronize(b));
+      float ms = 0.f;
+      CHAN_CHECK(hipEventElapsedTime(&ms, a, b));
+      bool ok = true;
+      for (chan_ctx* c : e->m) {
+        DevScalars tsc;
+        d2h(c, &tsc, c->d_sc, sizeof(tsc));
+        ok = ok && tsc.err == 0 && std::isfinite(tsc.cfl);
+      }
+      copy_state(true);
+      if (ok && ms / steps < best) { best = ms / steps; bestm = m; }
+    }
+    if (!(best < 1e300)) throw chan_error("autotune: every plan's trial steps failed the CFL check");
+    e->xmul = bestm;
+    ensemble_release(e);
+    cleanup();
+    if (best_ms) *best_ms = best;
+    return 0;
+  } catch (const chan_error& err) {
+    e->last_error = err.msg;
+  } catch (const std::exception& err) {
+    e->last_error = err.what();
+  }
+  cleanup();
+  ensemble_release(e);  // the next step plans and captures again
+  return 1;
 }
 
 int chan_ensemble_destroy(chan_ensemble* e) {

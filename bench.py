@@ -143,7 +143,8 @@ def ensemble_rate(cf, synth, steps, n=48, seed=3):
     (Nx, Nz in {32, 48, 64, 96}, Ny in {49, 65, 97}, Re_c uniform in
     [2000, 3500], box 2pi x 2 x pi, dt 0.005, laminar + 1e-2 perturbation),
     advanced together by one captured graph per step whose every phase is one
-    launch per transform length over all members.  Returns (ensemble steps/s,
+    launch per transform length over all members, its launch plan chosen by
+    chan_ensemble_autotune.  Returns (ensemble steps/s,
     grid-point steps/s summed over the members, grid points, algorithmic
     bytes per ensemble step = sum over members of 960 N_c)."""
     import math
@@ -161,6 +162,7 @@ def ensemble_rate(cf, synth, steps, n=48, seed=3):
         pts += nx * ny * nz
         byts += 960 * nz * (nx // 2 + 1) * (ny + 1)
     ens = cf.Ensemble(mem)
+    ens.autotune(steps=3)            # the x-pass plane-range plan, every member's state restored
     ens.step(3)
     for ch in mem:
         ch.sync()
@@ -405,7 +407,8 @@ def main():
         ensemble = {"metric": "RK3 steps/s of a 48-member batched ensemble (one captured graph per step)",
                     "config": "48 channels from seed 3: Nx, Nz in {32,48,64,96}, Ny in {49,65,97}, Re_c "
                               "uniform in [2000,3500], box 2pi x 2 x pi, 3/2 dealiased, dt 0.005; every phase "
-                              "of a substage one launch per transform length over all members",
+                              "of a substage one launch per transform length over all members; launch plan "
+                              "autotuned (chan_ensemble_autotune)",
                     "ensemble_steps_per_s": e_sps, "grid_point_steps_per_s": e_pts, "grid_points": npts,
                     "roofline": {"bound": "hbm", "achieved": e_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": e_gbs / HBM_PEAK_GBS, "bytes_per_step": e_bytes},

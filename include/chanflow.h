@@ -278,6 +278,13 @@ typedef struct chan_ensemble chan_ensemble;
 int chan_ensemble_create(int n, chan_ctx* const* members, int nstreams, int linearized, chan_ensemble** out);
 int chan_ensemble_step(chan_ensemble* ens, int nsteps);
 int chan_ensemble_invalidate(chan_ensemble* ens);
+/* Launch-plan tuning of a batched ensemble (nstreams = 0; no counterpart in
+ * the reference): times `steps` replayed steps of each x-pass plane-range
+ * plan (each length's planes cut for 1, 2, 4 or 8 times, or as many times as
+ * there are lengths, its resident workgroups), restores every member's state
+ * after each trial, keeps the fastest and returns its time per step.  The
+ * plans give every member the same arithmetic. */
+int chan_ensemble_autotune(chan_ensemble* ens, int steps, double* best_ms_per_step);
 int chan_ensemble_destroy(chan_ensemble* ens);
 const char* chan_ensemble_last_error(const chan_ensemble* ens);
 

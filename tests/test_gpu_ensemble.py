@@ -130,6 +130,38 @@ def test_batched_ensemble_48_members():
         ch.close()
 
 
+def test_batched_ensemble_autotune_keeps_state_and_results():
+    """chan_ensemble_autotune on 12 members of the seeded draw: every member's
+    state is the same after the tuning as before it (bit for bit), and the
+    tuned ensemble's next two steps equal those of an untuned twin ensemble
+    (the plans differ only in the x-pass plane ranges)."""
+    cf = __import__("helpers").chanflow()
+    draws = synth().ensemble_draw(12, seed=5)
+    mem = [_draw_member(d) for d in draws]
+    twins = [_draw_member(d) for d in draws]
+    ens = cf.Ensemble([m[2] for m in mem])
+    ref = cf.Ensemble([m[2] for m in twins])
+    before = [m[2].get_state() for m in mem]
+    ms = ens.autotune(steps=2)
+    assert ms > 0.0
+    for (_, _, ch), st in zip(mem, before):
+        for x, y in zip(ch.get_state(), st):
+            assert np.array_equal(x, y)
+    ens.step(2)
+    ref.step(2)
+    for (g, _, a), (_, _, b) in zip(mem, twins):
+        for x, y in zip(a.get_state(), b.get_state()):
+            assert np.array_equal(x, y), (g.nx, g.ny, g.nz)
+    streams = cf.Ensemble([m[2] for m in mem[:2]], nstreams=1)
+    with pytest.raises(cf.ChanError):
+        streams.autotune(steps=1)
+    streams.close()
+    ens.close()
+    ref.close()
+    for _, _, ch in mem + twins:
+        ch.close()
+
+
 def test_batched_ensemble_rejects_ineligible_member():
     cf = __import__("helpers").chanflow()
     g, _, a = _member(SHAPES[0], 40)
