@@ -1872,6 +1872,7 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
   if (!e) return fail(nullptr, "null ensemble");
   std::vector<void*> bk;
   hipEvent_t a = nullptr, b = nullptr;
+  const int xmul0 = e->xmul;  // kept if the tuning fails
   auto cleanup = [&] {
     for (void* p : bk) dfree(p);
     bk.clear();
@@ -1915,9 +1916,8 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
     }
     CHAN_CHECK(hipEventCreate(&a));
     CHAN_CHECK(hipEventCreate(&b));
-    const int deflt = e->xmul;
     double best = 1e300;
-    int bestm = deflt;
+    int bestm = xmul0;
     copy_state(false);
     for (int m : {0, 1, 2, 4, 8}) {
       e->xmul = m;
@@ -1951,6 +1951,7 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
     e->last_error = err.what();
   }
   cleanup();
+  e->xmul = xmul0;
   ensemble_release(e);  // the next step plans and captures again
   return 1;
 }
