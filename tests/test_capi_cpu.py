@@ -67,6 +67,18 @@ def test_null_context_is_rejected():
     assert lib.chan_destroy(None) == 0
 
 
+def test_null_ensemble_is_rejected():
+    """Ensemble entry points refuse a null ensemble before any device call."""
+    import ctypes as C
+    lib = chanflow().load_library()
+    ms = C.c_double(-1.0)
+    assert lib.chan_ensemble_step(None, 1) != 0
+    assert lib.chan_ensemble_autotune(None, 2, C.byref(ms)) != 0
+    assert ms.value == -1.0
+    assert lib.chan_ensemble_invalidate(None) != 0
+    assert lib.chan_ensemble_destroy(None) == 0
+
+
 C_CLIENT = r"""
 #include <stdio.h>
 #include <string.h>
