@@ -9,10 +9,16 @@ k^-5/3 fluctuations of rms 0.1.  One "step" = boundary + three
 
 Prints one JSON line.  `value` is RK3 steps/s of the whole job (one GPU).
 `roofline` is computed for the dominant part of the step from HIP events
-recorded on the library's stream; `cpu_baseline` times the numpy oracle on a
-bounded sample (2 steps) of the same workload on one host core.
+recorded on the library's stream; `fft_fp64` is the step's FFT flop rate
+against the FP64 vector peak; `counters` quotes the committed rocprofv3
+counter summary; `cpu_baseline` times the numpy oracle on a bounded sample
+(3 steps) of the same workload on one host core.  Beside the headline: the
+learned-closure LES speed-up with a live a-posteriori comparison, the
+48-member batched ensemble, the reference's own driver over the library and
+the jet-control run (progress lines on stderr).
 """
 import argparse
+import math
 import importlib.util
 import json
 import os
@@ -25,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "3d-navier-stokes-thesis-paper_amd")
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
 def load_pkg():
@@ -367,6 +373,12 @@ def main():
         name, dur_ms, bytes_launch = dom, per_launch[dom], alg[dom]
     achieved = bytes_launch / (dur_ms * 1e-3) / 1e9
     step_gbs = alg_step * steps_per_s / 1e9
+    # FFT flops of a substep, the FP64 utilisation figure: 12 real x-transforms
+    # of length nx2 per z-row and 9 complex z-transforms of length nz2 per kx
+    # column, per plane, at 2.5 N log2 N (real) and 5 N log2 N (complex)
+    nx2, nz2 = c["nx2"], c["nz2"]
+    fft_sub = nyp * (12 * nz2 * 2.5 * nx2 * math.log2(nx2) + 9 * nxh * 5 * nz2 * math.log2(nz2))
+    fft_tflops = 3 * fft_sub * steps_per_s / 1e12
     traffic, traffic_src = measured_traffic(name)
 
     note(f"headline: {steps_per_s:.1f} steps/s")
@@ -461,6 +473,8 @@ def main():
                                "192x129x192, 3/2 dealiased to 288x288, dt=0.005, constant mass flux",
                    "nx": 192, "ny": 129, "nz": 192, "nx2": 288, "nz2": 288, "dt": 0.005},
         "grid_point_steps_per_s": steps_per_s * c["nx"] * c["ny"] * c["nz"],
+        "fft_fp64": {"flops_per_step": 3 * fft_sub, "achieved": fft_tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": fft_tflops / FP64_PEAK_TFLOPS},
         "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_step": alg_step},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
