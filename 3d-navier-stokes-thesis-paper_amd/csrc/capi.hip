@@ -165,7 +165,6 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
   const int nyp = c->nyp, ny = c->prm.ny;
   const long zps = (long)g.nz2 * g.kxpad;
   if (lin && !c->d_base) throw chan_error("base flow not set (chan_set_baseflow)");
-  if (cfl) launch_fill_zero(c->d_cfl, sizeof(unsigned long long) * 3 * nyp, c->stream);
   const double* tau = nullptr;
   if (!lin && c->les) {
     KTimer t(c, KT_OTHER);
@@ -481,9 +480,8 @@ struct chan_ensemble {
   std::vector<BatchLaunch> zinv, zfwd, xpass[2], solve[3];  // xpass[1]: with the CFL maxima; solve[rk-1]
   CflBatchArgs* cfl_args = nullptr;
   FillBatchArgs* fill_cbc = nullptr;
-  FillBatchArgs* fill_cfl = nullptr;
   DevScalars** scs = nullptr;
-  size_t cbc_words = 0, cfl_words = 0;
+  size_t cbc_words = 0;
   std::vector<void*> dmem;     // device tables of the batched launches
   // the launches of one phase touch disjoint members: they run as parallel
   // branches of the graph (fork from the master stream, join back)
@@ -659,20 +657,17 @@ void ensemble_plan_batched(chan_ensemble* e) {
         [](chan_ctx* c) { return (long)c->nmodes * c->nyp; });
   }
   std::vector<CflBatchArgs> cf;
-  std::vector<FillBatchArgs> fc, fl;
+  std::vector<FillBatchArgs> fc;
   std::vector<DevScalars*> sc;
   for (chan_ctx* c : e->m) {
     cf.push_back(CflBatchArgs{c->ops, c->prm, c->d_cfl, c->d_sc});
-    const size_t cw = 6 * (size_t)nmodes(c) * sizeof(double2) / 8, lw = 3 * (size_t)c->nyp;
+    const size_t cw = 6 * (size_t)nmodes(c) * sizeof(double2) / 8;
     fc.push_back(FillBatchArgs{(unsigned long long*)c->d_cbc, cw});
-    fl.push_back(FillBatchArgs{c->d_cfl, lw});
     e->cbc_words = std::max(e->cbc_words, cw);
-    e->cfl_words = std::max(e->cfl_words, lw);
     sc.push_back(c->d_sc);
   }
   e->cfl_args = (CflBatchArgs*)upload_table(e, cf);
   e->fill_cbc = (FillBatchArgs*)upload_table(e, fc);
-  e->fill_cfl = (FillBatchArgs*)upload_table(e, fl);
   e->scs = (DevScalars**)upload_table(e, sc);
 }
 
@@ -703,7 +698,6 @@ void ensemble_step_batched(chan_ensemble* e, hipStream_t s) {
   };
   launch_fill_zero_batched(e->fill_cbc, n, e->cbc_words, s);
   for (int rk = 1; rk <= 3; ++rk) {
-    if (rk == 1) launch_fill_zero_batched(e->fill_cfl, n, e->cfl_words, s);
     phase(e->zinv, [&](const BatchLaunch& b, hipStream_t st) {
       launch_zinv_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, st);
     });
@@ -838,6 +832,9 @@ int chan_create(const chan_params* prm, chan_ctx** out) {
     CHAN_CHECK(hipMemset(c->d_P, 0, p_bytes(c)));
     CHAN_CHECK(hipMemset(c->d_cbc, 0, sizeof(double2) * 6 * nm));
     CHAN_CHECK(hipMemset(c->d_dudy, 0, sizeof(double) * p.ny));
+    // the CFL maxima start at zero here; the finalize kernel clears them
+    // after each read
+    CHAN_CHECK(hipMemset(c->d_cfl, 0, sizeof(unsigned long long) * 3 * nyp));
     DevScalars sc{};
     sc.dt = 0.25 * p.lx / (double)p.nx;       // PFC/setup.f90:1377
     sc.p_grad = -2.0 / p.re;                  // PFC/setup.f90:1384

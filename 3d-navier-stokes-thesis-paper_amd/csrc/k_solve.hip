@@ -1682,9 +1682,10 @@ void launch_subtract_grad_cols(const DevOps& ops, int ny, double2* crhs, const d
 // CFL number and dt adaptation (check_cfl PFC/pass1.f90:202-250, :106-124)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cfl_finalize_body(const DevOps& ops, const chan_params& prm,
-                                                  const unsigned long long* cfl, DevScalars* sc, int step) {
+                                                  unsigned long long* cfl, DevScalars* sc, int step) {
   // one wavefront: lane l takes the planes k = 2 + l, 2 + l + 64, ...; the
-  // maximum and the abort flag do not depend on the order of the planes
+  // maximum and the abort flag do not depend on the order of the planes.
+  // Each lane clears the maxima it has read, so the next pass starts from zero.
   const int lane = threadIdx.x;
   const int ny = prm.ny;
   const double dt = sc->dt;
@@ -1695,6 +1696,9 @@ __device__ __forceinline__ void cfl_finalize_body(const DevOps& ops, const chan_
     double amax = __longlong_as_double((long long)cfl[3 * kk + 0]);
     double vmax = __longlong_as_double((long long)cfl[3 * kk + 1]);
     double wmax = __longlong_as_double((long long)cfl[3 * kk + 2]);
+    cfl[3 * kk + 0] = 0ull;
+    cfl[3 * kk + 1] = 0ull;
+    cfl[3 * kk + 2] = 0ull;
     double tempa = 0.5 * amax * (double)prm.nx / (0.55 * prm.lx);
     double tempb = vmax / (1.73 * fmin(ops.hn[k - 1], ops.hn[k - 2]));
     double tempc = 0.5 * wmax * (double)prm.nz / (0.55 * prm.lz);
@@ -1718,7 +1722,7 @@ __device__ __forceinline__ void cfl_finalize_body(const DevOps& ops, const chan_
 }
 
 __global__ __launch_bounds__(64) void k_cfl_finalize(DevOps ops, chan_params prm,
-                                                     const unsigned long long* cfl, DevScalars* sc,
+                                                     unsigned long long* cfl, DevScalars* sc,
                                                      int step) {
   cfl_finalize_body(ops, prm, cfl, sc, step);
 }
