@@ -266,7 +266,7 @@ class Channel:
         return d1, d2, d4
 
     # -- launch-shape tuning
-    TUNING_KEYS = ("xpass_wave", "xpass_threads", "zfwd_v2", "solve_lanes", "solve_bl", "zinv_kx", "zfwd_kx", "xpass_prefetch", "solve_dma", "xpass_chunk", "nl_chunks", "solve_seg")
+    TUNING_KEYS = ("xpass_wave", "xpass_threads", "zfwd_v2", "solve_lanes", "solve_bl", "zinv_kx", "zfwd_kx", "xpass_prefetch", "solve_dma", "xpass_chunk", "nl_chunks", "solve_seg", "xpass_rows")
 
     def set_tuning(self, key, value):
         k = self.TUNING_KEYS.index(key) if isinstance(key, str) else int(key)

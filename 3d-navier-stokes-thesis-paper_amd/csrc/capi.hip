@@ -177,7 +177,7 @@ void nl_pass(chan_ctx* c, int step, int rk, bool lin, bool cfl) {
     a.cflbits = cfl ? c->d_cfl : nullptr;
     a.ny = ny; a.ka = ka; a.kb = kb; a.chunk = c->tune.xpass_chunk; a.rz = 0;
     a.wave = c->tune.xpass_wave; a.maxthr = c->tune.xpass_threads; a.prefetch = c->tune.xpass_prefetch;
-    a.tau = tau;
+    a.tau = tau; a.rows = c->tune.xpass_rows;
     launch_xpass(g, c->d_tw, c->ops, a, c->stream);
   };
   const int nch = std::max(1, std::min(c->tune.nl_chunks, ny - 1));
@@ -374,7 +374,7 @@ void les_prepare_state(chan_ctx* c) { ensure_vel_in_vel(c); }
 // ---------------------------------------------------------------------------
 // Tuning keys: 0 xpass_wave, 1 xpass_threads, 2 zfwd_v2, 3 solve_lanes, 4 solve_bl,
 // 5 zinv_kx, 6 zfwd_kx, 7 xpass_prefetch, 8 solve_dma, 9 xpass_chunk, 10 nl_chunks,
-// 11 solve_seg.
+// 11 solve_seg, 12 xpass_rows.
 Tuning tuning_defaults() {
   Tuning t;
   auto env = [](const char* n, int& v) {
@@ -393,6 +393,7 @@ Tuning tuning_defaults() {
   v = t.xpass_chunk; env("CHANFLOW_XPASS_CHUNK", v); tuning_set(t, 9, v);
   v = t.nl_chunks; env("CHANFLOW_NL_CHUNKS", v); tuning_set(t, 10, v);
   v = t.solve_seg; env("CHANFLOW_SOLVE_SEG", v); tuning_set(t, 11, v);
+  v = t.xpass_rows; env("CHANFLOW_XPASS_ROWS", v); tuning_set(t, 12, v);
   return t;
 }
 
@@ -410,6 +411,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 9: if (v != 0 && (v < 2 || v > 64)) return false; t.xpass_chunk = v; return true;
     case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
     case 11: if (v < 0 || v > 2) return false; t.solve_seg = v; return true;
+    case 12: if (v != 0 && v != 1 && v != 2 && v != 4) return false; t.xpass_rows = v; return true;
     default: return false;
   }
 }
@@ -428,6 +430,7 @@ int tuning_get(const Tuning& t, int key) {
     case 9: return t.xpass_chunk;
     case 10: return t.nl_chunks;
     case 11: return t.solve_seg;
+    case 12: return t.xpass_rows;
     default: return -1;
   }
 }
@@ -459,6 +462,7 @@ FFTPlan make_plan(int n, int tw_offset) {
 // length) and the work list that covers all their workgroups.
 struct BatchLaunch {
   int len = 0;                 // nz2 (z-transforms), nx2/2 (x-pass) or rows per segment (solve)
+  int rw = 1, nxh = 0;         // x-pass: z-rows per wavefront, largest nx/2+1 of the members
   void* args = nullptr;        // device array of per-channel arguments
   BatchItem* items = nullptr;  // device work list
   int nitems = 0;
@@ -608,12 +612,23 @@ void ensemble_plan_batched(chan_ensemble* e) {
   // planes are spread over its resident workgroups; spreading them over L
   // times as many keeps the workgroups short, so that the last ones to start
   // leave a short tail (at the price of one halo plane per shorter range).
+  // Each length takes the fewest z-rows per wavefront of its members' choices
+  // (tuning key 12; 1, 2 or 4 each divide the larger), so that every member's
+  // nz2 is a multiple of it.
+  std::map<int, int> xrw, xnxh;  // per length: rows per wavefront, largest nx/2+1
+  for (chan_ctx* c : e->m) {
+    const int M = c->gd.nx2 / 2, r = xpass_rows(M, c->gd.nz2, c->tune.xpass_rows, M <= 144, true);
+    xrw[M] = xrw.count(M) ? std::min(xrw[M], r) : r;
+    xnxh[M] = std::max(xnxh[M], c->nxh);
+  }
+  auto xgroups = [&](chan_ctx* c) { return (c->gd.nz2 / xrw[c->gd.nx2 / 2] + 3) / 4; };
   std::map<int, long> rows;  // per length: z-row groups x planes of all its members
-  for (chan_ctx* c : e->m) rows[c->gd.nx2 / 2] += (long)((c->gd.nz2 + 3) / 4) * (c->prm.ny - 1);
+  for (chan_ctx* c : e->m) rows[c->gd.nx2 / 2] += (long)xgroups(c) * (c->prm.ny - 1);
   std::map<int, int> chunk;
   const long nlen = (long)rows.size();
   for (auto& kv : rows) {
-    const long res = xpass_batched_resident(kv.first, lin) * (e->xmul > 0 ? (long)e->xmul : nlen);
+    const long res = xpass_batched_resident(kv.first, lin, xrw[kv.first], xnxh[kv.first]) *
+                     (e->xmul > 0 ? (long)e->xmul : nlen);
     chunk[kv.first] = (int)std::max(2L, (kv.second + res - 1) / res);
   }
   for (int w = 0; w < 2; ++w) {
@@ -628,13 +643,18 @@ void ensemble_plan_batched(chan_ensemble* e) {
           a.ny = c->prm.ny; a.ka = 2; a.kb = c->prm.ny;
           a.chunk = std::min(chunk[c->gd.nx2 / 2], c->prm.ny - 1);
           a.rz = 0; a.wave = 1; a.maxthr = c->tune.xpass_threads; a.prefetch = 1; a.tau = nullptr;
+          a.rows = xrw[c->gd.nx2 / 2];
           return x;
         },
         [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
           const int np = c->prm.ny - 1, ch = std::min(chunk[c->gd.nx2 / 2], np);
-          push_grid(it, k, (c->gd.nz2 + 3) / 4, (np + ch - 1) / ch);
+          push_grid(it, k, xgroups(c), (np + ch - 1) / ch);
         },
         [](chan_ctx* c) { return (long)c->gd.nz2 * c->gd.nx2 * c->nyp; });
+    for (BatchLaunch& b : e->xpass[w]) {
+      b.rw = xrw[b.len];
+      b.nxh = xnxh[b.len];
+    }
   }
   // the segmented solve, one table per substep
   for (int rk = 1; rk <= 3; ++rk) {
@@ -702,7 +722,7 @@ void ensemble_step_batched(chan_ensemble* e, hipStream_t s) {
       launch_zinv_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, st);
     });
     phase(e->xpass[rk == 1 ? 1 : 0], [&](const BatchLaunch& b, hipStream_t st) {
-      launch_xpass_batched(b.len, lin, (const XBatchArgs*)b.args, b.items, b.nitems, st);
+      launch_xpass_batched(b.len, lin, b.rw, b.nxh, (const XBatchArgs*)b.args, b.items, b.nitems, st);
     });
     phase(e->zfwd, [&](const BatchLaunch& b, hipStream_t st) {
       launch_zfwd_batched(b.len, (const ZBatchArgs*)b.args, b.items, b.nitems, st);

@@ -86,7 +86,12 @@ struct XPassArgs {
   int prefetch;           // 1: direct global-to-LDS prefetch of the next plane's rows
   const double* tau;      // subgrid stress [6][ny+1][nz2][nx2] at the cell centres yh
                           // (11, 12, 13, 22, 23, 33; zero on the wall planes) or nullptr
+  int rows;               // z-rows per wavefront of the wave kernel (0: by length; 1, 2, 4)
 };
+
+// LDS cells of one raw row of the wave x-pass's prefetch: nxh rounded up to
+// the 64 cells one direct global-to-LDS load writes.
+__host__ __device__ inline int xpass_raw_cells(int nxh) { return ((nxh + 63) / 64) * 64; }
 
 // Plane of a z-transform workgroup: grid row y covers planes off .. off+cnt-1
 // of each field, fields `stride` planes apart (identity: cnt = nplanes, stride = 0).
@@ -139,6 +144,7 @@ struct Tuning {
   int xpass_chunk = 0;    // planes per x-pass workgroup (2..64; 0: from the resident workgroups)
   int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
   int solve_seg = 1;      // segmented column solve (columns in registers, no scratch); 2: two modes per wavefront
+  int xpass_rows = 0;     // z-rows per wavefront of the wave x-pass (0: by length, 1, 2, 4)
 };
 Tuning tuning_defaults();
 bool tuning_set(Tuning& t, int key, int value);
@@ -228,9 +234,13 @@ void launch_zfwd_batched(int nz2, const ZBatchArgs* args, const BatchItem* items
 void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const XPassArgs& a,
                   hipStream_t s);
 bool xpass_batchable(int M);
-void launch_xpass_batched(int M, bool lin, const XBatchArgs* args, const BatchItem* items, int nitems,
-                          hipStream_t s);
-long xpass_batched_resident(int M, bool lin);
+// z-rows per wavefront of the wave x-pass: `req` (1, 2, 4) where the length
+// has that kernel and nz2 is a multiple of it, otherwise the default of a
+// single or a batched launch
+int xpass_rows(int M, int nz2, int req, bool pf, bool batched);
+void launch_xpass_batched(int M, bool lin, int rw, int nxh, const XBatchArgs* args, const BatchItem* items,
+                          int nitems, hipStream_t s);
+long xpass_batched_resident(int M, bool lin, int rw, int nxh);
 // (k_solve.hip)
 struct SolveArgs {
   const double2* nl;  // [3][ny+1][nxh][nz]

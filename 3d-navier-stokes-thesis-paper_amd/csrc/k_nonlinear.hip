@@ -254,73 +254,90 @@ __global__ __launch_bounds__(640) void k_xpass(XZGeom g, const double2* __restri
 
 // ---------------------------------------------------------------------------
 // Wave-centric x-pass for the compile-time lengths M <= 192.  A wavefront
-// owns one z-row and walks a chunk of planes on its own: lane l holds the
-// physical point pairs n = l + 64q (q < PPL) of the row, and the x transforms
-// of the row run in the wavefront's private LDS slice (four sequences),
-// ordered with wavefront-scope fences only.  The eight forward fields go
-// through the slice in two batches of four: {u^2, d/dy flux_u, w u,
-// d/dy flux_w} give Au, Bu, Aw, and {w^2, v-interp u, d/dy flux_v,
-// v-interp w} give Bw, Av, Bv.  The four wavefronts of a workgroup share
-// only the root table.
+// owns RW consecutive z-rows and walks a chunk of planes on its own: the
+// RW*M physical point pairs of its rows are spread over the lanes (lane l
+// holds the pairs v = l + 64q, q < PPL; pair v is point n = v mod M of row
+// v / M), and the x transforms of the rows run in the wavefront's private
+// LDS slice, ordered with wavefront-scope fences only.  The eight forward
+// fields go through the slice in two batches of four per row: {u^2, d/dy
+// flux_u, w u, d/dy flux_w} give Au, Bu, Aw, and {w^2, v-interp u, d/dy
+// flux_v, v-interp w} give Bw, Av, Bv.  Sequence (f, r) of a batch, field f
+// of row r, sits at f*RW + r.  Short rows take RW > 1: one row of 36 pairs
+// leaves 28 of 64 lanes idle in every pointwise step, and its 4-sequence
+// transform batches give the lanes 24 tasks.  The four wavefronts of a
+// workgroup share only the root table.
 // ---------------------------------------------------------------------------
 //
 // With PF the raw rows of the next plane are streamed into a per-wavefront
 // LDS buffer by direct global-to-LDS loads (global_load_lds_dwordx4: lane i
 // writes cells base + i) as soon as the current rows have been consumed, so
 // their latency overlaps a whole plane of work instead of stalling the
-// wavefront at the start of every plane.
-// (bx, by): the workgroup's group of four z-rows and its plane range
+// wavefront at the start of every plane.  A raw row takes nxh cells (M + 1
+// with one row per wavefront) rounded up to 64.
+// (bx, by): the workgroup's group of 4*RW z-rows and its plane range
 // (blockIdx, or a work-list entry of the batched form)
-template <bool LIN, int M, bool PF, bool LES = false>
+template <bool LIN, int M, bool PF, bool LES = false, int RW = 1>
 __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __restrict__ tw, const DevOps& ops,
                                              const XPassArgs& a, int bx, int by) {
   extern __shared__ double2 lds[];
-  constexpr int SEQ = fft_seq_cells(M), PPL = (M + 63) / 64;
-  constexpr int RAWN = ((M + 1 + 63) / 64) * 64;          // cells per raw row (nx/2+1 <= M+1)
-  constexpr int WSZ = 4 * SEQ + (PF ? 3 * RAWN : 0);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int SEQ = fft_seq_cells(M), NP = RW * M, PPL = (NP + 63) / 64;
+  constexpr int FS = RW * SEQ;                      // cells between the fields of a batch
+  // the wavefront index is uniform: as a scalar it keeps the row and plane
+  // offsets on the scalar unit
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
+  // one row per wavefront keeps the raw-row stride a constant (M + 1 cells
+  // rounded up), so the LDS offsets of its three raw rows fold into the reads
+  const int rawn = RW == 1 ? xpass_raw_cells(M + 1) : xpass_raw_cells(nxh);
   double2* rl = lds;
   load_tw2<M>(rl, tw + g.px.tw, tid, 256);
   __syncthreads();
-  const int zz = bx * 4 + wv;
-  if (zz >= g.nz2) return;  // after the only workgroup barrier
-  double2* W = lds + M + wv * WSZ;
-  double2* RAW = W + 4 * SEQ;
+  const int zb = (bx * 4 + wv) * RW;  // first z-row of the wavefront
+  if (zb >= nz2) return;  // after the only workgroup barrier
+  const long rowoff = (long)zb * kxp, prow = (long)zb * g.nx2;
+  double2* W = lds + M + wv * (4 * FS + (PF ? 3 * RW * rawn : 0));
+  double2* RAW = W + 4 * FS;
   const double2* rg = tw + g.px.tw;
   const double2* rootx = tw + g.twx;
   const int ny = a.ny, nyp = ny + 1;
   const int ka = a.ka + by * a.chunk;
   const int kb = min(a.kb, ka + a.chunk - 1);
-  const int nz2 = g.nz2, kxp = g.kxpad, nxh = g.nxh;
-  const long rowoff = (long)zz * kxp;
   const bool hasbuf = ops.buffer != nullptr;
 
+  // pair q of the lane: point nq[q] of row rq(q); pin/pout carry the row's
+  // sequence offset
   int nq[PPL], pin[PPL], pout[PPL];
   bool vq[PPL];
   double2 wc[PPL];
+  auto rq = [&](int q) { return RW == 1 ? 0 : (vq[q] ? (lane + 64 * q) / M : 0); };
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
-    nq[q] = lane + 64 * q;
-    vq[q] = nq[q] < M;
-    const int n = vq[q] ? nq[q] : 0;
+    const int v = lane + 64 * q;
+    vq[q] = v < NP;
+    const int vv = vq[q] ? v : 0;
+    const int r = RW == 1 ? 0 : vv / M, n = vv - r * M;
+    nq[q] = n;
     wc[q] = cconj(rootx[n]);
-    pin[q] = fft2_in_pos<M>(n);
-    pout[q] = fft2_out_pos<M>(n);
+    pin[q] = r * SEQ + fft2_in_pos<M>(n);
+    pout[q] = r * SEQ + fft2_out_pos<M>(n);
   }
 
-  // raw rows of plane p (1-based) into RAW[c][0..nxh) (PF only); every
+  // raw rows of plane p (1-based) into RAW[c*RW + r][0..nxh) (PF only); every
   // global address is clamped to an element of the row
   auto fetch = [&](int p) {
     if constexpr (PF) {
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): earlier reads of RAW are done
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
-        for (int i = 0; 64 * i < nxh; ++i) {
-          const int idx = min(64 * i + lane, nxh - 1);
-          __builtin_amdgcn_global_load_lds((const void*)(row + idx),
-                                           (__attribute__((address_space(3))) void*)(RAW + c * RAWN + 64 * i),
-                                           16, 0, 0);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const double2* row = a.Z + ((long)(c * nyp + (p - 1)) * nz2 + r) * kxp + rowoff;
+          for (int i = 0; 64 * i < nxh; ++i) {
+            const int idx = min(64 * i + lane, nxh - 1);
+            __builtin_amdgcn_global_load_lds((const void*)(row + idx),
+                                             (__attribute__((address_space(3))) void*)(RAW + (c * RW + r) * rawn + 64 * i),
+                                             16, 0, 0);
+          }
         }
       }
     }
@@ -335,26 +352,29 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
   auto c2r_plane = [&](int p, double2* U, double2* V, double2* Wv, bool needv) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double2* row = PF ? RAW + c * RAWN : a.Z + ((long)(c * nyp + (p - 1)) * nz2) * kxp + rowoff;
 #pragma unroll
       for (int q = 0; q < PPL; ++q) {
         if (!vq[q]) continue;
+        const double2* row = PF ? RAW + (c * RW + rq(q)) * rawn
+                                : a.Z + ((long)(c * nyp + (p - 1)) * nz2 + rq(q)) * kxp + rowoff;
         double2 v = make_double2(0.0, 0.0);
         if (c != 1 || needv) v = c2r_pack_nl(row, nxh, M, nq[q], wc[q]);
-        W[c * SEQ + pin[q]] = v;
+        W[c * FS + pin[q]] = v;
       }
     }
     wave_sync();
-    fft2p_wave<M, 1>(W, SEQ, 3, rl, rg, lane);
+    fft2p_wave<M, 1>(W, SEQ, 3 * RW, rl, rg, lane);
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       if (!vq[q]) continue;
-      U[q] = W[0 * SEQ + pout[q]];
-      V[q] = W[1 * SEQ + pout[q]];
-      Wv[q] = W[2 * SEQ + pout[q]];
+      U[q] = W[0 * FS + pout[q]];
+      V[q] = W[1 * FS + pout[q]];
+      Wv[q] = W[2 * FS + pout[q]];
     }
     wave_sync();
   };
+  // offset of pair q in a physical [nz2][nx2] plane
+  auto poff = [&](int q) { return prow + (long)rq(q) * g.nx2 + 2 * nq[q]; };
   auto load_base = [&](int p, double2* U, double2* V, double2* Wv) {
     if (!LIN) return;
     const double* b = a.base;
@@ -362,7 +382,7 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       if (!vq[q]) continue;
-      const long o = ((long)(p - 1) * nz2 + zz) * g.nx2 + 2 * nq[q];
+      const long o = (long)(p - 1) * nz2 * g.nx2 + poff(q);
       U[q] = *reinterpret_cast<const double2*>(b + o);
       V[q] = *reinterpret_cast<const double2*>(b + cs + o);
       Wv[q] = *reinterpret_cast<const double2*>(b + 2 * cs + o);
@@ -371,28 +391,38 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
   // Everything the plane loop reads from global memory besides the raw rows
   // is loaded ahead of the wait for those rows: with PF a later vector load
   // would make its consumer wait for the in-flight prefetch as well.
-  // The columns kx < 2M/3+1 (all of them with 3/2 dealiasing) have their
-  // root and wavenumber in registers; wider rows load the rest in place.
-  constexpr int KQ = (M + 64) / 64;                        // output columns per lane (nx/2+1 <= M+1)
-  constexpr int KP = ((2 * M) / 3 + 1 + 63) / 64 < KQ ? ((2 * M) / 3 + 1 + 63) / 64 : KQ;
+  // The output columns of the rows: output u = lane + 64i is column
+  // u mod nxh of row u / nxh.  The columns of the first KP iterations (all of
+  // them with 3/2 dealiasing, nxh = 2M/3 + 1) have their root and wavenumber
+  // in registers; wider rows load the rest in place.
+  constexpr int KQ = (RW * (M + 1) + 63) / 64;              // nxh <= M + 1
+  constexpr int KPD = (RW * ((2 * M) / 3 + 1) + 63) / 64;
+  constexpr int KP = KPD < KQ ? KPD : KQ;
+  const int nout = RW * nxh;
+  auto orow = [&](int i) {
+    const int u = min(lane + 64 * i, nout - 1);
+    int r = 0;
+#pragma unroll
+    for (int j = 1; j < RW; ++j) r += (u >= j * nxh) ? 1 : 0;
+    return r;
+  };
+  auto ocol = [&](int i) { return min(lane + 64 * i, nout - 1) - orow(i) * nxh; };
   double2 rxl[KP];
   double kxl[KP];
 #pragma unroll
   for (int i = 0; i < KP; ++i) {
-    const int kq = min(lane + 64 * i, nxh - 1);
+    const int kq = ocol(i);
     rxl[i] = rootx[kq];
     kxl[i] = ops.kx[kq];
   }
-  auto bufv = [&](int q) {
-    return *reinterpret_cast<const double2*>(ops.buffer + (long)zz * g.nx2 + 2 * nq[q]);
-  };
+  auto bufv = [&](int q) { return *reinterpret_cast<const double2*>(ops.buffer + poff(q)); };
   // Subgrid stress (LES): tau_ij is added to the flux u_i u_j wherever the
   // flux is formed, so the divergence of the fluxes carries -div(tau) with the
   // same discrete operators.  tau lives at the cell centres yh; at the v-points
   // y(k) tau_12 and tau_23 are interpolated with the iu weights, as u and w are.
   constexpr bool les = LES;
   auto tauv = [&](int c, int kk, int q) {
-    return *reinterpret_cast<const double2*>(a.tau + (((long)c * nyp + kk) * nz2 + zz) * g.nx2 + 2 * nq[q]);
+    return *reinterpret_cast<const double2*>(a.tau + ((long)c * nyp + kk) * nz2 * g.nx2 + poff(q));
   };
 
   const double2 z2 = make_double2(0.0, 0.0);
@@ -500,10 +530,10 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
       if (!LIN) { f0 = pmul(U0[q], U0[q]); f2 = pmul(W0[q], U0[q]); }
       else { f0 = pmul(psc(two, BU0[q]), U0[q]); f2 = padd(pmul(BW0[q], U0[q]), pmul(W0[q], BU0[q])); }
       if (les) { f0 = padd(f0, tauv(0, kk, q)); f2 = padd(f2, tauv(2, kk, q)); }
-      W[0 * SEQ + pin[q]] = f0;
-      W[1 * SEQ + pin[q]] = yfu;
-      W[2 * SEQ + pin[q]] = f2;
-      W[3 * SEQ + pin[q]] = yfw;
+      W[0 * FS + pin[q]] = f0;
+      W[1 * FS + pin[q]] = yfu;
+      W[2 * FS + pin[q]] = f2;
+      W[3 * FS + pin[q]] = yfw;
     }
     // CFL maxima for this plane (check_cfl :211-213)
     if (a.cflbits) {
@@ -529,23 +559,25 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
       }
     }
     wave_sync();
-    fft2p_wave<M, -1>(W, SEQ, 4, rl, rg, lane);
+    fft2p_wave<M, -1>(W, SEQ, 4 * RW, rl, rg, lane);
     const long bst = (long)nz2 * kxp;
-    const long ab0 = ((long)((0 * nyp + kk) * 2) * nz2 + zz) * kxp;
-    const long ab1 = ((long)((1 * nyp + kk) * 2) * nz2 + zz) * kxp;
-    const long ab2 = ((long)((2 * nyp + kk) * 2) * nz2 + zz) * kxp;
+    const long ab0 = ((long)((0 * nyp + kk) * 2) * nz2) * kxp + rowoff;
+    const long ab1 = ((long)((1 * nyp + kk) * 2) * nz2) * kxp + rowoff;
+    const long ab2 = ((long)((2 * nyp + kk) * 2) * nz2) * kxp + rowoff;
 #pragma unroll
     for (int i = 0; i < KQ; ++i) {
-      const int kq = lane + 64 * i;
-      if (kq >= nxh) break;
+      if (lane + 64 * i >= nout) break;
+      const int kq = RW == 1 ? lane + 64 * i : ocol(i);
+      const long oo = (long)orow(i) * kxp + kq;
+      const double2* S = W + orow(i) * SEQ;
       const double2 wk = i < KP ? rxl[i] : rootx[kq];
       const double kxv = i < KP ? kxl[i] : ops.kx[kq];
       auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
-      const double2 X0 = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk), X1 = r2c_post_nl<M>(W + 1 * SEQ, M, kq, wk);
-      const double2 X2 = r2c_post_nl<M>(W + 2 * SEQ, M, kq, wk), X3 = r2c_post_nl<M>(W + 3 * SEQ, M, kq, wk);
-      a.AB[ab0 + kq] = cadd(X1, ikxm(X0));
-      a.AB[ab0 + bst + kq] = X2;
-      a.AB[ab2 + kq] = cadd(X3, ikxm(X2));
+      const double2 X0 = r2c_post_nl<M>(S + 0 * FS, M, kq, wk), X1 = r2c_post_nl<M>(S + 1 * FS, M, kq, wk);
+      const double2 X2 = r2c_post_nl<M>(S + 2 * FS, M, kq, wk), X3 = r2c_post_nl<M>(S + 3 * FS, M, kq, wk);
+      a.AB[ab0 + oo] = cadd(X1, ikxm(X0));
+      a.AB[ab0 + bst + oo] = X2;
+      a.AB[ab2 + oo] = cadd(X3, ikxm(X2));
     }
     wave_sync();
     // batch 2: w^2 and, below the top plane, the v fluxes (:183-195)
@@ -567,7 +599,7 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
       if (les && !top) G1[q] = padd(G1[q], tauv(3, kk + 1, q));
       double2 f4 = LIN ? pmul(psc(two, BW0[q]), W0[q]) : pmul(W0[q], W0[q]);
       if (les) f4 = padd(f4, tauv(5, kk, q));
-      W[0 * SEQ + pin[q]] = f4;
+      W[0 * FS + pin[q]] = f4;
       if (!top) {
         double2 f5, f7;
         if (!LIN) {
@@ -588,26 +620,28 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
           const double tv = tgv;
           yfv = padd(yfv, pmul(bufv(q), make_double2(V0[q].x - tv, V0[q].y - tv)));
         }
-        W[1 * SEQ + pin[q]] = f5;
-        W[2 * SEQ + pin[q]] = yfv;
-        W[3 * SEQ + pin[q]] = f7;
+        W[1 * FS + pin[q]] = f5;
+        W[2 * FS + pin[q]] = yfv;
+        W[3 * FS + pin[q]] = f7;
       }
     }
     wave_sync();
-    fft2p_wave<M, -1>(W, SEQ, nb2, rl, rg, lane);
+    fft2p_wave<M, -1>(W, SEQ, nb2 * RW, rl, rg, lane);
 #pragma unroll
     for (int i = 0; i < KQ; ++i) {
-      const int kq = lane + 64 * i;
-      if (kq >= nxh) break;
+      if (lane + 64 * i >= nout) break;
+      const int kq = RW == 1 ? lane + 64 * i : ocol(i);
+      const long oo = (long)orow(i) * kxp + kq;
+      const double2* S = W + orow(i) * SEQ;
       const double2 wk = i < KP ? rxl[i] : rootx[kq];
       const double kxv = i < KP ? kxl[i] : ops.kx[kq];
       auto ikxm = [&](double2 v) { return make_double2(-kxv * v.y, kxv * v.x); };
-      a.AB[ab2 + bst + kq] = r2c_post_nl<M>(W + 0 * SEQ, M, kq, wk);
+      a.AB[ab2 + bst + oo] = r2c_post_nl<M>(S + 0 * FS, M, kq, wk);
       if (!top) {
-        const double2 X5 = r2c_post_nl<M>(W + 1 * SEQ, M, kq, wk), X6 = r2c_post_nl<M>(W + 2 * SEQ, M, kq, wk);
-        const double2 X7 = r2c_post_nl<M>(W + 3 * SEQ, M, kq, wk);
-        a.AB[ab1 + kq] = cadd(X6, ikxm(X5));
-        a.AB[ab1 + bst + kq] = X7;
+        const double2 X5 = r2c_post_nl<M>(S + 1 * FS, M, kq, wk), X6 = r2c_post_nl<M>(S + 2 * FS, M, kq, wk);
+        const double2 X7 = r2c_post_nl<M>(S + 3 * FS, M, kq, wk);
+        a.AB[ab1 + oo] = cadd(X6, ikxm(X5));
+        a.AB[ab1 + bst + oo] = X7;
       }
     }
     wave_sync();
@@ -621,69 +655,125 @@ __device__ __forceinline__ void xpass_w_body(const XZGeom& g, const double2* __r
   }
 }
 
-template <bool LIN, int M, bool PF, bool LES = false>
+template <bool LIN, int M, bool PF, bool LES = false, int RW = 1>
 __global__ __launch_bounds__(256) void k_xpass_w(XZGeom g, const double2* __restrict__ tw, DevOps ops,
                                                   XPassArgs a) {
-  xpass_w_body<LIN, M, PF, LES>(g, tw, ops, a, blockIdx.x, blockIdx.y);
+  xpass_w_body<LIN, M, PF, LES, RW>(g, tw, ops, a, blockIdx.x, blockIdx.y);
 }
 
 // Batched form: workgroup i runs work-list entry it[i] of channel it[i].m.
-template <bool LIN, int M, bool PF>
+template <bool LIN, int M, bool PF, int RW>
 __global__ __launch_bounds__(256) void k_xpass_w_b(const XBatchArgs* __restrict__ args,
                                                     const BatchItem* __restrict__ it) {
   const BatchItem w = it[blockIdx.x];
   const XBatchArgs& x = args[w.m];
-  xpass_w_body<LIN, M, PF, false>(x.g, x.tw, x.ops, x.a, w.bx, w.by);
+  xpass_w_body<LIN, M, PF, false, RW>(x.g, x.tw, x.ops, x.a, w.bx, w.by);
 }
 
 using XPassFn = void (*)(XZGeom, const double2*, DevOps, XPassArgs);
+using XPassBFn = void (*)(const XBatchArgs*, const BatchItem*);
+
+// Rows per wavefront with a kernel: 1 at every length, 2 for M <= 72 and 4
+// for M <= 24, the last two with the row prefetch only.
+static bool xpass_rw_built(int M, int rw, bool pf) {
+  return rw == 1 || (pf && ((rw == 2 && M <= 72) || (rw == 4 && M <= 24)));
+}
+
+// The default: two rows per wavefront where a row's pairs fill at most 36
+// lanes; the batched launches also at the lengths where that halves the
+// wavefronts per SIMD (their lengths share the GPU, so fewer idle lanes count
+// for more than latency hiding: 162.7 -> 175.5 steps/s on the 48-member
+// ensemble, where a single 64x65x64 channel's x-pass loses 1 %).
+int xpass_rows(int M, int nz2, int req, bool pf, bool batched) {
+  auto ok = [&](int r) { return xpass_rw_built(M, r, pf) && nz2 % r == 0; };
+  if (req > 0 && ok(req)) return req;
+  const int d = (M <= 24 || (batched && M <= 72)) ? 2 : 1;
+  return ok(d) ? d : 1;
+}
+
+// LDS of a wavefront x-pass workgroup: the root table and, per wavefront, 4*rw
+// sequences and (with the prefetch) 3*rw raw rows (of M + 1 cells rounded up
+// to 64 for one row per wavefront, of nxh cells rounded up otherwise)
+size_t xpass_w_lds(int M, int rw, int nxh, bool pf) {
+  const int rawn = rw == 1 ? xpass_raw_cells(M + 1) : xpass_raw_cells(nxh);
+  return sizeof(double2) * (M + 4 * (4 * rw * fft_seq_cells(M) + (pf ? 3 * rw * rawn : 0)));
+}
+
+template <int N, bool PF, int RW>
+static XPassFn xpass_w_variant(bool les, bool lin) {
+  if (les) return k_xpass_w<false, N, PF, true, RW>;
+  return lin ? (XPassFn)k_xpass_w<true, N, PF, false, RW> : (XPassFn)k_xpass_w<false, N, PF, false, RW>;
+}
+
+template <int N>
+static XPassFn xpass_w_fn(bool les, bool lin, bool pf, int rw) {
+  if constexpr (N <= 144) {
+    if (pf) {
+      if constexpr (N <= 24) {
+        if (rw == 4) return xpass_w_variant<N, true, 4>(les, lin);
+      }
+      if constexpr (N <= 72) {
+        if (rw == 2) return xpass_w_variant<N, true, 2>(les, lin);
+      }
+      return rw == 1 ? xpass_w_variant<N, true, 1>(les, lin) : nullptr;
+    }
+  }
+  return rw == 1 ? xpass_w_variant<N, false, 1>(les, lin) : nullptr;
+}
+
+template <int N, int RW>
+static XPassBFn xpass_wb_variant(bool lin) {
+  constexpr bool pf = N <= 144;
+  return lin ? (XPassBFn)k_xpass_w_b<true, N, pf, RW> : (XPassBFn)k_xpass_w_b<false, N, pf, RW>;
+}
+
+template <int N>
+static XPassBFn xpass_wb_fn(bool lin, int rw) {
+  if constexpr (N <= 24) {
+    if (rw == 4) return xpass_wb_variant<N, 4>(lin);
+  }
+  if constexpr (N <= 72) {
+    if (rw == 2) return xpass_wb_variant<N, 2>(lin);
+  }
+  return rw == 1 ? xpass_wb_variant<N, 1>(lin) : nullptr;
+}
+
+static XPassBFn xpass_wb_pick(int M, bool lin, int rw) {
+  XPassBFn fn = nullptr;
+#define XPASSB_CASE(N)                                   \
+  if constexpr (N > 0 && N <= 192) fn = xpass_wb_fn<N>(lin, rw);
+  FFT_DISPATCH(M, XPASSB_CASE)
+#undef XPASSB_CASE
+  if (!fn) throw chan_error("batched x-pass: no wavefront kernel for nx2/2 = " + std::to_string(M) + " with " +
+                            std::to_string(rw) + " rows per wavefront");
+  return fn;
+}
 
 // The wavefront x-pass of several channels with the same nx2/2 = M in one
 // launch (the batched ensemble); the row prefetch is on wherever the single
-// launch uses it (M <= 144).
+// launch uses it (M <= 144).  rw: rows per wavefront (every member's nz2 a
+// multiple of it); nxh: the largest nx/2+1 of the members.
 bool xpass_batchable(int M) { return fft2_split(M).n1 > 0 && M <= 192; }
 
-void launch_xpass_batched(int M, bool lin, const XBatchArgs* args, const BatchItem* items, int nitems,
-                          hipStream_t s) {
+void launch_xpass_batched(int M, bool lin, int rw, int nxh, const XBatchArgs* args, const BatchItem* items,
+                          int nitems, hipStream_t s) {
   if (nitems <= 0) return;
   if (!xpass_batchable(M)) throw chan_error("batched x-pass: nx2/2 = " + std::to_string(M) + " has no wavefront kernel");
-  const bool pf = M <= 144;
-  const int rawn = ((M + 1 + 63) / 64) * 64;
-  const size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
-#define XPASSB_CASE(N)                                                                                   \
-  if constexpr (N > 0 && N <= 192) {                                                                     \
-    if constexpr (N <= 144) {                                                                            \
-      if (lin) hipLaunchKernelGGL((k_xpass_w_b<true, N, true>), dim3(nitems), dim3(256), ldw, s, args, items); \
-      else hipLaunchKernelGGL((k_xpass_w_b<false, N, true>), dim3(nitems), dim3(256), ldw, s, args, items);    \
-    } else {                                                                                             \
-      if (lin) hipLaunchKernelGGL((k_xpass_w_b<true, N, false>), dim3(nitems), dim3(256), ldw, s, args, items); \
-      else hipLaunchKernelGGL((k_xpass_w_b<false, N, false>), dim3(nitems), dim3(256), ldw, s, args, items);    \
-    }                                                                                                    \
-  }
-  FFT_DISPATCH(M, XPASSB_CASE)
-#undef XPASSB_CASE
+  const XPassBFn fn = xpass_wb_pick(M, lin, rw);
+  hipLaunchKernelGGL(fn, dim3(nitems), dim3(256), xpass_w_lds(M, rw, nxh, M <= 144), s, args, items);
   LAUNCH_CHECK("k_xpass_w_b");
 }
 
 // Resident workgroups of the batched wavefront x-pass at length M on the
 // current device (the plane-range rule of the single launch, over the whole batch).
-long xpass_batched_resident(int M, bool lin) {
-  const bool pf = M <= 144;
-  const int rawn = ((M + 1 + 63) / 64) * 64;
-  const size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
-  const void* fn = nullptr;
-#define XPASSR_CASE(N)                                                                              \
-  if constexpr (N > 0 && N <= 192) {                                                                \
-    if constexpr (N <= 144) fn = lin ? (const void*)k_xpass_w_b<true, N, true> : (const void*)k_xpass_w_b<false, N, true>; \
-    else fn = lin ? (const void*)k_xpass_w_b<true, N, false> : (const void*)k_xpass_w_b<false, N, false>; \
-  }
-  FFT_DISPATCH(M, XPASSR_CASE)
-#undef XPASSR_CASE
+long xpass_batched_resident(int M, bool lin, int rw, int nxh) {
+  const void* fn = (const void*)xpass_wb_pick(M, lin, rw);
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
     cus = 256;
-  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, ldw) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, xpass_w_lds(M, rw, nxh, M <= 144)) != hipSuccess ||
+      per_cu < 1)
     per_cu = 1;
   return (long)per_cu * cus;
 }
@@ -724,25 +814,16 @@ void launch_xpass(const XZGeom& g, const double2* tw, const DevOps& ops, const X
     throw chan_error("the subgrid closure needs the wavefront x-pass (nx2/2 <= 192 of the form 2^a 3^b)");
   if (a.wave && ct && M <= 192) {
     const bool pf = a.prefetch && M <= 144;
-    const int rawn = ((M + 1 + 63) / 64) * 64;
-    size_t ldw = sizeof(double2) * (M + 4 * (4 * fft_seq_cells(M) + (pf ? 3 * rawn : 0)));
+    if (a.tau && a.base) throw chan_error("the subgrid closure is not defined for the linearized pass");
+    const int rw = xpass_rows(M, g.nz2, a.rows, pf, false);
+    const size_t ldw = xpass_w_lds(M, rw, g.nxh, pf);
     XPassFn fn = nullptr;
 #define XPASSW_CASE(N)                                                                  \
-  if constexpr (N > 0 && N <= 192) {                                                    \
-    if (a.tau) {                                                                        \
-      if (a.base) throw chan_error("the subgrid closure is not defined for the linearized pass"); \
-      if constexpr (N <= 144) fn = pf ? k_xpass_w<false, N, true, true> : k_xpass_w<false, N, false, true>; \
-      else fn = k_xpass_w<false, N, false, true>;                                       \
-    } else if (pf) {                                                                    \
-      if constexpr (N <= 144) fn = a.base ? k_xpass_w<true, N, true> : k_xpass_w<false, N, true>; \
-    } else {                                                                            \
-      fn = a.base ? k_xpass_w<true, N, false> : k_xpass_w<false, N, false>;             \
-    }                                                                                   \
-  }
+  if constexpr (N > 0 && N <= 192) fn = xpass_w_fn<N>(a.tau != nullptr, a.base != nullptr, pf, rw);
     FFT_DISPATCH(M, XPASSW_CASE)
 #undef XPASSW_CASE
     if (!fn) throw chan_error("x-pass: no wavefront kernel for this length");
-    const long zw = (g.nz2 + 3) / 4;
+    const long zw = (g.nz2 / rw + 3) / 4;
     if (a.chunk <= 0) {
       // resident workgroups of this kernel on the current device
       int dev = 0, cus = 0;

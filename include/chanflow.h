@@ -251,7 +251,11 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * ranges the nonlinear pass runs in turn (1..64; 1: every kernel over all
  * planes), 11 segmented column solve (1 on with 4 modes per wavefront, 2
  * with 2 modes per wavefront, 0 off) wherever ny+1 rows fit 16 (32) segments
- * of 5 to 9 (5) rows (keys 3, 4 and 8 then do not apply).
+ * of 5 to 9 (5) rows (keys 3, 4 and 8 then do not apply), 12 z-rows per
+ * wavefront of the wavefront x-pass (0, the default: by length; 1; 2 for
+ * nx2/2 <= 72; 4 for nx2/2 <= 24; with the prefetch, where nz2 is a multiple;
+ * otherwise the default applies; a batched ensemble takes the fewest rows its
+ * members of one length choose).
  * chan_autotune times `steps`
  * timesteps of every candidate, keeps the fastest and restores the state. */
 int chan_set_tuning(chan_ctx* ctx, int key, int value);

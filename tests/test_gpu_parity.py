@@ -474,6 +474,44 @@ def test_nonlinear_x_lengths(dims):
     ch.close()
 
 
+@pytest.mark.parametrize("lin", [False, True])
+@pytest.mark.parametrize("dims,rows", [((16, 17, 16, 24, 24), 4), ((32, 17, 16, 48, 24), 2),
+                                       ((32, 17, 16, 48, 24), 4), ((32, 17, 12, 48, 18), 4),
+                                       ((48, 17, 16, 72, 24), 2), ((64, 17, 32, 96, 48), 2),
+                                       ((96, 17, 16, 144, 24), 2), ((32, 17, 16, 48, 24), 1)])
+def test_nonlinear_xpass_rows(dims, rows, lin):
+    """The wavefront x-pass with several z-rows per wavefront (tuning key 12):
+    one nonlinear pass, and its linearized form, at nx2/2 = 16 ... 72 with 1,
+    2 and 4 rows against the oracle, CFL included, and against the pass with
+    one row per wavefront to round-off (the arithmetic per point is the same;
+    only the grouping of the sequences differs).  nz2 = 18 with 4 rows asked
+    for falls back to 2 (18 is no multiple of 4)."""
+    nx, ny, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, ny, nz, nx2, nz2, 2.0, 1.0, 2800.0, stretch=2.75, fix_mass=0)
+    s = __import__("helpers").synth()
+    base = O.laminar_baseflow(g) if lin else None
+    amp = 1e-3 if lin else 5e-2
+    U = O.remove_initial_divergence(g, s.laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=14, amp=amp, kmax=6.0))
+    out = []
+    for r in (1, rows):
+        ch = make_channel(g)
+        ch.set_tuning("xpass_rows", r)
+        assert ch.get_tuning()["xpass_rows"] == r
+        if lin:
+            ch.set_baseflow(base)
+        ch.set_state(data1=U)
+        dt, cfl = ch.nonlinear(1, 1, dt=0.002, linearized=lin)
+        out.append((ch.get_data1(), cfl))
+        ch.close()
+    if lin:
+        ref, _, cflr = O.nonlinear_linearized(g, U, base, step=1, rk_step=1, dt=0.002, return_cfl=True)
+    else:
+        ref, _, cflr = O.nonlinear(g, U, step=1, rk_step=1, dt=0.002, return_cfl=True)
+    assert rel_l2(out[1][0], out[0][0]) <= 1e-14
+    assert rel_l2(out[1][0], ref) < TOL1
+    assert abs(out[1][1] - cflr) <= 1e-14 * abs(cflr)
+
+
 @pytest.mark.parametrize("dims", [(16, 17, 16, 24, 24), (32, 17, 16, 48, 24), (64, 97, 64, 96, 96),
                                   (96, 17, 16, 144, 24), (192, 9, 16, 288, 24)])
 def test_nonlinear_linearized_x_lengths(dims):
