@@ -410,7 +410,7 @@ bool tuning_set(Tuning& t, int key, int v) {
     case 8: if (v != 0 && v != 1) return false; t.solve_dma = v; return true;
     case 9: if (v != 0 && (v < 2 || v > 64)) return false; t.xpass_chunk = v; return true;
     case 10: if (v < 1 || v > 64) return false; t.nl_chunks = v; return true;
-    case 11: if (v < 0 || v > 2) return false; t.solve_seg = v; return true;
+    case 11: if (v < 0 || v > 3) return false; t.solve_seg = v; return true;
     case 12: if (v != 0 && v != 1 && v != 2 && v != 4) return false; t.xpass_rows = v; return true;
     default: return false;
   }
@@ -526,7 +526,8 @@ std::string batch_ineligible(const chan_ctx* c, bool lin) {
   if (!solve_seg_rows(c->prm.ny)) return "ny+1 rows do not fit the segmented solve";
   if (3.0 * c->nyp * (double)c->nmodes * 16.0 >= 4.0e9) return "the segmented solve's 32-bit offsets";
   const Tuning& t = c->tune;
-  if (!(t.xpass_wave == 1 && t.zfwd_v2 == 1 && t.zinv_kx == 8 && t.solve_seg == 1 && t.nl_chunks == 1 &&
+  if (!(t.xpass_wave == 1 && t.zfwd_v2 == 1 && t.zinv_kx == 8 && (t.solve_seg == 1 || t.solve_seg == 3) &&
+        t.nl_chunks == 1 &&
         t.xpass_prefetch == 1))
     return "its launch shapes are not the defaults";
   if (c->timing) return "kernel timing is on";
@@ -659,7 +660,7 @@ void ensemble_plan_batched(chan_ensemble* e) {
   // the segmented solve, one table per substep
   for (int rk = 1; rk <= 3; ++rk) {
     e->solve[rk - 1] = plan_batches<SBatchArgs>(
-        e, [](chan_ctx* c) { return solve_seg_rows(c->prm.ny); },
+        e, [](chan_ctx* c) { return solve_seg_layout(c->prm.ny, c->tune.solve_seg); },
         [&](chan_ctx* c) {
           SBatchArgs x{};
           x.ops = c->ops;
@@ -672,7 +673,8 @@ void ensemble_plan_batched(chan_ensemble* e) {
           return x;
         },
         [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
-          push_grid(it, k, (int)((nmodes(c) + 3) / 4), 1);
+          const int modes = solve_seg_layout(c->prm.ny, c->tune.solve_seg) / 16;
+          push_grid(it, k, (int)((nmodes(c) + modes - 1) / modes), 1);
         },
         [](chan_ctx* c) { return (long)c->nmodes * c->nyp; });
   }

@@ -143,7 +143,8 @@ struct Tuning {
   int solve_dma = 1;      // 3-lane solve streams its rows through an LDS ring
   int xpass_chunk = 0;    // planes per x-pass workgroup (2..64; 0: from the resident workgroups)
   int nl_chunks = 1;      // plane ranges the NL pass runs in turn (1: all planes per kernel)
-  int solve_seg = 1;      // segmented column solve (columns in registers, no scratch); 2: two modes per wavefront
+  int solve_seg = 1;      // segmented column solve (columns in registers, no scratch); 2: two modes per
+                          // wavefront; 3: eight modes per wavefront where ny+1 rows fit 8 segments
   int xpass_rows = 0;     // z-rows per wavefront of the wave x-pass (0: by length, 1, 2, 4)
 };
 Tuning tuning_defaults();
@@ -275,7 +276,11 @@ struct FillBatchArgs {
   unsigned long long* p;
   size_t n;   // 8-byte words
 };
-void launch_solve_batched(int seg, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s);
+void launch_solve_batched(int layout, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s);
+// Batched segmented-solve layout of a channel: 16 * modes per wavefront + rows
+// per segment (8 modes where ny+1 rows fit 8 segments, all of them busy for
+// solve_seg 1, any for solve_seg 3; else 4), 0 if ny+1 rows fit neither
+int solve_seg_layout(int ny, int solve_seg);
 void launch_cfl_finalize_batched(const CflBatchArgs* args, int n, int step, hipStream_t s);
 void launch_fill_zero_batched(const FillBatchArgs* args, int n, size_t max_words, hipStream_t s);
 void launch_advance_time_batched(DevScalars* const* sc, int n, hipStream_t s);

@@ -682,7 +682,7 @@ static bool xpass_rw_built(int M, int rw, bool pf) {
 // The default: two rows per wavefront where a row's pairs fill at most 36
 // lanes; the batched launches also at the lengths where that halves the
 // wavefronts per SIMD (their lengths share the GPU, so fewer idle lanes count
-// for more than latency hiding: 162.7 -> 175.5 steps/s on the 48-member
+// for more than latency hiding: 163 -> 178 steps/s on the 48-member
 // ensemble, where a single 64x65x64 channel's x-pass loses 1 %).
 int xpass_rows(int M, int nz2, int req, bool pf, bool batched) {
   auto ok = [&](int r) { return xpass_rw_built(M, r, pf) && nz2 % r == 0; };

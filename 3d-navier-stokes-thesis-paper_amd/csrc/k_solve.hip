@@ -1475,6 +1475,23 @@ void launch_solve(const DevOps& ops, const SolveArgs& a, hipStream_t s) {
     LAUNCH_CHECK("k_solve_seg");
     return;
   }
+  // eight modes per wavefront (8 segments) where ny+1 rows fit 8 segments of
+  // 5..9 rows: 128-byte row pieces, and fewer idle segment lanes at ny + 1
+  // just under 8 SEG (66 rows: 8 segments of 9 rows, against 11 of 16 lanes
+  // groups busy with 6-row segments)
+  const int seg8 = (a.seg == 3) ? solve_seg_rows(a.ny, 8) : 0;
+  if (seg8 && off32) {
+    dim3 gs((unsigned)((nm + 7) / 8));
+    switch (seg8) {
+      case 5: hipLaunchKernelGGL((k_solve_seg<8, 5>), gs, dim3(64), 0, s, ops, a); break;
+      case 6: hipLaunchKernelGGL((k_solve_seg<8, 6>), gs, dim3(64), 0, s, ops, a); break;
+      case 7: hipLaunchKernelGGL((k_solve_seg<8, 7>), gs, dim3(64), 0, s, ops, a); break;
+      case 8: hipLaunchKernelGGL((k_solve_seg<8, 8>), gs, dim3(64), 0, s, ops, a); break;
+      default: hipLaunchKernelGGL((k_solve_seg<8, 9>), gs, dim3(64), 0, s, ops, a); break;
+    }
+    LAUNCH_CHECK("k_solve_seg");
+    return;
+  }
   const int seg = a.seg ? solve_seg_rows(a.ny) : 0;
   if (seg && off32) {
     dim3 gs((unsigned)((nm + 3) / 4));
@@ -1787,17 +1804,37 @@ void launch_advance_time_batched(DevScalars* const* sc, int n, hipStream_t s) {
 }
 
 // The segmented solve of several channels with the same segment length in one launch.
-void launch_solve_batched(int seg, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s) {
+// layout = 16 * modes per wavefront + rows per segment (solve_seg_layout)
+void launch_solve_batched(int layout, const SBatchArgs* args, const BatchItem* items, int nitems, hipStream_t s) {
   if (nitems <= 0) return;
-  switch (seg) {
-    case 5: hipLaunchKernelGGL((k_solve_seg_b<4, 5>), dim3(nitems), dim3(64), 0, s, args, items); break;
-    case 6: hipLaunchKernelGGL((k_solve_seg_b<4, 6>), dim3(nitems), dim3(64), 0, s, args, items); break;
-    case 7: hipLaunchKernelGGL((k_solve_seg_b<4, 7>), dim3(nitems), dim3(64), 0, s, args, items); break;
-    case 8: hipLaunchKernelGGL((k_solve_seg_b<4, 8>), dim3(nitems), dim3(64), 0, s, args, items); break;
-    case 9: hipLaunchKernelGGL((k_solve_seg_b<4, 9>), dim3(nitems), dim3(64), 0, s, args, items); break;
-    default: throw chan_error("batched solve: no segmented solve with " + std::to_string(seg) + "-row segments");
+  switch (layout) {
+    case 64 + 5: hipLaunchKernelGGL((k_solve_seg_b<4, 5>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 64 + 6: hipLaunchKernelGGL((k_solve_seg_b<4, 6>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 64 + 7: hipLaunchKernelGGL((k_solve_seg_b<4, 7>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 64 + 8: hipLaunchKernelGGL((k_solve_seg_b<4, 8>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 64 + 9: hipLaunchKernelGGL((k_solve_seg_b<4, 9>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 128 + 5: hipLaunchKernelGGL((k_solve_seg_b<8, 5>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 128 + 6: hipLaunchKernelGGL((k_solve_seg_b<8, 6>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 128 + 7: hipLaunchKernelGGL((k_solve_seg_b<8, 7>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 128 + 8: hipLaunchKernelGGL((k_solve_seg_b<8, 8>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    case 128 + 9: hipLaunchKernelGGL((k_solve_seg_b<8, 9>), dim3(nitems), dim3(64), 0, s, args, items); break;
+    default: throw chan_error("batched solve: no segmented solve of layout " + std::to_string(layout));
   }
   LAUNCH_CHECK("k_solve_seg_b");
+}
+
+int solve_seg_layout(int ny, int solve_seg) {
+  // The batched launches take 8 modes per wavefront by default where all 8
+  // segments are busy (66 rows: 8 of 9 rows, against 11 of 16 segments of 6
+  // rows with 4 modes): 179 -> 182 steps/s on the 48-member ensemble, whose
+  // launches keep the GPU full.  Where segments would idle (50 rows: 6 of 8)
+  // it was slower, and so is a single channel with fewer than ~4000 modes,
+  // whose wavefronts are then too few (64x65x64: 21 -> 26 us per solve).
+  int s8 = (solve_seg == 1 || solve_seg == 3) ? solve_seg_rows(ny, 8) : 0;
+  if (s8 && solve_seg == 1 && (ny + 1 + s8 - 1) / s8 < 8) s8 = 0;
+  if (s8) return 128 + s8;
+  const int s4 = solve_seg_rows(ny);
+  return s4 ? 64 + s4 : 0;
 }
 
 // max |ikx u + dh (v_j - v_{j-1}) + ikz w| over non-mean modes, j = 2..ny

@@ -250,8 +250,10 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * pass spread evenly over the workgroups resident at a time), 10 plane
  * ranges the nonlinear pass runs in turn (1..64; 1: every kernel over all
  * planes), 11 segmented column solve (1 on with 4 modes per wavefront, 2
- * with 2 modes per wavefront, 0 off) wherever ny+1 rows fit 16 (32) segments
- * of 5 to 9 (5) rows (keys 3, 4 and 8 then do not apply), 12 z-rows per
+ * with 2 modes per wavefront, 3 with 8 modes per wavefront, 0 off) wherever
+ * ny+1 rows fit 16 (32, 8) segments of 5 to 9 (5, 5 to 9) rows (keys 3, 4
+ * and 8 then do not apply; value 3 falls back to 1 where 8 segments are too
+ * few), 12 z-rows per
  * wavefront of the wavefront x-pass (0, the default: by length; 1; 2 for
  * nx2/2 <= 72; 4 for nx2/2 <= 24; with the prefetch, where nz2 is a multiple;
  * otherwise the default applies; a batched ensemble takes the fewest rows its

@@ -86,12 +86,14 @@ def test_autotune_restores_state_and_keeps_results():
     a.close(); b.close()
 
 
-def _draw_member(d, rows=0):
+def _draw_member(d, rows=0, seg8=False):
     g = O.make_grid(d["nx"], d["ny"], d["nz"], d["nx2"], d["nz2"], 2.0, 2.0, d["re"], stretch=1.5, fix_mass=1)
     U0 = synth().laminar_perturbed(d["nx"], d["ny"], d["nz"], g.yh, g.y, seed=d["seed"], amp=1e-2, kmax=4.0)
     ch = make_channel(g)
     if rows:
         ch.set_tuning("xpass_rows", rows)
+    if seg8:
+        ch.set_tuning("solve_seg", 3)
     ch.set_state(data1=U0)
     ch.set_scalars(dt=0.005, t_total=0.0, p_grad=-2.0 / d["re"])
     ch.initialize_flow()
@@ -107,12 +109,15 @@ def test_batched_ensemble_48_members():
     batched x-pass takes two z-rows per wavefront at every length of the draw
     (nx2/2 <= 72), a single channel's only at nx2/2 <= 24, so members and
     twins are asked for two rows (tuning key 12; the initial projection runs a
-    nonlinear pass too): the same launch shapes, the same arithmetic."""
+    nonlinear pass too); the batched solve takes 8 modes per wavefront at
+    ny = 65 (8 segments of 9 rows), so those twins are asked for that layout
+    (tuning key 11 = 3): the same launch shapes, the same arithmetic."""
     cf = __import__("helpers").chanflow()
     draws = synth().ensemble_draw(48, seed=3)
     assert all(d["nx2"] // 2 <= 72 for d in draws)
     mem = [_draw_member(d, rows=2) for d in draws]
-    twins = [_draw_member(d, rows=2) for d in draws]
+    assert {d["ny"] for d in draws} == {49, 65, 97}
+    twins = [_draw_member(d, rows=2, seg8=d["ny"] == 65) for d in draws]
     ens = cf.Ensemble([m[2] for m in mem])            # batched launches
     for step in (1, 2):
         print(f"batched ensemble replay {step}", flush=True)

@@ -281,7 +281,8 @@ def test_rhs_solve_variants(g0, variant):
             assert rel_l2(x, y) <= 1e-14, rk
 
 
-@pytest.mark.parametrize("ny,layout", [(33, 1), (65, 1), (97, 1), (129, 1), (33, 2), (97, 2), (129, 2)])
+@pytest.mark.parametrize("ny,layout", [(33, 1), (65, 1), (97, 1), (129, 1), (33, 2), (97, 2), (129, 2),
+                                       (33, 3), (49, 3), (65, 3), (97, 3)])
 def test_rhs_solve_segmented(ny, layout):
     """The segmented column solve (partitioned tridiagonal solves, columns in
     registers) against the oracle for the three substeps, with a perturbed RK
@@ -289,7 +290,10 @@ def test_rhs_solve_segmented(ny, layout):
     solve to 1e-12.  Layout 1, 4 modes per wavefront: segments of 5 rows
     (ny = 33: 7 segments, idle lanes past them), 6 (65: 11), 7 (97: 14) and
     9 (129: 15).  Layout 2, 2 modes per wavefront: 5-row segments, 7 of 32
-    at ny = 33, 20 at 97 (3 rows in the last), 26 at 129."""
+    at ny = 33, 20 at 97 (3 rows in the last), 26 at 129.  Layout 3, 8 modes
+    per wavefront: 5-row segments at ny = 33 (7 of 8), 9-row segments at 49
+    (6) and 65 (all 8, 3 rows in the last); 98 rows do not fit 8 segments, so
+    ny = 97 takes layout 1."""
     # 9 x 16 = 144 modes: 72 two-mode groups, 64 in the XCD-grouped prefix of
     # the grid and 8 past it
     g = O.make_grid(16, ny, 16, 24, 24, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1, dt_flag=0)
