@@ -585,9 +585,10 @@ class Ensemble:
         self.lib.chan_ensemble_invalidate(self.h)
 
     def autotune(self, steps=3):
-        """Batched ensembles: time each x-pass plane-range plan over `steps`
-        replayed steps (every member's state restored after each), keep the
-        fastest; returns its ms per step."""
+        """Batched ensembles: time each x-pass plane-range plan, then the tile
+        shapes (z-rows per x-pass wavefront, modes per solve wavefront) over
+        `steps` replayed steps each (every member's state restored after
+        each), keep the fastest; returns its ms per step."""
         ms = C.c_double(0.0)
         if self.lib.chan_ensemble_autotune(self.h, steps, C.byref(ms)) != 0:
             raise ChanError(self.lib.chan_ensemble_last_error(self.h).decode())

@@ -495,6 +495,9 @@ struct chan_ensemble {
   // x-pass plane ranges of the batched launches: each length's rows x planes
   // spread over its resident workgroups times xmul (0: the number of lengths)
   int xmul = 0;
+  // launch-plan overrides the autotuner may choose: one z-row per x-pass
+  // wavefront at every length; 4 modes per solve wavefront everywhere
+  int xrows1 = 0, solve4 = 0;
 };
 
 namespace {
@@ -618,7 +621,8 @@ void ensemble_plan_batched(chan_ensemble* e) {
   // nz2 is a multiple of it.
   std::map<int, int> xrw, xnxh;  // per length: rows per wavefront, largest nx/2+1
   for (chan_ctx* c : e->m) {
-    const int M = c->gd.nx2 / 2, r = xpass_rows(M, c->gd.nz2, c->tune.xpass_rows, M <= 144, true);
+    const int M = c->gd.nx2 / 2;
+    const int r = e->xrows1 ? 1 : xpass_rows(M, c->gd.nz2, c->tune.xpass_rows, M <= 144, true);
     xrw[M] = xrw.count(M) ? std::min(xrw[M], r) : r;
     xnxh[M] = std::max(xnxh[M], c->nxh);
   }
@@ -658,9 +662,12 @@ void ensemble_plan_batched(chan_ensemble* e) {
     }
   }
   // the segmented solve, one table per substep
+  auto layout = [e](chan_ctx* c) {
+    return e->solve4 ? 64 + solve_seg_rows(c->prm.ny) : solve_seg_layout(c->prm.ny, c->tune.solve_seg);
+  };
   for (int rk = 1; rk <= 3; ++rk) {
     e->solve[rk - 1] = plan_batches<SBatchArgs>(
-        e, [](chan_ctx* c) { return solve_seg_layout(c->prm.ny, c->tune.solve_seg); },
+        e, layout,
         [&](chan_ctx* c) {
           SBatchArgs x{};
           x.ops = c->ops;
@@ -673,7 +680,7 @@ void ensemble_plan_batched(chan_ensemble* e) {
           return x;
         },
         [&](chan_ctx* c, unsigned short k, std::vector<BatchItem>& it) {
-          const int modes = solve_seg_layout(c->prm.ny, c->tune.solve_seg) / 16;
+          const int modes = layout(c) / 16;
           push_grid(it, k, (int)((nmodes(c) + modes - 1) / modes), 1);
         },
         [](chan_ctx* c) { return (long)c->nmodes * c->nyp; });
@@ -1730,14 +1737,30 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
       cp(bk + 2 * vb + pb + sizeof(DevScalars), c->d_dudy, db);
       CHAN_CHECK(hipStreamSynchronize(c->stream));
     };
+    // Candidates: the x-pass kernel and, for the wavefront kernel, its z-rows
+    // per wavefront where the length has them; the z-forward kernel; the
+    // segmented solve's layouts (4, 2 or 8 modes per wavefront where ny + 1
+    // rows fit) or the sweep-based solves with their block shapes.
+    const int M = c->gd.nx2 / 2;
+    const bool pf = c->tune.xpass_prefetch == 1 && M <= 144;
+    std::vector<int> rows_c = {0};
+    for (int r : {1, 2, 4})
+      if (xpass_rows(M, c->gd.nz2, r, pf, false) == r && r != xpass_rows(M, c->gd.nz2, 0, pf, false))
+        rows_c.push_back(r);
+    std::vector<int> seg_c;
+    if (solve_seg_rows(c->prm.ny)) seg_c.push_back(1);
+    if (solve_seg_rows(c->prm.ny, 32) == 5) seg_c.push_back(2);
+    if (solve_seg_rows(c->prm.ny, 8)) seg_c.push_back(3);
     std::vector<Tuning> cand;
     for (int w : {1, 0})
       for (int z : {1, 0}) {
-        if (solve_seg_rows(c->prm.ny)) {  // the segmented solve has no further shape
-          Tuning t = c->tune;
-          t.xpass_wave = w; t.zfwd_v2 = z; t.solve_seg = 1;
-          cand.push_back(t);
-        }
+        for (int sg : seg_c)
+          for (int r : rows_c) {
+            if (r != 0 && w != 1) continue;  // rows per wavefront: the wavefront kernel only
+            Tuning t = c->tune;
+            t.xpass_wave = w; t.zfwd_v2 = z; t.solve_seg = sg; t.xpass_rows = r;
+            cand.push_back(t);
+          }
         for (int sl : {3, 1})
           for (int bl : {8, 16})
             for (int dm : {1, 0}) {
@@ -1891,7 +1914,7 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
   if (!e) return fail(nullptr, "null ensemble");
   std::vector<void*> bk;
   hipEvent_t a = nullptr, b = nullptr;
-  const int xmul0 = e->xmul;  // kept if the tuning fails
+  const int xmul0 = e->xmul, xrows0 = e->xrows1, solve0 = e->solve4;  // kept if the tuning fails
   auto cleanup = [&] {
     for (void* p : bk) dfree(p);
     bk.clear();
@@ -1935,11 +1958,9 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
     }
     CHAN_CHECK(hipEventCreate(&a));
     CHAN_CHECK(hipEventCreate(&b));
-    double best = 1e300;
-    int bestm = xmul0;
-    copy_state(false);
-    for (int m : {0, 1, 2, 4, 8}) {
-      e->xmul = m;
+    // one plan's trial: capture, one untimed and `steps` timed replays, the
+    // members' state restored; +inf if a member tripped the CFL check
+    auto trial = [&]() {
       ensemble_release(e);
       ensemble_capture(e);
       CHAN_CHECK(hipGraphLaunch(e->exec, e->master));  // first replay, untimed
@@ -1956,10 +1977,32 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
         ok = ok && tsc.err == 0 && std::isfinite(tsc.cfl);
       }
       copy_state(true);
-      if (ok && ms / steps < best) { best = ms / steps; bestm = m; }
+      return ok ? (double)ms / steps : 1e300;
+    };
+    double best = 1e300;
+    int bestm = xmul0, bestr = 0, bests = 0;
+    copy_state(false);
+    // the x-pass plane ranges with the default tile shapes, then the tile
+    // shapes (one z-row per x-pass wavefront; 4 modes per solve wavefront)
+    // with the best ranges
+    e->xrows1 = 0;
+    e->solve4 = 0;
+    for (int m : {0, 1, 2, 4, 8}) {
+      e->xmul = m;
+      const double ms = trial();
+      if (ms < best) { best = ms; bestm = m; }
+    }
+    e->xmul = bestm;
+    for (int v : {1, 2, 3}) {
+      e->xrows1 = v & 1;
+      e->solve4 = v >> 1;
+      const double ms = trial();
+      if (ms < best) { best = ms; bestr = e->xrows1; bests = e->solve4; }
     }
     if (!(best < 1e300)) throw chan_error("autotune: every plan's trial steps failed the CFL check");
     e->xmul = bestm;
+    e->xrows1 = bestr;
+    e->solve4 = bests;
     ensemble_release(e);
     cleanup();
     if (best_ms) *best_ms = best;
@@ -1971,6 +2014,8 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
   }
   cleanup();
   e->xmul = xmul0;
+  e->xrows1 = xrows0;
+  e->solve4 = solve0;
   ensemble_release(e);  // the next step plans and captures again
   return 1;
 }

@@ -258,8 +258,11 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * nx2/2 <= 72; 4 for nx2/2 <= 24; with the prefetch, where nz2 is a multiple;
  * otherwise the default applies; a batched ensemble takes the fewest rows its
  * members of one length choose).
- * chan_autotune times `steps`
- * timesteps of every candidate, keeps the fastest and restores the state. */
+ * chan_autotune times `steps` timesteps of every candidate (keys 0 and 2;
+ * with the wavefront x-pass, key 12's row counts the length has; the
+ * segmented solve's layouts of key 11 that ny + 1 rows fit, or the
+ * sweep-based solves with keys 3, 4 and 8), keeps the fastest whose trial
+ * passed the CFL check and restores the state. */
 int chan_set_tuning(chan_ctx* ctx, int key, int value);
 int chan_get_tuning(chan_ctx* ctx, int key, int* value);
 int chan_autotune(chan_ctx* ctx, int steps, double* best_ms_per_step);
@@ -287,9 +290,13 @@ int chan_ensemble_invalidate(chan_ensemble* ens);
 /* Launch-plan tuning of a batched ensemble (nstreams = 0; no counterpart in
  * the reference): times `steps` replayed steps of each x-pass plane-range
  * plan (each length's planes cut for 1, 2, 4 or 8 times, or as many times as
- * there are lengths, its resident workgroups), restores every member's state
- * after each trial, keeps the fastest and returns its time per step.  The
- * plans give every member the same arithmetic. */
+ * there are lengths, its resident workgroups), then, with the fastest ranges,
+ * the tile shapes (one z-row per x-pass wavefront at every length instead of
+ * the batched default; 4 modes per solve wavefront everywhere instead of 8
+ * where all 8 segments are busy), restores every member's state after each
+ * trial, keeps the fastest plan and returns its time per step.  The ranges
+ * give every member the same arithmetic; the tile shapes the same to
+ * round-off. */
 int chan_ensemble_autotune(chan_ensemble* ens, int steps, double* best_ms_per_step);
 int chan_ensemble_destroy(chan_ensemble* ens);
 const char* chan_ensemble_last_error(const chan_ensemble* ens);

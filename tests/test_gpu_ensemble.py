@@ -146,7 +146,9 @@ def test_batched_ensemble_autotune_keeps_state_and_results():
     """chan_ensemble_autotune on 12 members of the seeded draw: every member's
     state is the same after the tuning as before it (bit for bit), and the
     tuned ensemble's next two steps equal those of an untuned twin ensemble
-    (the plans differ only in the x-pass plane ranges)."""
+    to round-off (the plans differ in the x-pass plane ranges, which leave
+    the arithmetic as it is, and may differ in the tile shapes: rows per
+    x-pass wavefront, modes per solve wavefront)."""
     cf = __import__("helpers").chanflow()
     draws = synth().ensemble_draw(12, seed=5)
     mem = [_draw_member(d) for d in draws]
@@ -163,7 +165,8 @@ def test_batched_ensemble_autotune_keeps_state_and_results():
     ref.step(2)
     for (g, _, a), (_, _, b) in zip(mem, twins):
         for x, y in zip(a.get_state(), b.get_state()):
-            assert np.array_equal(x, y), (g.nx, g.ny, g.nz)
+            scale = max(float(np.max(np.abs(y))), 1e-300)
+            assert float(np.max(np.abs(x - y))) <= 1e-13 * scale, (g.nx, g.ny, g.nz)
     streams = cf.Ensemble([m[2] for m in mem[:2]], nstreams=1)
     with pytest.raises(cf.ChanError):
         streams.autotune(steps=1)
