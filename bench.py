@@ -14,8 +14,9 @@ against the FP64 vector peak; `counters` quotes the committed rocprofv3
 counter summary; `cpu_baseline` times the numpy oracle on a bounded sample
 (3 steps) of the same workload on one host core.  Beside the headline: the
 learned-closure LES speed-up with a live a-posteriori comparison, the
-48-member batched ensemble, the reference's own driver over the library and
-the jet-control run (progress lines on stderr).
+48-member batched ensemble, the reference's own driver over the library,
+the jet-control run and the Arnoldi matvec of the linearized-stability
+configuration (progress lines on stderr).
 """
 import argparse
 import math
@@ -222,6 +223,35 @@ def dropin_rate(cf, c, U0, steps=400, window=200):
     return window / w[-1], w[-1]
 
 
+def arnoldi_matvec_rate(cf, nmv=20):
+    """The linearized-stability configuration's Arnoldi matvec
+    (chan_linearized_matvec, compute_time_deriv of PFC/arnoldi.f90:485-567:
+    tend linearized RK3 steps about the laminar profile from a scaled start
+    field, host field in and out): 64 x 97 x 64 (96 x 96 dealiased, box
+    2pi x 2 x pi, stretch 2.75), Re_c 2800, dt = 0.95 (lx/nx2)/32 and
+    tend = max(int(0.1/dt), 10) as init_baseflow and arnoldi.f90 set them
+    (PFC/pass1_linearized.f90:46, PFC/arnoldi.f90:36).  Returns (matvecs/s,
+    linearized steps/s, tend)."""
+    pi = np.pi
+    ch = cf.Channel(64, 97, 64, 96, 96, 2.0 * pi, pi, 2800.0, fix_mass=0, dt_flag=0)
+    y, yh = ch.build_operators(2.75)
+    base = np.zeros((3, 98, 96, 96))
+    base[0] = (1.0 - np.asarray(yh) ** 2)[:, None, None]    # laminar branch of init_baseflow
+    ch.set_baseflow(base)
+    dt = 0.95 * (2.0 * pi / 96.0) / 32.0
+    tend = max(int(0.1 / dt), 10)
+    ch.set_scalars(dt=dt)
+    v = 1.0e-6 * np.random.default_rng(9).standard_normal((3, 98, 64, 64))
+    ch.linearized_matvec(v, tend)            # untimed
+    t0 = time.perf_counter()
+    for _ in range(nmv):
+        w, _t = ch.linearized_matvec(v, tend)
+    t1 = time.perf_counter()
+    ch.close()
+    mps = nmv / (t1 - t0)
+    return mps, mps * tend, tend
+
+
 def jet_control_rate(cf, synth, steps=1000):
     """The jet-control run of the linearized-stability configuration:
     64 x 97 x 64 (96 x 96 dealiased, box 2pi x 2 x pi, stretch 2.75), Re_c
@@ -280,6 +310,8 @@ def main():
                     help="steps of the 48-member batched ensemble timed (0: skip)")
     ap.add_argument("--dropin-steps", type=int, default=400,
                     help="steps of the reference's own driver on the headline workload (0: skip)")
+    ap.add_argument("--arnoldi-matvecs", type=int, default=20,
+                    help="timed Arnoldi matvecs of the linearized-stability configuration (0: skip)")
     ap.add_argument("--jet-steps", type=int, default=1000,
                     help="steps of the 64x97x64 jet-control run (0: skip)")
     ap.add_argument("--les-spinup", type=float, default=40.0,
@@ -448,6 +480,15 @@ def main():
                          "gains 0.5, dt 0.002", "steps": args.jet_steps, "steps_per_s": jsps,
                "max_divergence": jdiv, "finite": jfin}
 
+    arn = None
+    if args.arnoldi_matvecs > 0:
+        note("Arnoldi matvecs")
+        mps, lsps, tend = arnoldi_matvec_rate(cf, args.arnoldi_matvecs)
+        arn = {"config": "64x97x64 linearized about the laminar profile, Re_c 2800, box 2pi x 2 x pi, "
+                         f"dt 0.95 lx/nx2/32, tend {tend} steps per matvec (the Arnoldi operator of "
+                         "arnoldi.f90), host field in and out", "matvecs": args.arnoldi_matvecs,
+               "matvecs_per_s": mps, "linearized_steps_per_s": lsps}
+
     cpu = None
     if not args.no_cpu_baseline:
         note("CPU baseline (oracle)")
@@ -489,6 +530,7 @@ def main():
         "ensemble": ensemble,
         "dropin": dropin,
         "jet_control": jet,
+        "arnoldi": arn,
     }
     print(json.dumps(out), flush=True)
 
