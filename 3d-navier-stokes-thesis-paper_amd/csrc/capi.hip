@@ -252,26 +252,35 @@ void boundary_dev(chan_ctx* c) {
   const long np = (long)g.nz2 * g.nx2;
   const double* dm = c->h_dm;
   double* ph = (double*)stage(c, sizeof(double) * np * 6);
+  const long zps = (long)g.nz2 * g.kxpad;
   for (int q = 0; q < 2; ++q) {
     if (gains[q] == 0.0) continue;
     const int comp = (q == 0) ? 0 : 2;
     const double add = (q == 0) ? 2.0 : 0.0;
     const double2* base = c->d_vel + (long)comp * c->nyp * nm;
-    // planes 2,3 (0-based 1,2) and ny, ny-1 (0-based ny-1, ny-2)
-    dev_c2r(c, g, base + 1 * nm, 1, ph + 0 * np);
-    dev_c2r(c, g, base + 2 * nm, 1, ph + 1 * np);
-    dev_c2r(c, g, base + (long)(ny - 1) * nm, 1, ph + 2 * np);
-    dev_c2r(c, g, base + (long)(ny - 2) * nm, 1, ph + 3 * np);
-    // bottom: (dm(2,1) u2 + dm(3,1) u3 - add)/(gain - dm(1,1))
-    launch_ctrl_law((int)np, ph + 0 * np, ph + 1 * np, dm[1], dm[2], -add, 1.0, gains[q] - dm[0], ph + 4 * np,
+    // planes 2,3 and ny-1, ny (0-based 1, 2, ny-2, ny-1) in one z-inverse:
+    // grid row y is plane 1 + y % 2 + (ny - 3) (y / 2), written to the same
+    // plane of the intermediate; then the x c2r of each pair, giving
+    // ph[0..3] = u(1), u(2), u(ny-2), u(ny-1)
+    if (ny >= 5) {
+      launch_zinv(g, c->d_tw, base, nm, c->d_Z, zps, 4, c->stream, 8, 2, ny - 3, 1);
+      launch_xc2r(g, c->d_tw, c->d_Z + 1 * zps, zps, ph + 0 * np, np, 2, c->stream);
+      launch_xc2r(g, c->d_tw, c->d_Z + (long)(ny - 2) * zps, zps, ph + 2 * np, np, 2, c->stream);
+    } else {  // the two pairs overlap
+      dev_c2r(c, g, base + 1 * nm, 2, ph + 0 * np);
+      dev_c2r(c, g, base + (long)(ny - 2) * nm, 2, ph + 2 * np);
+    }
+    // top (ph[4]): -(dm(2,2) u_ny + dm(1,2) u_{ny-1} + add)/(gain + dm(3,2));
+    // bottom (ph[5]): (dm(2,1) u2 + dm(3,1) u3 - add)/(gain - dm(1,1))
+    launch_ctrl_law((int)np, CtrlLaw{ph + 3 * np, ph + 2 * np, dm[4], dm[3], add, -1.0, gains[q] + dm[5], ph + 4 * np},
+                    CtrlLaw{ph + 0 * np, ph + 1 * np, dm[1], dm[2], -add, 1.0, gains[q] - dm[0], ph + 5 * np},
                     c->stream);
-    // top: -(dm(2,2) u_ny + dm(1,2) u_{ny-1} + add)/(gain + dm(3,2))
-    launch_ctrl_law((int)np, ph + 2 * np, ph + 3 * np, dm[4], dm[3], add, -1.0, gains[q] + dm[5], ph + 5 * np,
-                    c->stream);
-    double2* top = c->d_cbc + (0L * 3 + comp) * nm;
-    double2* bot = c->d_cbc + (1L * 3 + comp) * nm;
-    dev_r2c(c, g, ph + 4 * np, 1, bot);
-    dev_r2c(c, g, ph + 5 * np, 1, top);
+    // both walls in one x r2c and one z-forward: the top into plane 0 and the
+    // bottom into plane 3 of the intermediate and of cbc + comp, the wall
+    // data's [wall][component] planes (top = wall 0, bottom = wall 1)
+    launch_xr2c(g, c->d_tw, ph + 4 * np, nullptr, c->ops.kx, np, c->d_AB, 3 * zps, 2, c->stream);
+    launch_zfwd(g, c->d_tw, c->d_AB, nullptr, zps, c->ops.kz, c->d_cbc + (long)comp * nm, nm, 2, 0, c->stream,
+                1, 8, 1, 3, 0);
   }
 }
 

@@ -314,8 +314,14 @@ void launch_helmholtz_batch(int n, int nsys, double2* f, const double* alpha, do
                             hipStream_t s);
 void launch_solvep_batch(int ny, int nsys, double2* p, const double* alpha, const DevOps& ops,
                          double* scr, hipStream_t s);
-void launch_ctrl_law(int npts, const double* p2, const double* p3, double ca, double cb, double add,
-                     double sgn, double den, double* out, hipStream_t s);
+// out = sgn (ca p2 + cb p3 + add) / den, pointwise over npts
+struct CtrlLaw {
+  const double* p2;
+  const double* p3;
+  double ca, cb, add, sgn, den;
+  double* out;
+};
+void launch_ctrl_law(int npts, const CtrlLaw& w0, const CtrlLaw& w1, hipStream_t s);
 void launch_absmax(long n, const double* x, unsigned long long* out, hipStream_t s);
 void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, const double* dyh,
                           const unsigned long long* vmax, double eps, int dir, hipStream_t s);

@@ -75,11 +75,13 @@ __global__ void k_solvep_batch(int ny, int nsys, double2* p, const double* alpha
 }
 
 // Feedback law: out = (s0*(dmA*p2 + dmB*p3) + add) / den  (pointwise on physical planes)
-__global__ void k_ctrl_law(int npts, const double* p2, const double* p3, double ca, double cb,
-                           double add, double sgn, double den, double* out) {
+// the feedback law of both walls (blockIdx.y: law w0 or w1), pointwise on
+// the physical wall-adjacent planes
+__global__ void k_ctrl_law(int npts, CtrlLaw w0, CtrlLaw w1) {
+  const CtrlLaw w = blockIdx.y ? w1 : w0;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts) return;
-  out[i] = sgn * (ca * p2[i] + cb * p3[i] + add) / den;
+  w.out[i] = w.sgn * (w.ca * w.p2[i] + w.cb * w.p3[i] + w.add) / w.den;
 }
 
 // max |x| over n doubles into *out (bits of a nonnegative double)
@@ -125,10 +127,8 @@ void launch_solvep_batch(int ny, int nsys, double2* p, const double* alpha, cons
   LAUNCH_CHECK("k_solvep_batch");
 }
 
-void launch_ctrl_law(int npts, const double* p2, const double* p3, double ca, double cb, double add,
-                     double sgn, double den, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_ctrl_law, dim3((npts + 255) / 256), dim3(256), 0, s, npts, p2, p3, ca, cb, add,
-                     sgn, den, out);
+void launch_ctrl_law(int npts, const CtrlLaw& w0, const CtrlLaw& w1, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_law, dim3((npts + 255) / 256, 2), dim3(256), 0, s, npts, w0, w1);
   LAUNCH_CHECK("k_ctrl_law");
 }
 
