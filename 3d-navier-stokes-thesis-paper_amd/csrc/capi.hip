@@ -1785,8 +1785,10 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
     CHAN_CHECK(hipEventCreate(&b));
     double best = 1e300;
     Tuning bestt = c->tune;
+    bool saved = false;
     try {
       save(false);
+      saved = true;
       for (const Tuning& t : cand) {
         c->tune = t;
         step_body(c, 1, false);  // first launch of each variant, untimed
@@ -1807,6 +1809,12 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
       if (!(best < 1e300)) throw chan_error("autotune: every candidate's trial steps failed the CFL check");
     } catch (...) {
       c->tune = bestt;
+      if (saved) {  // put the state back where possible
+        try {
+          save(true);
+        } catch (...) {
+        }
+      }
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
       dfree(bk);
@@ -1930,6 +1938,37 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
   };
+  // every member's state: velocity, RK register, pressure, scalars, d_um_dy
+  auto sizes = [](chan_ctx* c) {
+    return std::array<size_t, 5>{vel_bytes(c), vel_bytes(c), p_bytes(c), sizeof(DevScalars),
+                                 sizeof(double) * (size_t)c->prm.ny};
+  };
+  auto ptrs = [](chan_ctx* c) {
+    return std::array<void*, 5>{c->d_vel, c->d_R, c->d_P, c->d_sc, c->d_dudy};
+  };
+  auto copy_state = [&](bool back) {
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      chan_ctx* c = e->m[i];
+      const auto sz = sizes(c);
+      const auto p = ptrs(c);
+      char* q = static_cast<char*>(bk[i]);
+      for (int k = 0; k < 5; ++k) {
+        if (back) d2d(c, p[k], q, sz[k]);
+        else d2d(c, q, p[k], sz[k]);
+        q += sz[k];
+      }
+      CHAN_CHECK(hipStreamSynchronize(c->stream));
+    }
+  };
+  bool saved = false;  // the backups hold every member's state
+  auto restore = [&] {  // on failure: put the saved state back where possible
+    if (!saved) return;
+    try {
+      if (e->master) (void)hipStreamSynchronize(e->master);
+      copy_state(true);
+    } catch (...) {
+    }
+  };
   try {
     if (steps < 1) throw chan_error("steps must be >= 1");
     if (!e->batched) throw chan_error("autotune applies to the batched launches (nstreams = 0); tune the members with chan_autotune");
@@ -1939,28 +1978,6 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
       c->h_dt_known = false;
       CHAN_CHECK(hipStreamSynchronize(c->stream));
     }
-    // every member's state: velocity, RK register, pressure, scalars, d_um_dy
-    auto sizes = [](chan_ctx* c) {
-      return std::array<size_t, 5>{vel_bytes(c), vel_bytes(c), p_bytes(c), sizeof(DevScalars),
-                                   sizeof(double) * (size_t)c->prm.ny};
-    };
-    auto ptrs = [](chan_ctx* c) {
-      return std::array<void*, 5>{c->d_vel, c->d_R, c->d_P, c->d_sc, c->d_dudy};
-    };
-    auto copy_state = [&](bool back) {
-      for (size_t i = 0; i < e->m.size(); ++i) {
-        chan_ctx* c = e->m[i];
-        const auto sz = sizes(c);
-        const auto p = ptrs(c);
-        char* q = static_cast<char*>(bk[i]);
-        for (int k = 0; k < 5; ++k) {
-          if (back) d2d(c, p[k], q, sz[k]);
-          else d2d(c, q, p[k], sz[k]);
-          q += sz[k];
-        }
-        CHAN_CHECK(hipStreamSynchronize(c->stream));
-      }
-    };
     for (chan_ctx* c : e->m) {
       const auto sz = sizes(c);
       bk.push_back(dalloc(sz[0] + sz[1] + sz[2] + sz[3] + sz[4]));
@@ -1991,6 +2008,7 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
     double best = 1e300;
     int bestm = xmul0, bestr = 0, bests = 0;
     copy_state(false);
+    saved = true;
     // the x-pass plane ranges with the default tile shapes, then the tile
     // shapes (one z-row per x-pass wavefront; 4 modes per solve wavefront)
     // with the best ranges
@@ -2021,6 +2039,7 @@ int chan_ensemble_autotune(chan_ensemble* e, int steps, double* best_ms) {
   } catch (const std::exception& err) {
     e->last_error = err.what();
   }
+  restore();
   cleanup();
   e->xmul = xmul0;
   e->xrows1 = xrows0;

@@ -294,7 +294,8 @@ int chan_ensemble_invalidate(chan_ensemble* ens);
  * the tile shapes (one z-row per x-pass wavefront at every length instead of
  * the batched default; 4 modes per solve wavefront everywhere instead of 8
  * where all 8 segments are busy), restores every member's state after each
- * trial, keeps the fastest plan and returns its time per step.  The ranges
+ * trial, keeps the fastest plan and returns its time per step (on failure
+ * the state is restored as well and the previous plan kept).  The ranges
  * give every member the same arithmetic; the tile shapes the same to
  * round-off. */
 int chan_ensemble_autotune(chan_ensemble* ens, int steps, double* best_ms_per_step);

@@ -165,8 +165,8 @@ def test_batched_ensemble_autotune_keeps_state_and_results():
     ref.step(2)
     for (g, _, a), (_, _, b) in zip(mem, twins):
         for x, y in zip(a.get_state(), b.get_state()):
-            scale = max(float(np.max(np.abs(y))), 1e-300)
-            assert float(np.max(np.abs(x - y))) <= 1e-13 * scale, (g.nx, g.ny, g.nz)
+            # round-off of the O(1) velocity carried into the O(1e-4) pressure
+            assert rel_l2(x, y) <= 1e-10, (g.nx, g.ny, g.nz)
     streams = cf.Ensemble([m[2] for m in mem[:2]], nstreams=1)
     with pytest.raises(cf.ChanError):
         streams.autotune(steps=1)
@@ -174,6 +174,30 @@ def test_batched_ensemble_autotune_keeps_state_and_results():
     ens.close()
     ref.close()
     for _, _, ch in mem + twins:
+        ch.close()
+
+
+def test_batched_ensemble_autotune_failure_restores_state():
+    """chan_ensemble_autotune when every plan's trial trips the CFL check (dt
+    far above the limit): it reports the failure, and every member's state
+    and scalars are what they were before the tuning."""
+    cf = __import__("helpers").chanflow()
+    draws = synth().ensemble_draw(3, seed=11)
+    mem = [_draw_member(d) for d in draws]
+    for _, _, ch in mem:
+        ch.set_scalars(dt=5.0)
+    ens = cf.Ensemble([m[2] for m in mem])
+    before = [m[2].get_state() for m in mem]
+    sc_before = [m[2].get_scalars() for m in mem]
+    with pytest.raises(cf.ChanError, match="failed the CFL check"):
+        ens.autotune(steps=1)
+    for (_, _, ch), st, sc in zip(mem, before, sc_before):
+        for x, y in zip(ch.get_state(), st):
+            assert np.array_equal(x, y)
+        after = ch.get_scalars()
+        assert after["dt"] == sc["dt"] and after["t_total"] == sc["t_total"]
+    ens.close()
+    for _, _, ch in mem:
         ch.close()
 
 
