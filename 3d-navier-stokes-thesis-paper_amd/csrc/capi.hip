@@ -251,6 +251,28 @@ void boundary_dev(chan_ctx* c) {
   const int ny = c->prm.ny;
   const long np = (long)g.nz2 * g.nx2;
   const double* dm = c->h_dm;
+  if (g.nx2 > c->prm.nx && g.nz2 > c->prm.nz) {
+    // The law is pointwise and affine in the wall-adjacent velocities, and on
+    // a dealiased grid the physical-space round trip (c2r, law, r2c) leaves a
+    // spectrum as it is except for the kx = 0 column, of which it keeps the
+    // Hermitian part: the law is applied to the Fourier coefficients with that
+    // projection, one launch for every controlled component and both walls.
+    CtrlSpec cs[2];
+    int n = 0;
+    for (int q = 0; q < 2; ++q) {
+      if (gains[q] == 0.0) continue;
+      const int comp = (q == 0) ? 0 : 2;
+      const double add = (q == 0) ? 2.0 : 0.0;
+      const double2* base = c->d_vel + (long)comp * c->nyp * nm;
+      // bottom: (dm(2,1) u2 + dm(3,1) u3 - add)/(gain - dm(1,1));
+      // top: -(dm(2,2) u_ny + dm(1,2) u_{ny-1} + add)/(gain + dm(3,2))
+      cs[n++] = CtrlSpec{base + 1 * nm, base + 2 * nm, base + (long)(ny - 1) * nm, base + (long)(ny - 2) * nm,
+                         dm[1], dm[2], -add, gains[q] - dm[0], dm[4], dm[3], add, gains[q] + dm[5],
+                         c->d_cbc + (1L * 3 + comp) * nm, c->d_cbc + (0L * 3 + comp) * nm};
+    }
+    launch_ctrl_spec(nm, c->prm.nz, cs[0], n == 2 ? cs[1] : cs[0], n, c->stream);
+    return;
+  }
   double* ph = (double*)stage(c, sizeof(double) * np * 6);
   const long zps = (long)g.nz2 * g.kxpad;
   for (int q = 0; q < 2; ++q) {

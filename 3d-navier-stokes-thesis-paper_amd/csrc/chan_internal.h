@@ -322,6 +322,18 @@ struct CtrlLaw {
   double* out;
 };
 void launch_ctrl_law(int npts, const CtrlLaw& w0, const CtrlLaw& w1, hipStream_t s);
+// The feedback law of one velocity component applied to its Fourier
+// coefficients: bottom (sgn +1) from planes 1 and 2, top (sgn -1) from planes
+// ny-1 and ny-2, each out = P(sgn (ca p2 + cb p3) / den) + sgn add / den at
+// the mean mode, where P keeps the Hermitian part of the kx = 0 column (what
+// the physical-space round trip of a dealiased grid leaves of a spectrum)
+struct CtrlSpec {
+  const double2 *p2b, *p3b, *p2t, *p3t;
+  double cab, cbb, addb, denb;
+  double cat, cbt, addt, dent;
+  double2 *outb, *outt;
+};
+void launch_ctrl_spec(long nm, int nz, const CtrlSpec& c0, const CtrlSpec& c1, int ncomp, hipStream_t s);
 void launch_absmax(long n, const double* x, unsigned long long* out, hipStream_t s);
 void launch_arnoldi_scale(int ny, long plane, double2* x, const double* dy, const double* dyh,
                           const unsigned long long* vmax, double eps, int dir, hipStream_t s);

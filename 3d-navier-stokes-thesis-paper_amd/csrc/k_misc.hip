@@ -127,6 +127,37 @@ void launch_solvep_batch(int ny, int nsys, double2* p, const double* alpha, cons
   LAUNCH_CHECK("k_solvep_batch");
 }
 
+// blockIdx.y: component c0 or c1; one thread per (kx, kz) mode m = ix nz + iz
+__global__ void k_ctrl_spec(long nm, int nz, CtrlSpec c0, CtrlSpec c1) {
+  const CtrlSpec c = blockIdx.y ? c1 : c0;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int iz = (int)(m % nz);
+  const bool kx0 = m < nz;
+  auto law = [&](const double2* p2, const double2* p3, double ca, double cb, double add, double sgn, double den) {
+    auto lin = [&](long k) {
+      const double2 a = p2[k], b = p3[k];
+      return make_double2(ca * a.x + cb * b.x, ca * a.y + cb * b.y);
+    };
+    double2 v = lin(m);
+    if (kx0 && iz == 0) {
+      v.y = 0.0;
+      v.x += add;
+    } else if (kx0 && 2 * iz != nz) {
+      const double2 w = lin(nz - iz);  // the partner mode -kz of the kx = 0 column
+      v = make_double2(0.5 * (v.x + w.x), 0.5 * (v.y - w.y));
+    }
+    return make_double2(sgn * v.x / den, sgn * v.y / den);
+  };
+  c.outb[m] = law(c.p2b, c.p3b, c.cab, c.cbb, c.addb, 1.0, c.denb);
+  c.outt[m] = law(c.p2t, c.p3t, c.cat, c.cbt, c.addt, -1.0, c.dent);
+}
+
+void launch_ctrl_spec(long nm, int nz, const CtrlSpec& c0, const CtrlSpec& c1, int ncomp, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_spec, dim3((unsigned)((nm + 255) / 256), ncomp), dim3(256), 0, s, nm, nz, c0, c1);
+  LAUNCH_CHECK("k_ctrl_spec");
+}
+
 void launch_ctrl_law(int npts, const CtrlLaw& w0, const CtrlLaw& w1, hipStream_t s) {
   hipLaunchKernelGGL(k_ctrl_law, dim3((npts + 255) / 256, 2), dim3(256), 0, s, npts, w0, w1);
   LAUNCH_CHECK("k_ctrl_law");

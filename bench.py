@@ -324,6 +324,8 @@ def main():
                     help="only run warmup + this many steps (for rocprof), print no JSON")
     ap.add_argument("--les-profile-steps", type=int, default=0,
                     help="only run this many steps of the learned-closure LES (for rocprof), print no JSON")
+    ap.add_argument("--jet-profile-steps", type=int, default=0,
+                    help="run only this many steps of the jet-control run (for a profiler) and exit")
     ap.add_argument("--ensemble-profile-steps", type=int, default=0,
                     help="only run this many steps of the 48-member batched ensemble (for rocprof), print no JSON")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -355,6 +357,10 @@ def main():
     if args.les_profile_steps:
         sps, fb = les_rate(cf, synth, c, args.les_profile_steps)
         print(f"LES {args.les_profile_steps} steps, {sps:.1f} steps/s, fallback fraction {fb:.3f}", flush=True)
+        return
+    if args.jet_profile_steps:
+        jsps, _, _ = jet_control_rate(cf, synth, args.jet_profile_steps)
+        print(f"jet control {args.jet_profile_steps} steps, {jsps:.1f} steps/s", flush=True)
         return
     ch, y, yh, lx, lz = make_channel(cf, c)
     U0 = synth.turbulent_like(c["nx"], c["ny"], c["nz"], yh, y, seed=c["seed"], rms=c["rms"], re_tau=c["re_tau"])

@@ -429,6 +429,31 @@ def test_feedback_control(g0):
     ch.close()
 
 
+@pytest.mark.parametrize("dims", [(32, 33, 32, 48, 48), (32, 33, 32, 32, 32), (16, 33, 24, 24, 36)])
+def test_feedback_control_grids(dims):
+    """The feedback law of boundary() (PFC/read_write.f90:1465-1532) on
+    dealiased grids, where the library applies it to the Fourier
+    coefficients (the physical-space round trip there keeps a spectrum except
+    for the kx = 0 column's Hermitian part), and on an undealiased grid, where
+    it goes through the physical wall-adjacent planes: 3 steps against the
+    oracle's physical-space law, both gains on, at 1e-11."""
+    nx, ny, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, ny, nz, nx2, nz2, 2.0, 2.0, 2500.0, stretch=1.5, fix_mass=1, dt_flag=0,
+                    ctrlu=0.4, ctrlw=0.7)
+    s = __import__("helpers").synth()
+    U0 = s.laminar_perturbed(nx, ny, nz, g.yh, g.y, seed=31, amp=1e-2, kmax=4.0)
+    ch = make_channel(g)
+    ch.set_state(data1=U0)
+    ch.set_scalars(dt=0.01)
+    ch.initialize_flow()
+    st = O.initial_state(g, U0, dt=0.01)
+    ch.step(3, step0=1)
+    for k in (1, 2, 3):
+        O.rk3_step(g, st, k)
+    assert rel_l2(ch.get_data1(), st.U) < 1e-11
+    ch.close()
+
+
 def test_linearized_matvec():
     """Arnoldi matvec compute_time_deriv on a small grid."""
     g = O.make_grid(16, 33, 16, 24, 24, 2.0, 2.0, 2800.0, stretch=2.75, fix_mass=0)

@@ -577,3 +577,31 @@ def test_l2_norm_closed_form():
     U[0, :, 1, 0] = a / 2
     l2 = O.l2_norm(g, U)
     assert abs(l2 - math.sqrt(a * a / 2 * float(np.sum(g.dy)) / 2)) <= 1e-14
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 48, 48), (48, 32, 72, 48), (32, 16, 36, 18), (64, 64, 64, 64)])
+def test_physical_round_trip_is_the_kx0_hermitian_projection(dims):
+    """xz_r2c(xz_c2r(C)) on a dealiased grid (nx2 > nx, nz2 > nz) keeps every
+    coefficient except those of the kx = 0 column, of which it keeps the
+    Hermitian part: C(0, kz) -> (C(0, kz) + conj C(0, -kz)) / 2, the real part
+    at kz = 0, the z-Nyquist coefficient as it is.  The device applies the
+    (affine, pointwise) wall feedback law to Fourier coefficients with this
+    projection instead of through the physical planes; on an undealiased grid
+    (the last case) the round trip is not this projection, and the device
+    keeps the physical path there."""
+    nx, nz, nx2, nz2 = dims
+    g = O.make_grid(nx, 17, nz, nx2, nz2, 2.0, 2.0, 2800.0, stretch=2.75)
+    rng = np.random.default_rng(3)
+    C = rng.standard_normal((3, g.nxh, g.nz)) + 1j * rng.standard_normal((3, g.nxh, g.nz))
+    R = O.xz_r2c(g, O.xz_c2r(g, C))
+    P = C.copy()
+    h = nz // 2
+    P[:, 0, 0] = C[:, 0, 0].real
+    for iz in range(1, nz):
+        if iz != h:
+            P[:, 0, iz] = 0.5 * (C[:, 0, iz] + np.conj(C[:, 0, nz - iz]))
+    err = np.abs(R - P).max()
+    if nx2 > nx and nz2 > nz:
+        assert err <= 1e-14
+    else:
+        assert err > 1e-3
