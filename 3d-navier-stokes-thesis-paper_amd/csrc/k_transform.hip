@@ -561,19 +561,24 @@ void launch_zfwd_batched(int nz2, const ZBatchArgs* args, const BatchItem* items
   LAUNCH_CHECK("k_zfwd2_b");
 }
 
-static int rows_for(int M, int nz2, int nf) {
+// Rows per workgroup of the x c2r / r2c kernels: the largest divisor of nz2
+// whose nf * rows * M points fit 3 per thread of a 512-thread workgroup
+// (1 per thread for the runtime-planned transform, whose staging is bounded),
+// so that the transform passes of short rows still give most threads a task.
+static int rows_for(int M, int nz2, int nf, bool ct) {
+  const int cap = (ct ? 1536 : 512) / nf;
   int best = 1;
   for (int d = 1; d <= nz2; ++d)
-    if (nz2 % d == 0 && d * M <= 512 / nf) best = d;
+    if (nz2 % d == 0 && d * M <= cap) best = d;
   return best;
 }
 
 void launch_xc2r(const XZGeom& g, const double2* tw, const double2* Z, long z_ps, double* out,
                  long out_ps, int nplanes, hipStream_t s) {
   const int M = g.nx2 / 2;
-  int rows = rows_for(M, g.nz2, 1);
-  int nthr = std::min(512, std::max(64, ((rows * M + 63) / 64) * 64));
   const bool ct = fft2_split(M).n1 > 0;
+  int rows = rows_for(M, g.nz2, 1, ct);
+  int nthr = std::min(512, std::max(64, ((rows * M + 63) / 64) * 64));
   if (!ct) check_stage_fit(g.px, rows, nthr);
   dim3 grid(g.nz2 / rows, nplanes);
   size_t lds = sizeof(double2) * (ct ? rows * fft_seq_cells(M) + M : rows * (M + 1));
@@ -587,9 +592,9 @@ void launch_xr2c(const XZGeom& g, const double2* tw, const double* fy, const dou
                  const double* kx, long in_ps, double2* out, long out_ps, int nplanes, hipStream_t s) {
   const int M = g.nx2 / 2;
   const int nf = fx ? 2 : 1;
-  int rows = rows_for(M, g.nz2, nf);
-  int nthr = std::min(512, std::max(64, ((nf * rows * M + 63) / 64) * 64));
   const bool ct = fft2_split(M).n1 > 0;
+  int rows = rows_for(M, g.nz2, nf, ct);
+  int nthr = std::min(512, std::max(64, ((nf * rows * M + 63) / 64) * 64));
   if (!ct) check_stage_fit(g.px, nf * rows, nthr);
   dim3 grid(g.nz2 / rows, nplanes);
   size_t lds = sizeof(double2) * (ct ? nf * rows * fft_seq_cells(M) + M : nf * rows * (M + 1));
