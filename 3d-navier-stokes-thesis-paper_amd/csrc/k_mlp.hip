@@ -97,26 +97,28 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ pac
 }
 
 // GELU for the bf16 inference path, without transcendental instructions or
-// branches: GELU(x) = x/2 + |x| e(t), e(t) = erf(t/sqrt 2)/2 at t = min(|x|, 4.5),
-// with e a degree-9 polynomial in u = 2t/4.5 - 1 (a least-squares Chebyshev
-// fit weighted by t, made for this kernel, converted to monomials in u).  In
-// float32 its GELU error is <= 6.4e-5 absolute for |x| <= 8 and below 1e-5
-// relative beyond (erf(4.5/sqrt 2) differs from 1 by 7e-6), far inside the
-// 2^-9 relative rounding of the bf16 result the value feeds.  11 VALU
-// operations per value (|x| is a source modifier).
+// branches: GELU(x) = x/2 + |x| e(w), e(w) = erf(4.5 w/sqrt 2)/2 at
+// w = min(|x|/4.5, 1), with e a degree-9 polynomial in w (a least-squares
+// Chebyshev fit weighted by t = 4.5 w, made for this kernel, converted to
+// monomials in w).  In float32 with fused multiply-adds its GELU error is
+// <= 6.3e-5 absolute for |x| <= 8 and below 1e-5 relative beyond
+// (erf(4.5/sqrt 2) differs from 1 by 7e-6), far inside the 2^-9 relative
+// rounding of the bf16 result the value feeds.  12 VALU operations per value:
+// w is one multiply with the clamp-to-[0,1] output modifier (|x| is a source
+// modifier), then 9 fused multiply-adds, the multiply by 1/2 and the last
+// fused multiply-add.
 __device__ __forceinline__ float gelu_bf(float x) {
-  const float t = fminf(fabsf(x), 4.5f);
-  const float u = fmaf(t, 2.0f / 4.5f, -1.0f);
-  float e = 2.060715482e-02f;
-  e = fmaf(e, u, -3.009064123e-02f);
-  e = fmaf(e, u, -5.218758062e-02f);
-  e = fmaf(e, u, 1.301724166e-01f);
-  e = fmaf(e, u, -3.599490970e-02f);
-  e = fmaf(e, u, -1.570081264e-01f);
-  e = fmaf(e, u, 2.462200224e-01f);
-  e = fmaf(e, u, -1.808419824e-01f);
-  e = fmaf(e, u, 7.135432214e-02f);
-  e = fmaf(e, u, 4.877772331e-01f);
+  const float w = __builtin_amdgcn_fmed3f(fabsf(x) * (1.0f / 4.5f), 0.0f, 1.0f);
+  float e = 1.055086327e+01f;
+  e = fmaf(e, w, -5.518209076e+01f);
+  e = fmaf(e, w, 1.190905762e+02f);
+  e = fmaf(e, w, -1.329954224e+02f);
+  e = fmaf(e, w, 7.579547882e+01f);
+  e = fmaf(e, w, -1.441165447e+01f);
+  e = fmaf(e, w, -3.997066498e+00f);
+  e = fmaf(e, w, -1.498871893e-01f);
+  e = fmaf(e, w, 1.799196482e+00f);
+  e = fmaf(e, w, 9.890630281e-06f);
   return fmaf(fabsf(x), e, 0.5f * x);
 }
 
