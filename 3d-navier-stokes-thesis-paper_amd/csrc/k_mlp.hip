@@ -137,7 +137,17 @@ __device__ __forceinline__ f32x4 bias_tile(const float* __restrict__ bias, int o
 
 #define MLP_NT 3  // point tiles of 16 per wavefront (each weight operand read from LDS serves all of them)
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ packed, const float* __restrict__ p, int n,
+// Two workgroups per compute unit (their 2 x 75 KB of LDS fill it): the bound
+// keeps the kernel within 256 registers, so it runs at two wavefronts per SIMD
+// with its accumulators in VGPRs (175, no AGPRs, no spills), instead of 196
+// VGPRs + 72 AGPRs at one wavefront per SIMD with a copy out of the AGPRs
+// before every GELU (114 -> 103 us at 589 824 points).  Six-wavefront
+// workgroups, three wavefronts per SIMD, are slower (120 us).  What bounds the
+// kernel is its vector work: per 48 points 3666 VALU instructions (the GELU's
+// 12 per value over 3 x 128 values, at 4 cycles per wavefront instruction
+// without packed FP32) against 228 matrix products.
+#define MLP_WAVES 4
+__global__ __launch_bounds__(64 * MLP_WAVES, 2) void k_mlp_infer(const bf16x4* __restrict__ packed, const float* __restrict__ p, int n,
                                                    const float* __restrict__ x, float* __restrict__ y) {
   __shared__ __align__(16) bf16x4 W[MLP_NBLK * 64];
   const bf16x8* W8 = reinterpret_cast<const bf16x8*>(W);
@@ -153,29 +163,52 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nw = gridDim.x * (blockDim.x >> 6);
   const int q = lane >> 4, col = lane & 15;
+  // features 4q..4q+3 of the points b + 16 nt + col (zero past n and past
+  // feature 10), loaded without branches from clamped in-range addresses (n >= 1)
+  float xv[MLP_NT][4];
+  auto load_x = [&](long b) {
+#pragma unroll
+    for (int nt = 0; nt < MLP_NT; ++nt) {
+      const long pt = b + 16 * nt + col;
+      const float* xp = x + (pt < n ? pt : (long)n - 1) * 10;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int f = 4 * q + t;
+        const float v = xp[f < 10 ? f : 9];
+        xv[nt][t] = (pt < n && f < 10) ? v : 0.0f;
+      }
+    }
+  };
   for (long base = ((long)blockIdx.x * (blockDim.x >> 6) + wv) * 16 * MLP_NT; base < n; base += (long)nw * 16 * MLP_NT) {
     bf16x4 h[MLP_NT][8];
     // layer 1: B operand = features 4q..4q+3 of point base + 16 nt + col
+    load_x(base);
 #pragma unroll
-    for (int nt = 0; nt < MLP_NT; ++nt) {
-      const long pt = base + 16 * nt + col;
-      float v[4];
-      for (int t = 0; t < 4; ++t) {
-        const int f = 4 * q + t;
-        v[t] = (pt < n && f < 10) ? x[pt * 10 + f] : 0.0f;
-      }
-      h[nt][0] = to_bf16x4(v[0], v[1], v[2], v[3]);
-    }
-    f32x4 acc[MLP_NT][1];
+    for (int nt = 0; nt < MLP_NT; ++nt) h[nt][0] = to_bf16x4(xv[nt][0], xv[nt][1], xv[nt][2], xv[nt][3]);
+    // Each layer is software-pipelined over its 8 output tiles: the matrix
+    // products of tile ot+1 are issued before the GELU of tile ot, so that the
+    // wavefront's vector instructions run while its products are in flight.
+    f32x4 acc[2][MLP_NT];
     bf16x4 g[MLP_NT][8];
-#pragma unroll
-    for (int ot = 0; ot < 8; ++ot) {
+    auto layer1 = [&](int ot, f32x4 (&o)[MLP_NT]) {
       const bf16x4 a = W[(MLP_BLK1 + ot) * 64 + lane];
       const f32x4 b0 = bias_tile(bias, ot, lane);
 #pragma unroll
-      for (int nt = 0; nt < MLP_NT; ++nt) {
-        acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[nt][0], b0, 0, 0, 0);
-        g[nt][ot] = act_tile(acc[nt][0]);
+      for (int nt = 0; nt < MLP_NT; ++nt) o[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[nt][0], b0, 0, 0, 0);
+    };
+    layer1(0, acc[0]);
+#pragma unroll
+    for (int ot = 0; ot < 8; ++ot) {
+      if (ot + 1 < 8) layer1(ot + 1, acc[(ot + 1) & 1]);
+#pragma unroll
+      for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[ot & 1][nt]);
+      if (ot + 1 < 8) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+        for (int i = 0; i < MLP_NT; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, 48, 0);
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -188,21 +221,34 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x4* __restrict__ pa
     for (int layer = 0; layer < 2; ++layer) {
       const int blk0 = layer == 0 ? MLP_BLK2 : MLP_BLK3;
       const float* bl = bias + 128 * (layer + 1);
-#pragma unroll
-      for (int ot = 0; ot < 8; ++ot) {
+      auto products = [&](int ot, f32x4 (&o)[MLP_NT]) {
         const f32x4 b0 = bias_tile(bl, ot, lane);
 #pragma unroll
-        for (int nt = 0; nt < MLP_NT; ++nt) acc[nt][0] = b0;
+        for (int nt = 0; nt < MLP_NT; ++nt) o[nt] = b0;
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
           const bf16x8 a = W8[(blk0 + (ot * 4 + kb) * 2) * 32 + lane];
 #pragma unroll
           for (int nt = 0; nt < MLP_NT; ++nt)
-            acc[nt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cat8(h[nt][2 * kb], h[nt][2 * kb + 1]),
-                                                                 acc[nt][0], 0, 0, 0);
+            o[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cat8(h[nt][2 * kb], h[nt][2 * kb + 1]), o[nt], 0, 0, 0);
         }
+      };
+      products(0, acc[0]);
 #pragma unroll
-        for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[nt][0]);
+      for (int ot = 0; ot < 8; ++ot) {
+        if (ot + 1 < 8) products(ot + 1, acc[(ot + 1) & 1]);
+#pragma unroll
+        for (int nt = 0; nt < MLP_NT; ++nt) g[nt][ot] = act_tile(acc[ot & 1][nt]);
+        if (ot + 1 < 8) {
+          // the weight and bias reads, then each product followed by its
+          // share of the GELU's vector instructions
+          __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
+#pragma unroll
+          for (int i = 0; i < 4 * MLP_NT; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 12, 0);
+          }
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
@@ -467,7 +513,7 @@ static int mlp_blocks(long n) {
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
     cus = 256;
   const long tiles = (n + 16 * MLP_NT - 1) / (16 * MLP_NT);
-  return (int)std::max<long>(1, std::min<long>(2L * cus, (tiles + 3) / 4));
+  return (int)std::max<long>(1, std::min<long>(2L * cus, (tiles + MLP_WAVES - 1) / MLP_WAVES));
 }
 
 void mlp_prepare_device(chan_mlp* m) {
@@ -481,7 +527,7 @@ void mlp_infer_device(chan_mlp* m, long n, const float* x, float* y, hipStream_t
   if (!m->packed_ok) mlp_prepare_device(m);
   if (n <= 0) return;
   if (n > 0x7fffffffL) throw chan_error("too many points for one inference");
-  hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(256), 0, s, m->d_packed, m->d_p, (int)n, x, y);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(64 * MLP_WAVES), 0, s, m->d_packed, m->d_p, (int)n, x, y);
   LAUNCH_CHECK("k_mlp_infer");
 }
 
@@ -553,7 +599,7 @@ int chan_mlp_infer(chan_mlp* m, int n, const float* x, float* y) {
     ensure_cap(m, n);
     pack(m);
     mchk(hipMemcpyAsync(m->d_x, x, sizeof(float) * 10 * n, hipMemcpyHostToDevice, m->stream), "copy");
-    hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(256), 0, m->stream, m->d_packed, m->d_p, n, m->d_x,
+    hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n)), dim3(64 * MLP_WAVES), 0, m->stream, m->d_packed, m->d_p, n, m->d_x,
                        m->d_y);
     LAUNCH_CHECK("k_mlp_infer");
     mchk(hipMemcpyAsync(y, m->d_y, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, m->stream), "copy");
