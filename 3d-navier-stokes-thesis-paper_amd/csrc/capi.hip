@@ -1768,10 +1768,12 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
       cp(bk + 2 * vb + pb + sizeof(DevScalars), c->d_dudy, db);
       CHAN_CHECK(hipStreamSynchronize(c->stream));
     };
-    // Candidates: the x-pass kernel and, for the wavefront kernel, its z-rows
-    // per wavefront where the length has them; the z-forward kernel; the
-    // segmented solve's layouts (4, 2 or 8 modes per wavefront where ny + 1
-    // rows fit) or the sweep-based solves with their block shapes.
+    // Candidates: the x-pass kernel (the wavefront kernel only when a
+    // subgrid closure is configured: it alone adds the stress to the fluxes)
+    // and, for the wavefront kernel, its z-rows per wavefront where the
+    // length has them; the z-forward kernel; the segmented solve's layouts
+    // (4, 2 or 8 modes per wavefront where ny + 1 rows fit) or the
+    // sweep-based solves with their block shapes.
     const int M = c->gd.nx2 / 2;
     const bool pf = c->tune.xpass_prefetch == 1 && M <= 144;
     std::vector<int> rows_c = {0};
@@ -1785,6 +1787,7 @@ int chan_autotune(chan_ctx* c, int steps, double* best_ms) {
     std::vector<Tuning> cand;
     for (int w : {1, 0})
       for (int z : {1, 0}) {
+        if (w == 0 && c->les) continue;
         for (int sg : seg_c)
           for (int r : rows_c) {
             if (r != 0 && w != 1) continue;  // rows per wavefront: the wavefront kernel only

@@ -9,7 +9,10 @@ Tolerances (FP64 unless noted):
   * network inputs (float32): max abs difference <= 2e-6;
   * learned stress with the gate, from the device's own network outputs:
     relative L2 <= 1e-12, same number of fallback points;
-  * 5 RK3 steps with the dynamic model: relative L2 <= 1e-10.
+  * 5 RK3 steps with the dynamic model: relative L2 <= 1e-10;
+  * chan_autotune with a closure configured: state restored bit for bit, and
+    the tuned channel's next steps within relative L2 1e-10 of an untuned
+    twin's (a different launch shape may reorder sums).
 """
 import numpy as np
 import pytest
@@ -270,3 +273,27 @@ def test_filtered_dns_on_device():
     les.les_filter_from(dns, tau=False)
     assert rel_l2(les.get_data1(), Ur) <= 1e-12
     dns.close(); les.close()
+
+
+def test_autotune_with_a_closure_configured(g0):
+    """chan_autotune on a channel with the dynamic closure: only the x-pass
+    kernel that adds the stress is a candidate; the state comes back as it
+    was and the chosen launch shapes give the same steps as the defaults."""
+    U = init_cfg0(O, g0, seed=14, amp=0.05)
+    a = _chan_with(g0, U, "dynamic")
+    b = _chan_with(g0, U, "dynamic")
+    a.set_scalars(dt=0.01, p_grad=-2.0 / g0.re)
+    b.set_scalars(dt=0.01, p_grad=-2.0 / g0.re)
+    a.initialize_flow()
+    b.initialize_flow()
+    before = a.get_state()
+    assert a.autotune(steps=1) > 0.0
+    for x, y in zip(before, a.get_state()):
+        assert np.array_equal(x, y)
+    assert a.get_tuning()["xpass_wave"] == 1
+    a.step(2, step0=1)
+    b.step(2, step0=1)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert rel_l2(x, y) <= 1e-10
+    a.close()
+    b.close()

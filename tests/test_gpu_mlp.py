@@ -33,11 +33,15 @@ def test_param_roundtrip_and_count():
     net.close()
 
 
-def test_bf16_inference_matches_restatement():
-    net, W, b, X, T = _setup(n=1000)   # not a multiple of the 32-point tile
+@pytest.mark.parametrize("n", [1, 47, 1000])
+def test_bf16_inference_matches_restatement(n):
+    # a wavefront takes 48 points at a time: one point, one short of a batch,
+    # and a count with a ragged last batch
+    net, W, b, X, T = _setup(n=n)
     y = net.infer(X)
     ref = L.forward_bf16(W, b, X)
     scale = np.abs(ref).max()
+    assert y.shape == (n, 6)
     assert np.abs(y - ref).max() <= 2e-2 * scale
     # and close to the exact network
     assert np.abs(y - L.forward(W, b, X)).max() <= 5e-2 * scale
