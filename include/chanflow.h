@@ -258,8 +258,9 @@ int chan_jet_configure(chan_ctx* ctx, int wall, double amplitude, double radius,
  * nx2/2 <= 72; 4 for nx2/2 <= 24; with the prefetch, where nz2 is a multiple;
  * otherwise the default applies; a batched ensemble takes the fewest rows its
  * members of one length choose).
- * chan_autotune times `steps` timesteps of every candidate (keys 0 and 2;
- * with the wavefront x-pass, key 12's row counts the length has; the
+ * chan_autotune times `steps` timesteps of every candidate (keys 0 and 2,
+ * key 0 fixed at 1 while a subgrid closure is configured; with the wavefront
+ * x-pass, key 12's row counts the length has; the
  * segmented solve's layouts of key 11 that ny + 1 rows fit, or the
  * sweep-based solves with keys 3, 4 and 8), keeps the fastest whose trial
  * passed the CFL check and restores the state. */
