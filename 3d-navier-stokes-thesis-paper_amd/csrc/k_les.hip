@@ -104,11 +104,17 @@ __device__ __forceinline__ double contract(const double* a, const double* b) {
 // u_i u_j, |S| S_ij at every halo point; every point of the tile filters
 // them, forms L_ij = T(u_i u_j) - T(u_i) T(u_j),
 // M_ij = 2 D^2 (T(|S| S_ij) - 4 |S^| S^_ij), and the workgroup sums L:M and
-// M:M (partials per tile, reduced per plane by k_les_cs2).
+// M:M (partials per tile, reduced per plane by k_les_cs2).  With FEAT it also
+// writes the network inputs of its tile's points from the gradients it holds
+// in LDS, as k_les_feat does (same arithmetic), so that the learned closure
+// needs no separate pass over the gradients where C(y) is recomputed.
 #define DYN_T 16
 #define DYN_H (DYN_T + 2)
+template <bool FEAT>
 __global__ __launch_bounds__(256) void k_les_dyn(int ny, int nz2, int nx2, const double* __restrict__ Ph,
-                                                 const double* __restrict__ delta2, double* __restrict__ part) {
+                                                 const double* __restrict__ delta2, double* __restrict__ part,
+                                                 const double* __restrict__ ydist, float* __restrict__ X,
+                                                 double* __restrict__ scale) {
   __shared__ double F[24][DYN_H * DYN_H];
   __shared__ double red[2][256];
   const int kk = blockIdx.y, nyp = ny + 1;
@@ -166,6 +172,22 @@ __global__ __launch_bounds__(256) void k_les_dyn(int ny, int nz2, int nx2, const
       }
       lm = contract(L, M);
       mm = contract(M, M);
+      if (FEAT) {
+        const int hp = (lz + 1) * DYN_H + lx + 1;
+        double gp[9], n2 = 0.0;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+          gp[f] = F[f][hp];
+          n2 += gp[f] * gp[f];
+        }
+        const double gn = sqrt(n2);
+        const double inv = gn > 0.0 ? 1.0 / gn : 0.0;
+        const long p = (long)(kk - 1) * ps + (long)(tz0 + lz) * nx2 + tx0 + lx;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) X[p * 10 + f] = (float)(gp[f] * inv);
+        X[p * 10 + 9] = (float)ydist[kk];
+        scale[p] = delta2[kk] * n2;
+      }
     }
   }
   red[0][threadIdx.x] = lm;
@@ -324,7 +346,8 @@ void les_c2r(chan_ctx* c, LesState* L, const double2* in, int np, double* out) {
 }
 
 // G, its physical fields Ph and (dynamic) the dynamic coefficient
-void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
+// (with `feat`, the dynamic procedure also writes the network inputs)
+void les_fields(chan_ctx* c, LesState* L, bool dynamic, bool feat = false) {
   const int ny = c->prm.ny, nyp = c->nyp;
   dim3 gm((unsigned)((L->nm + 255) / 256), nyp);
   hipLaunchKernelGGL(k_les_grad, gm, dim3(256), 0, c->stream, c->ops, ny, c->nxh, c->prm.nz, c->d_vel, L->G);
@@ -332,8 +355,12 @@ void les_fields(chan_ctx* c, LesState* L, bool dynamic) {
   les_c2r(c, L, L->G, LES_NF * nyp, L->Ph);
   if (!dynamic) return;
   const XZGeom& g = c->gd;
-  hipLaunchKernelGGL(k_les_dyn, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, g.nz2, g.nx2, L->Ph, L->delta2,
-                     L->part);
+  if (feat)
+    hipLaunchKernelGGL(k_les_dyn<true>, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, g.nz2, g.nx2, L->Ph,
+                       L->delta2, L->part, L->ydist, L->X, L->scale);
+  else
+    hipLaunchKernelGGL(k_les_dyn<false>, dim3(L->nblk, nyp), dim3(256), 0, c->stream, ny, g.nz2, g.nx2, L->Ph,
+                       L->delta2, L->part, nullptr, nullptr, nullptr);
   LAUNCH_CHECK("k_les_dyn");
   hipLaunchKernelGGL(k_les_cs2, dim3((nyp + 63) / 64), dim3(64), 0, c->stream, ny, L->nblk, L->part, L->cs2);
   LAUNCH_CHECK("k_les_cs2");
@@ -345,7 +372,7 @@ void les_compute(chan_ctx* c, LesState* L, bool dyn) {
   const int ny = c->prm.ny, nyp = c->nyp;
   if (L->mode == 3) return;
   const bool d = dyn || !L->cs2_valid;
-  les_fields(c, L, d);
+  les_fields(c, L, d, d && L->mode == 2);
   L->cs2_valid = true;
   dim3 gp((unsigned)((L->ps + 255) / 256), nyp);
   if (L->mode == 1) {
@@ -354,9 +381,11 @@ void les_compute(chan_ctx* c, LesState* L, bool dyn) {
     return;
   }
   dim3 gi((unsigned)((L->ps + 255) / 256), ny - 1);
-  hipLaunchKernelGGL(k_les_feat, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->ydist, L->delta2, L->X,
-                     L->scale);
-  LAUNCH_CHECK("k_les_feat");
+  if (!d) {  // otherwise k_les_dyn wrote the network inputs
+    hipLaunchKernelGGL(k_les_feat, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->ydist, L->delta2, L->X,
+                       L->scale);
+    LAUNCH_CHECK("k_les_feat");
+  }
   mlp_infer_device(L->net, L->npts, L->X, L->Y, c->stream);
   hipLaunchKernelGGL(k_les_tau_mlp, gi, dim3(256), 0, c->stream, ny, L->ps, L->Ph, L->Y, L->scale, L->cs2,
                      L->delta2, L->beta, L->tau, L->gated);
