@@ -88,44 +88,51 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x4* __restrict__ pac
     row = ot * 16 + (L & 15);
     f0 = 32 * kb + 16 * part + 4 * (L >> 4);
   }
+  // the hidden layers' weights halved (exactly: a power of two), so that
+  // their products are x/2 for the GELU (gelu_half)
+  const float h = blk < MLP_BLK4 ? 0.5f : 1.0f;
   float w[4];
   for (int t = 0; t < 4; ++t) {
     const int k = f0 + t;
-    w[t] = (row < nout && k < nin) ? p[wofs + row * nin + k] : 0.0f;
+    w[t] = (row < nout && k < nin) ? h * p[wofs + row * nin + k] : 0.0f;
   }
   packed[blk * 64 + l] = to_bf16x4(w[0], w[1], w[2], w[3]);
 }
 
 // GELU for the bf16 inference path, without transcendental instructions or
-// branches: GELU(x) = x/2 + |x| e(w), e(w) = erf(4.5 w/sqrt 2)/2 at
-// w = min(|x|/4.5, 1), with e a degree-9 polynomial in w (a least-squares
+// branches, from y = x/2: the hidden layers run with halved weights and
+// biases, which halves every product, sum and rounding exactly (a power of
+// two), so their accumulators hold x/2 bit for bit.  GELU(x) = x/2 + |x| e(w)
+// = y + |y| 2e(w), e(w) = erf(4.5 w/sqrt 2)/2 at w = min(|y|/2.25, 1) =
+// min(|x|/4.5, 1), with 2e a degree-9 polynomial in w (a least-squares
 // Chebyshev fit weighted by t = 4.5 w, made for this kernel, converted to
-// monomials in w).  In float32 with fused multiply-adds its GELU error is
-// <= 6.3e-5 absolute for |x| <= 8 and below 1e-5 relative beyond
-// (erf(4.5/sqrt 2) differs from 1 by 7e-6), far inside the 2^-9 relative
-// rounding of the bf16 result the value feeds.  12 VALU operations per value:
-// w is one multiply with the clamp-to-[0,1] output modifier (|x| is a source
-// modifier), then 9 fused multiply-adds, the multiply by 1/2 and the last
-// fused multiply-add.
-__device__ __forceinline__ float gelu_bf(float x) {
-  const float w = __builtin_amdgcn_fmed3f(fabsf(x) * (1.0f / 4.5f), 0.0f, 1.0f);
-  float e = 1.055086327e+01f;
-  e = fmaf(e, w, -5.518209076e+01f);
-  e = fmaf(e, w, 1.190905762e+02f);
-  e = fmaf(e, w, -1.329954224e+02f);
-  e = fmaf(e, w, 7.579547882e+01f);
-  e = fmaf(e, w, -1.441165447e+01f);
-  e = fmaf(e, w, -3.997066498e+00f);
-  e = fmaf(e, w, -1.498871893e-01f);
-  e = fmaf(e, w, 1.799196482e+00f);
-  e = fmaf(e, w, 9.890630281e-06f);
-  return fmaf(fabsf(x), e, 0.5f * x);
+// monomials in w and doubled).  Doubling the coefficients and halving y are
+// exact, so the result is that of the same polynomial on x, rounded once at
+// the last fused multiply-add.  In float32 its GELU error is <= 6.3e-5
+// absolute for |x| <= 8 and below 1e-5 relative beyond (erf(4.5/sqrt 2)
+// differs from 1 by 7e-6), far inside the 2^-9 relative rounding of the bf16
+// result the value feeds.  11 VALU operations per value: w is one multiply
+// with the clamp-to-[0,1] output modifier (|y| is a source modifier), then
+// 9 fused multiply-adds and the last fused multiply-add.
+__device__ __forceinline__ float gelu_half(float y) {
+  const float w = __builtin_amdgcn_fmed3f(fabsf(y) * (1.0f / 2.25f), 0.0f, 1.0f);
+  float e = 2.110172653e+01f;
+  e = fmaf(e, w, -1.103641815e+02f);
+  e = fmaf(e, w, 2.381811523e+02f);
+  e = fmaf(e, w, -2.659908447e+02f);
+  e = fmaf(e, w, 1.515909576e+02f);
+  e = fmaf(e, w, -2.882330894e+01f);
+  e = fmaf(e, w, -7.994132996e+00f);
+  e = fmaf(e, w, -2.997743785e-01f);
+  e = fmaf(e, w, 3.598392963e+00f);
+  e = fmaf(e, w, 1.978126056e-05f);
+  return fmaf(fabsf(y), e, y);
 }
 
-// GELU on a result tile (its bias already in the accumulator), rounded to
-// bf16 as the next B operand.
+// GELU on a result tile of a hidden layer (x/2, its halved bias already in
+// the accumulator), rounded to bf16 as the next B operand.
 __device__ __forceinline__ bf16x4 act_tile(f32x4 acc) {
-  return to_bf16x4(gelu_bf(acc[0]), gelu_bf(acc[1]), gelu_bf(acc[2]), gelu_bf(acc[3]));
+  return to_bf16x4(gelu_half(acc[0]), gelu_half(acc[1]), gelu_half(acc[2]), gelu_half(acc[3]));
 }
 
 // The bias of rows 4(lane/16)..+3 of output tile ot: the accumulator's
@@ -154,9 +161,9 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 2) void k_mlp_infer(const bf16x4* _
   __shared__ float bias[128 * 3 + 16];
   for (int i = threadIdx.x; i < MLP_NBLK * 64; i += blockDim.x) W[i] = packed[i];
   for (int i = threadIdx.x; i < 128; i += blockDim.x) {
-    bias[i] = p[MLP_B1 + i];
-    bias[128 + i] = p[MLP_B2 + i];
-    bias[256 + i] = p[MLP_B3 + i];
+    bias[i] = 0.5f * p[MLP_B1 + i];  // halved with the hidden layers' weights
+    bias[128 + i] = 0.5f * p[MLP_B2 + i];
+    bias[256 + i] = 0.5f * p[MLP_B3 + i];
   }
   if (threadIdx.x < 16) bias[384 + threadIdx.x] = threadIdx.x < 6 ? p[MLP_B4 + threadIdx.x] : 0.0f;
   __syncthreads();

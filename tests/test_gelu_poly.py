@@ -1,5 +1,6 @@
-"""The branch-free GELU of the bf16 inference kernel (gelu_bf in
-csrc/k_mlp.hip), evaluated in float32 exactly as the kernel writes it from the
+"""The branch-free GELU of the bf16 inference kernel (gelu_half in
+csrc/k_mlp.hip, which takes y = x/2 from the hidden layers' halved weights and
+biases), evaluated in float32 exactly as the kernel writes it from the
 constants in that source file, against GELU(x) = x (1 + erf(x / sqrt 2)) / 2.
 Each fused multiply-add is emulated as an exact float64 product and sum
 rounded once to float32 (a float32 product is exact in float64).  This pins
@@ -16,7 +17,7 @@ SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 
 
 def _kernel_constants():
-    body = open(SRC).read().split("float gelu_bf(float x) {", 1)[1].split("\n}", 1)[0]
+    body = open(SRC).read().split("float gelu_half(float y) {", 1)[1].split("\n}", 1)[0]
     first = re.search(r"float e = ([-+0-9.e]+)f;", body).group(1)
     rest = re.findall(r"e = fmaf\(e, w, ([-+0-9.e]+)f\);", body)
     return [np.float32(float(v)) for v in [first] + rest]
@@ -29,11 +30,12 @@ def _fma32(a, b, c):
 
 def _eval(c, x):
     x = np.asarray(x, np.float32)
-    w = np.clip((np.abs(x) * np.float32(1.0 / 4.5)).astype(np.float32), np.float32(0.0), np.float32(1.0))
+    y = (np.float32(0.5) * x).astype(np.float32)   # exact
+    w = np.clip((np.abs(y) * np.float32(1.0 / 2.25)).astype(np.float32), np.float32(0.0), np.float32(1.0))
     e = np.full_like(w, c[0])
     for k in c[1:]:
         e = _fma32(e, w, k)
-    return _fma32(np.abs(x), e, (np.float32(0.5) * x).astype(np.float32))
+    return _fma32(np.abs(y), e, y)
 
 
 def _gelu_kernel(x):
@@ -71,3 +73,19 @@ def test_a_wrong_constant_fails():
     bad[3] = -bad[3]
     g = _eval(bad, x)
     assert np.abs(g - _gelu(x.astype(np.float64))).max() > 1e-3
+
+
+def test_halved_form_is_the_polynomial_on_x():
+    """gelu_half(x/2) with the doubled coefficients equals, value for value,
+    x/2 + |x| e(min(|x|/4.5, 1)) with the halved ones: halving the argument
+    and doubling the coefficients only move every rounding by a power of two."""
+    c = _kernel_constants()
+    half = [np.float32(v) * np.float32(0.5) for v in c]
+    x = np.concatenate([np.linspace(-8.0, 8.0, 40001), np.geomspace(1e-6, 300.0, 2001),
+                        -np.geomspace(1e-6, 300.0, 2001)]).astype(np.float32)
+    w = np.clip((np.abs(x) * np.float32(1.0 / 4.5)).astype(np.float32), np.float32(0.0), np.float32(1.0))
+    e = np.full_like(w, half[0])
+    for k in half[1:]:
+        e = _fma32(e, w, k)
+    direct = _fma32(np.abs(x), e, (np.float32(0.5) * x).astype(np.float32))
+    assert np.array_equal(_gelu_kernel(x), direct)
