@@ -73,6 +73,10 @@ def test_les_prescribed_stress_nonlinear(g0):
 
 
 def test_les_learned_closure_and_gate(g0):
+    """The network inputs of chan_les_features (k_les_feat) against the
+    oracle; then les_update, whose inputs k_les_dyn writes from its LDS tile,
+    against the oracle's stress from the first inputs: a stress within 1e-12
+    with the same fallback count holds the two input paths to each other."""
     cf = chanflow()
     W, b = L.init_params(seed=21)
     net = cf.Mlp(cf.Mlp.flatten(W, b))
